@@ -1,0 +1,288 @@
+"""Python binding of libgbs_hip.so, the MI355X bit-serial GEMM C ABI.
+
+The product interface is the C ABI (include/gbs_hip.h) and the C++ drop-in
+header include/gemmbitserial.hpp.  This module is a thin ctypes layer over
+the same C ABI for tests and benchmarks; it names things after the
+reference's API (BitSerialMatrix, importRegular, gemmBitSerial, sumRows,
+exportRegular: gemmbitserial.hpp:19-479, arch-generic.hpp:331-345).
+
+Loading fails loudly when the library has not been built: there is no CPU
+fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libgbs_hip.so")
+
+# gbs_dtype
+I8, U8, U8_RAW, I16, U16, I32, U32, I64, U64 = range(9)
+_NP_DT = {np.dtype(np.int8): I8, np.dtype(np.uint8): U8,
+          np.dtype(np.int16): I16, np.dtype(np.uint16): U16,
+          np.dtype(np.int32): I32, np.dtype(np.uint32): U32,
+          np.dtype(np.int64): I64, np.dtype(np.uint64): U64}
+# gbs_algo
+AUTO, POPCOUNT, MFMA = 0, 1, 2
+ALGOS = {"auto": AUTO, "popcount": POPCOUNT, "mfma": MFMA}
+
+EXPORTED = [
+    "gbs_init", "gbs_last_error", "gbs_version", "gbs_host_alloc", "gbs_host_free",
+    "gbs_matrix_create", "gbs_matrix_destroy", "gbs_matrix_device_planes",
+    "gbs_matrix_mark_modified",
+    "gbs_import", "gbs_import_quantize", "gbs_planes_from_host", "gbs_planes_to_host",
+    "gbs_export", "gbs_sum_rows", "gbs_workspace_create", "gbs_workspace_destroy",
+    "gbs_gemm", "gbs_gemm_async", "gbs_workspace_reserve", "gbs_gemm_select",
+    "gbs_stream", "gbs_stream_sync", "gbs_event_create", "gbs_event_destroy",
+    "gbs_event_record", "gbs_event_elapsed_ms",
+]
+
+
+class GbsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"gbs error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+_u64, _p, _i = C.c_uint64, C.c_void_p, C.c_int
+
+
+def lib():
+    """The loaded library with argtypes set.  Raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run python -m gemmbitserial_amd.build")
+    L = C.CDLL(LIB_PATH)
+    sig = {
+        "gbs_init": [_i], "gbs_version": [], "gbs_host_alloc": [C.c_size_t, _p],
+        "gbs_host_free": [_p],
+        "gbs_matrix_create": [_u64, _u64, _u64, _i, _u64, _u64, _p],
+        "gbs_matrix_destroy": [_p], "gbs_matrix_device_planes": [_p, _p],
+        "gbs_matrix_mark_modified": [_p],
+        "gbs_import": [_p, _p, _i, _i, _i, _p],
+        "gbs_import_quantize": [_p, _p, _p, _i, _i, _i, _p],
+        "gbs_planes_from_host": [_p, _p], "gbs_planes_to_host": [_p, _p],
+        "gbs_export": [_p, _p, _i], "gbs_sum_rows": [_p, _p, _i],
+        "gbs_workspace_create": [_p], "gbs_workspace_destroy": [_p],
+        "gbs_gemm": [_p, _p, _p, _p, _i, _i], "gbs_gemm_async": [_p, _p, _p, _p, _i, _p],
+        "gbs_workspace_reserve": [_p, _p, _p], "gbs_gemm_select": [_p, _p, _p],
+        "gbs_stream": [_p], "gbs_stream_sync": [_p], "gbs_event_create": [_p],
+        "gbs_event_destroy": [_p], "gbs_event_record": [_p, _p],
+        "gbs_event_elapsed_ms": [_p, _p, _p],
+    }
+    for name, args in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = _i
+    L.gbs_last_error.argtypes = []
+    L.gbs_last_error.restype = C.c_char_p
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != 0:
+        raise GbsError(rc, lib().gbs_last_error().decode())
+    return rc
+
+
+def _ptr(a):
+    return C.c_void_p(a.ctypes.data)
+
+
+def init(device=0):
+    check(lib().gbs_init(device))
+
+
+class DeviceMatrix:
+    """A gbs_matrix: bit-planes resident in HBM, reference layout
+    [nbits][nrows_a][ncols_a/64] (BitSerialMatrix, gemmbitserial.hpp:19-50)."""
+
+    def __init__(self, nbits, nrows, ncols, issigned, nrows_a=None, ncols_a=None):
+        self.nbits, self.nrows, self.ncols = int(nbits), int(nrows), int(ncols)
+        self.issigned = bool(issigned)
+        self.nrows_a = self.nrows if nrows_a is None else int(nrows_a)
+        self.ncols_a = ((self.ncols + 63) // 64) * 64 if ncols_a is None else int(ncols_a)
+        h = C.c_void_p()
+        check(lib().gbs_matrix_create(self.nbits, self.nrows, self.ncols,
+                                      int(self.issigned), self.nrows_a, self.ncols_a,
+                                      C.byref(h)))
+        self.h = h
+
+    @property
+    def words(self):
+        return self.nbits * self.nrows_a * (self.ncols_a // 64)
+
+    def close(self):
+        if self.h:
+            check(lib().gbs_matrix_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def device_planes(self):
+        p = C.c_void_p()
+        check(lib().gbs_matrix_device_planes(self.h, C.byref(p)))
+        return p.value
+
+    def import_regular(self, src, col_major=False, raw_u8=False, want_planes=False):
+        """importRegular on the GPU (gemmbitserial.hpp:212-275); src is
+        nrows x ncols (or ncols x nrows when col_major).  Returns the host
+        copy of the planes when want_planes."""
+        src = np.ascontiguousarray(src)
+        dt = _NP_DT[src.dtype]
+        if raw_u8:
+            assert dt == U8
+            dt = U8_RAW
+        if src.size != self.nrows * self.ncols:
+            raise ValueError("source size does not match the matrix")
+        out = np.zeros(self.words, np.uint64) if want_planes else None
+        check(lib().gbs_import(self.h, _ptr(src), dt, int(col_major), 0,
+                               _ptr(out) if out is not None else None))
+        return out
+
+    def import_quantize(self, src, thresholds, col_major=False, want_planes=False):
+        src = np.ascontiguousarray(src, dtype=np.int32)
+        thr = np.ascontiguousarray(thresholds, dtype=np.int32)
+        out = np.zeros(self.words, np.uint64) if want_planes else None
+        check(lib().gbs_import_quantize(self.h, _ptr(src), _ptr(thr), thr.shape[0],
+                                        int(col_major), 0,
+                                        _ptr(out) if out is not None else None))
+        return out
+
+    def planes(self):
+        out = np.zeros(self.words, np.uint64)
+        check(lib().gbs_planes_to_host(self.h, _ptr(out)))
+        return out
+
+    def set_planes(self, words):
+        w = np.ascontiguousarray(words, dtype=np.uint64)
+        assert w.size == self.words
+        check(lib().gbs_planes_from_host(self.h, _ptr(w)))
+
+    def export_regular(self):
+        out = np.zeros((self.nrows, self.ncols), np.int32)
+        check(lib().gbs_export(self.h, _ptr(out), 0))
+        return out
+
+    def sum_rows(self):
+        out = np.zeros(self.nrows, np.int32)
+        check(lib().gbs_sum_rows(self.h, _ptr(out), 0))
+        return out
+
+
+def from_values(vals, nbits, issigned, raw_u8=False, nrows_a=None, ncols_a=None):
+    vals = np.ascontiguousarray(vals)
+    m = DeviceMatrix(nbits, vals.shape[0], vals.shape[1], issigned, nrows_a, ncols_a)
+    m.import_regular(vals, raw_u8=raw_u8)
+    return m
+
+
+def gemm(lhs, rhs, algo="auto"):
+    """gemmBitSerial: res[n, m] = sum_k lhs[m,k] * rhs[n,k] (N x M int32)."""
+    res = np.zeros((rhs.nrows, lhs.nrows), np.int32)
+    check(lib().gbs_gemm(None, lhs.h, rhs.h, _ptr(res), 0, ALGOS[algo]))
+    return res
+
+
+def select(lhs, rhs):
+    a = C.c_int()
+    check(lib().gbs_gemm_select(lhs.h, rhs.h, C.byref(a)))
+    return {v: k for k, v in ALGOS.items()}[a.value]
+
+
+class DeviceBuffer:
+    """Raw device memory for results that stay in HBM (hipMalloc through
+    libamdhip64, plumbing for timing loops)."""
+
+    _hip = None
+
+    @classmethod
+    def hip(cls):
+        if cls._hip is None:
+            rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+            h = C.CDLL(os.path.join(rocm, "lib", "libamdhip64.so"))
+            h.hipMalloc.argtypes = [_p, C.c_size_t]
+            h.hipFree.argtypes = [_p]
+            h.hipMemcpy.argtypes = [_p, _p, C.c_size_t, _i]
+            h.hipDeviceSynchronize.argtypes = []
+            cls._hip = h
+        return cls._hip
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        rc = self.hip().hipMalloc(C.byref(p), max(self.nbytes, 8))
+        if rc != 0:
+            raise GbsError(2, f"hipMalloc({nbytes}) failed: {rc}")
+        self.ptr = p.value
+
+    def to_host(self, dtype, shape):
+        out = np.zeros(shape, dtype)
+        rc = self.hip().hipMemcpy(_ptr(out), C.c_void_p(self.ptr), out.nbytes, 2)
+        if rc != 0:
+            raise GbsError(3, f"hipMemcpy D2H failed: {rc}")
+        return out
+
+    def close(self):
+        if self.ptr:
+            self.hip().hipFree(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Workspace:
+    def __init__(self):
+        h = C.c_void_p()
+        check(lib().gbs_workspace_create(C.byref(h)))
+        self.h = h
+
+    def reserve(self, lhs, rhs):
+        check(lib().gbs_workspace_reserve(self.h, lhs.h, rhs.h))
+
+    def gemm_async(self, lhs, rhs, d_res_ptr, algo="auto", stream=None):
+        check(lib().gbs_gemm_async(self.h, lhs.h, rhs.h, C.c_void_p(d_res_ptr),
+                                   ALGOS[algo], stream))
+
+    def close(self):
+        if self.h:
+            check(lib().gbs_workspace_destroy(self.h))
+            self.h = None
+
+
+def stream():
+    s = C.c_void_p()
+    check(lib().gbs_stream(C.byref(s)))
+    return s
+
+
+def sync(s=None):
+    check(lib().gbs_stream_sync(s))
+
+
+class Event:
+    def __init__(self):
+        e = C.c_void_p()
+        check(lib().gbs_event_create(C.byref(e)))
+        self.e = e
+
+    def record(self, s=None):
+        check(lib().gbs_event_record(self.e, s))
+
+    def elapsed_ms(self, end):
+        ms = C.c_float()
+        check(lib().gbs_event_elapsed_ms(self.e, end.e, C.byref(ms)))
+        return ms.value

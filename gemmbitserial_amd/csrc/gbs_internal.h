@@ -1,0 +1,95 @@
+// gbs_internal.h -- shared declarations of libgbs_hip.so (not installed).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gbs_hip.h"
+
+// A device-resident bit-serial matrix.  Plane words live in HBM in the
+// reference layout [nbits][nrows_a][ncols_a/64] (gemmbitserial.hpp:44-50),
+// so host mirrors are plain copies.
+struct gbs_matrix {
+  uint64_t nbits, nrows, ncols, nrows_a, ncols_a;
+  bool issigned;
+  int device;
+  uint64_t *d_planes;
+  // int8 copy of the values for the MFMA engine, made from the planes on
+  // first use after each change: [vals_rows][vals_ld], zero padded.
+  int8_t *d_vals = nullptr;
+  uint64_t vals_rows = 0, vals_ld = 0;
+  uint64_t planes_version = 1, vals_version = 0;
+  uint64_t wpr() const { return ncols_a / 64; }
+  uint64_t wpp() const { return nrows_a * wpr(); }
+  uint64_t words() const { return nbits * wpp(); }
+  bool bipolar() const { return nbits == 1 && issigned; }
+  // words holding real columns; the rest of each row is zero padding
+  uint64_t kwords() const { return (ncols + 63) / 64; }
+};
+
+struct gbs_workspace {
+  int device = 0;
+  int32_t *d_res = nullptr;      // result buffer for host-result calls
+  size_t res_cap = 0;            // bytes
+  int32_t *d_rowsum = nullptr;   // bipolar x regular correction
+  size_t rowsum_cap = 0;         // bytes
+  void *d_stage = nullptr;       // staging for host sources (import)
+  size_t stage_cap = 0;          // bytes
+};
+
+namespace gbs {
+
+// error plumbing: record a message for gbs_last_error, return code
+int fail(int code, const char *fmt, ...);
+int hip_fail(hipError_t e, const char *what);
+#define GBS_HIP(call)                                   \
+  do {                                                  \
+    hipError_t e_ = (call);                             \
+    if (e_ != hipSuccess) return ::gbs::hip_fail(e_, #call); \
+  } while (0)
+
+hipStream_t thread_stream();
+int ensure_device();
+int grow(void **p, size_t *cap, size_t need);
+
+// operand view handed to the GEMM launchers
+struct Operand {
+  const uint64_t *planes;
+  uint64_t nrows, wpr, wpp;
+  uint32_t nbits;
+  bool issigned, bipolar;
+};
+inline Operand view(const gbs_matrix *m) {
+  return Operand{m->d_planes, m->nrows, m->wpr(), m->wpp(), (uint32_t)m->nbits,
+                 m->issigned, m->bipolar()};
+}
+
+// kernels (gbs_pack.hip)
+int launch_pack(const gbs_matrix *m, const void *d_src, gbs_dtype dt,
+                bool col_major, hipStream_t s);
+int launch_pack_quantize(const gbs_matrix *m, const int32_t *d_src,
+                         const int32_t *d_thr, int nthres, bool col_major,
+                         hipStream_t s);
+int launch_export(const gbs_matrix *m, int32_t *d_dst, hipStream_t s);
+int launch_sum_rows(const Operand &o, uint64_t kwords, uint64_t ncols,
+                    int32_t *d_dst, hipStream_t s);
+int launch_fill_i32(int32_t *d, uint64_t n, int32_t v, hipStream_t s);
+
+// GEMM engines.  d_rowsum: row sums of the regular operand when exactly one
+// side is bipolar (nullptr otherwise).
+int launch_popc_gemm(const Operand &l, const Operand &r, uint64_t kwords,
+                     uint64_t K, int32_t *d_res, const int32_t *d_rowsum,
+                     hipStream_t s);
+// int8 MFMA engine (gbs_mfma.hip).  ensure_vals makes m->d_vals current
+// (allocating it on first use when may_allocate).
+int ensure_vals(gbs_matrix *m, hipStream_t s, bool may_allocate);
+int launch_mfma_gemm(const gbs_matrix *l, const gbs_matrix *r, int32_t *d_res,
+                     hipStream_t s);
+bool mfma_supported(const Operand &l, const Operand &r, uint64_t K);
+gbs_algo select_algo(const Operand &l, const Operand &r, uint64_t K);
+constexpr uint64_t kValsRowAlign = 256;  // rows of the int8 copy
+constexpr uint64_t kValsColAlign = 256;  // bytes per row of the int8 copy
+
+size_t dtype_size(gbs_dtype dt);
+
+}  // namespace gbs

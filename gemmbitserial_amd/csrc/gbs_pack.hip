@@ -1,0 +1,302 @@
+// gbs_pack.hip -- regular <-> bit-serial conversion kernels for gfx950.
+//
+//   pack (row-major)  one wave per 64 consecutive plane words; for each word
+//                     the 64 lanes load the 64 columns it covers (a coalesced
+//                     row segment), quantize, and one __ballot per plane
+//                     turns "bit b of my code" into that plane word.  Lane t
+//                     keeps word t, so each plane is written back as one
+//                     coalesced 512-byte store.
+//   pack (col-major)  one wave per 64 rows x 1 word; lane = row, so loads of
+//                     src[c*nrows + r] are coalesced; each lane shifts its
+//                     64 bits into its own words.
+//   export / sum_rows the inverse and the row-wise sums.
+//
+// Semantics follow gemmbitserial.hpp:169-186 (quantize), 212-275
+// (importRegular and its raw uint8_t* overload), 281-308
+// (importRegularAndQuantize), 312-333 (exportRegular) and
+// arch-generic.hpp:6-37 (sumRows).  For integral element types the code's
+// low nbits equal the element modulo 2^nbits (the signed branch of quantize
+// adds 2^nbits before truncating to uint8), bipolar takes "element > 0",
+// and the raw overload takes the element's low bits as they are.
+#include "gbs_internal.h"
+
+namespace gbs {
+namespace {
+
+constexpr int kWave = 64;
+
+// --- element -> code ---------------------------------------------------
+template <typename T, bool RAW>
+struct CodeOf {
+  __device__ static inline uint32_t get(T e, bool bipolar) {
+    if (RAW) return (uint32_t)(uint8_t)e;
+    if (bipolar) return e > (T)0 ? 1u : 0u;
+    return (uint32_t)(uint8_t)e;  // low 8 bits, two's complement
+  }
+};
+
+// Source functors: code of real element (r, c).
+template <typename T, bool RAW>
+struct RowMajorSrc {
+  const T *p;
+  uint64_t nrows, ncols;
+  bool bipolar;
+  __device__ inline uint32_t code(uint64_t r, uint64_t c) const {
+    return CodeOf<T, RAW>::get(p[r * ncols + c], bipolar);
+  }
+};
+
+template <typename T, bool RAW>
+struct ColMajorSrc {
+  const T *p;
+  uint64_t nrows, ncols;
+  bool bipolar;
+  __device__ inline uint32_t code(uint64_t r, uint64_t c) const {
+    return CodeOf<T, RAW>::get(p[c * nrows + r], bipolar);
+  }
+};
+
+// importRegularAndQuantize: index of the first threshold >= e (ascending
+// per row), nthres when e exceeds all of them.
+struct ThresholdSrc {
+  const int32_t *p, *thr;
+  uint64_t nrows, ncols;
+  int nthres;
+  bool col_major;
+  __device__ inline uint32_t code(uint64_t r, uint64_t c) const {
+    int32_t e = col_major ? p[c * nrows + r] : p[r * ncols + c];
+    uint32_t q = (uint32_t)nthres;
+    for (int t = 0; t < nthres; t++) {
+      if (e <= thr[(uint64_t)t * nrows + r]) { q = (uint32_t)t; break; }
+    }
+    return q & 0xffu;
+  }
+};
+
+// --- row-major pack: ballot per plane word -----------------------------
+// Each wave owns flattened words f0 .. f0+63 of every plane (f = r*wpr + w).
+template <typename Src>
+__global__ __launch_bounds__(256) void pack_rows_kernel(
+    Src src, uint64_t *planes, uint64_t nrows, uint64_t ncols, uint64_t wpr,
+    uint64_t W, uint32_t nbits) {
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  const uint64_t wave = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  const uint64_t f0 = wave * kWave;
+  if (f0 >= W) return;
+  uint64_t r = f0 / wpr, w = f0 - r * wpr;
+  const uint32_t tmax = (uint32_t)((W - f0) < kWave ? (W - f0) : kWave);
+  uint64_t out[8];
+#pragma unroll
+  for (int b = 0; b < 8; b++) out[b] = 0;
+  for (uint32_t t = 0; t < tmax; t++) {
+    const uint64_t c = w * kWave + lane;
+    uint32_t code = 0;
+    if (r < nrows && c < ncols) code = src.code(r, c);
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      if ((uint32_t)b < nbits) {
+        uint64_t m = __ballot((code >> b) & 1u);
+        out[b] = (lane == t) ? m : out[b];
+      }
+    }
+    if (++w == wpr) { w = 0; ++r; }
+  }
+  if (lane < tmax) {
+#pragma unroll
+    for (int b = 0; b < 8; b++)
+      if ((uint32_t)b < nbits) planes[(uint64_t)b * W + f0 + lane] = out[b];
+  }
+}
+
+// --- column-major pack: lane = row, each lane builds its own words ------
+template <typename Src>
+__global__ __launch_bounds__(256) void pack_cols_kernel(
+    Src src, uint64_t *planes, uint64_t nrows, uint64_t ncols, uint64_t nrows_a,
+    uint64_t wpr, uint64_t W, uint32_t nbits) {
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  const uint64_t wave = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  const uint64_t row_groups = (nrows_a + kWave - 1) / kWave;
+  if (wave >= row_groups * wpr) return;
+  const uint64_t w = wave / row_groups;
+  const uint64_t r = (wave - w * row_groups) * kWave + lane;
+  uint64_t out[8];
+#pragma unroll
+  for (int b = 0; b < 8; b++) out[b] = 0;
+  if (r < nrows) {
+    const uint64_t c0 = w * kWave;
+    const uint32_t jmax = c0 >= ncols ? 0u
+                          : (uint32_t)((ncols - c0) < kWave ? (ncols - c0) : kWave);
+    for (uint32_t j = 0; j < jmax; j++) {
+      uint32_t code = src.code(r, c0 + j);
+#pragma unroll
+      for (int b = 0; b < 8; b++)
+        if ((uint32_t)b < nbits) out[b] |= (uint64_t)((code >> b) & 1u) << j;
+    }
+  }
+  if (r < nrows_a) {
+#pragma unroll
+    for (int b = 0; b < 8; b++)
+      if ((uint32_t)b < nbits) planes[(uint64_t)b * W + r * wpr + w] = out[b];
+  }
+}
+
+template <typename T, bool RAW>
+int pack_typed(const gbs_matrix *m, const void *d_src, bool col_major,
+               hipStream_t s) {
+  const uint64_t W = m->wpp();
+  const uint32_t nb = (uint32_t)m->nbits;
+  if (W == 0) return GBS_OK;
+  if (!col_major) {
+    RowMajorSrc<T, RAW> src{(const T *)d_src, m->nrows, m->ncols, m->bipolar()};
+    const uint64_t waves = (W + kWave - 1) / kWave;
+    const uint64_t blocks = (waves + 3) / 4;
+    hipLaunchKernelGGL((pack_rows_kernel<RowMajorSrc<T, RAW>>), dim3((uint32_t)blocks),
+                       dim3(256), 0, s, src, m->d_planes, m->nrows, m->ncols,
+                       m->wpr(), W, nb);
+  } else {
+    ColMajorSrc<T, RAW> src{(const T *)d_src, m->nrows, m->ncols, m->bipolar()};
+    const uint64_t waves = (m->nrows_a + kWave - 1) / kWave * m->wpr();
+    const uint64_t blocks = (waves + 3) / 4;
+    hipLaunchKernelGGL((pack_cols_kernel<ColMajorSrc<T, RAW>>), dim3((uint32_t)blocks),
+                       dim3(256), 0, s, src, m->d_planes, m->nrows, m->ncols,
+                       m->nrows_a, m->wpr(), W, nb);
+  }
+  GBS_HIP(hipGetLastError());
+  return GBS_OK;
+}
+
+// --- export ---------------------------------------------------------------
+__global__ __launch_bounds__(256) void export_kernel(
+    const uint64_t *planes, uint64_t nrows, uint64_t ncols, uint64_t wpr,
+    uint64_t W, uint32_t nbits, bool issigned, int32_t *out) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrows * ncols) return;
+  const uint64_t r = i / ncols, c = i - r * ncols;
+  const uint64_t off = r * wpr + (c >> 6);
+  const uint32_t sh = (uint32_t)(c & 63);
+  int32_t v;
+  if (nbits == 1 && issigned) {
+    v = ((planes[off] >> sh) & 1) ? 1 : -1;
+  } else {
+    uint32_t acc = 0;
+    for (uint32_t b = 0; b < nbits; b++) {
+      uint32_t bit = (uint32_t)((planes[(uint64_t)b * W + off] >> sh) & 1);
+      uint32_t t = bit << b;
+      acc += (issigned && b == nbits - 1) ? 0u - t : t;
+    }
+    v = (int32_t)acc;
+  }
+  out[i] = v;
+}
+
+// --- row sums: one wave per row --------------------------------------------
+__global__ __launch_bounds__(256) void sum_rows_kernel(
+    const uint64_t *planes, uint64_t nrows, uint64_t wpr, uint64_t W,
+    uint64_t kwords, uint32_t nbits, bool issigned, uint32_t ncols,
+    int32_t *out) {
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  const uint64_t r = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  if (r >= nrows) return;
+  uint32_t acc = 0;
+  const bool bip = nbits == 1 && issigned;
+  for (uint32_t b = 0; b < nbits; b++) {
+    uint32_t pc = 0;
+    const uint64_t *row = planes + (uint64_t)b * W + r * wpr;
+    for (uint64_t k = lane; k < kwords; k += kWave) pc += (uint32_t)__popcll(row[k]);
+    if (bip) {
+      acc = pc;
+    } else {
+      uint32_t t = pc << b;
+      acc += (issigned && b == nbits - 1) ? 0u - t : t;
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += (uint32_t)__shfl_xor((int)acc, off, kWave);
+  if (lane == 0) out[r] = bip ? (int32_t)(2u * acc - ncols) : (int32_t)acc;
+}
+
+__global__ void fill_i32_kernel(int32_t *d, uint64_t n, int32_t v) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t j = i; j < n; j += stride) d[j] = v;
+}
+
+}  // namespace
+
+size_t dtype_size(gbs_dtype dt) {
+  switch (dt) {
+    case GBS_I8: case GBS_U8: case GBS_U8_RAW: return 1;
+    case GBS_I16: case GBS_U16: return 2;
+    case GBS_I32: case GBS_U32: return 4;
+    case GBS_I64: case GBS_U64: return 8;
+  }
+  return 0;
+}
+
+int launch_pack(const gbs_matrix *m, const void *d_src, gbs_dtype dt,
+                bool col_major, hipStream_t s) {
+  switch (dt) {
+    case GBS_I8: return pack_typed<int8_t, false>(m, d_src, col_major, s);
+    case GBS_U8: return pack_typed<uint8_t, false>(m, d_src, col_major, s);
+    case GBS_U8_RAW: return pack_typed<uint8_t, true>(m, d_src, col_major, s);
+    case GBS_I16: return pack_typed<int16_t, false>(m, d_src, col_major, s);
+    case GBS_U16: return pack_typed<uint16_t, false>(m, d_src, col_major, s);
+    case GBS_I32: return pack_typed<int32_t, false>(m, d_src, col_major, s);
+    case GBS_U32: return pack_typed<uint32_t, false>(m, d_src, col_major, s);
+    case GBS_I64: return pack_typed<int64_t, false>(m, d_src, col_major, s);
+    case GBS_U64: return pack_typed<uint64_t, false>(m, d_src, col_major, s);
+  }
+  return fail(GBS_EINVAL, "gbs_import: unknown dtype %d", (int)dt);
+}
+
+int launch_pack_quantize(const gbs_matrix *m, const int32_t *d_src,
+                         const int32_t *d_thr, int nthres, bool col_major,
+                         hipStream_t s) {
+  const uint64_t W = m->wpp();
+  if (W == 0) return GBS_OK;
+  ThresholdSrc src{d_src, d_thr, m->nrows, m->ncols, nthres, col_major};
+  if (!col_major) {
+    const uint64_t blocks = ((W + kWave - 1) / kWave + 3) / 4;
+    hipLaunchKernelGGL((pack_rows_kernel<ThresholdSrc>), dim3((uint32_t)blocks), dim3(256),
+                       0, s, src, m->d_planes, m->nrows, m->ncols, m->wpr(), W,
+                       (uint32_t)m->nbits);
+  } else {
+    const uint64_t waves = (m->nrows_a + kWave - 1) / kWave * m->wpr();
+    hipLaunchKernelGGL((pack_cols_kernel<ThresholdSrc>), dim3((uint32_t)((waves + 3) / 4)),
+                       dim3(256), 0, s, src, m->d_planes, m->nrows, m->ncols,
+                       m->nrows_a, m->wpr(), W, (uint32_t)m->nbits);
+  }
+  GBS_HIP(hipGetLastError());
+  return GBS_OK;
+}
+
+int launch_export(const gbs_matrix *m, int32_t *d_dst, hipStream_t s) {
+  const uint64_t n = m->nrows * m->ncols;
+  if (n == 0) return GBS_OK;
+  hipLaunchKernelGGL(export_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s,
+                     m->d_planes, m->nrows, m->ncols, m->wpr(), m->wpp(),
+                     (uint32_t)m->nbits, m->issigned, d_dst);
+  GBS_HIP(hipGetLastError());
+  return GBS_OK;
+}
+
+int launch_sum_rows(const Operand &o, uint64_t kwords, uint64_t ncols,
+                    int32_t *d_dst, hipStream_t s) {
+  if (o.nrows == 0) return GBS_OK;
+  hipLaunchKernelGGL(sum_rows_kernel, dim3((uint32_t)((o.nrows + 3) / 4)), dim3(256), 0,
+                     s, o.planes, o.nrows, o.wpr, o.wpp, kwords, o.nbits,
+                     o.issigned, (uint32_t)ncols, d_dst);
+  GBS_HIP(hipGetLastError());
+  return GBS_OK;
+}
+
+int launch_fill_i32(int32_t *d, uint64_t n, int32_t v, hipStream_t s) {
+  if (n == 0) return GBS_OK;
+  uint64_t blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(fill_i32_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, d, n, v);
+  GBS_HIP(hipGetLastError());
+  return GBS_OK;
+}
+
+}  // namespace gbs

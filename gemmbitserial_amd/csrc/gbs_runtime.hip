@@ -1,0 +1,492 @@
+// gbs_runtime.hip -- the C ABI of libgbs_hip.so (include/gbs_hip.h):
+// device selection, per-thread streams, matrices, workspaces and the
+// synchronous / asynchronous GEMM entry points.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <mutex>
+#include <string>
+
+#include "gbs_internal.h"
+
+namespace gbs {
+
+namespace {
+thread_local std::string t_err;
+thread_local int t_device = -1;
+thread_local hipStream_t t_stream = nullptr;
+thread_local int t_stream_device = -1;
+thread_local gbs_workspace *t_ws = nullptr;
+}  // namespace
+
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  t_err = buf;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+  return fail(e == hipErrorOutOfMemory ? GBS_ENOMEM : GBS_EDEVICE, "%s: %s", what,
+              hipGetErrorString(e));
+}
+
+int ensure_device() {
+  if (t_device < 0) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n == 0)
+      return fail(GBS_EDEVICE, "no HIP device available (%s)", hipGetErrorString(e));
+    t_device = 0;
+  }
+  GBS_HIP(hipSetDevice(t_device));
+  return GBS_OK;
+}
+
+hipStream_t thread_stream() {
+  if (t_stream == nullptr || t_stream_device != t_device) {
+    hipStream_t s = nullptr;
+    // default priority, non-blocking w.r.t. the null stream
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return nullptr;
+    t_stream = s;
+    t_stream_device = t_device;
+  }
+  return t_stream;
+}
+
+int grow(void **p, size_t *cap, size_t need) {
+  if (*cap >= need && *p) return GBS_OK;
+  if (*p) {
+    GBS_HIP(hipDeviceSynchronize());
+    GBS_HIP(hipFree(*p));
+    *p = nullptr;
+    *cap = 0;
+  }
+  size_t sz = need < 256 ? 256 : need;
+  GBS_HIP(hipMalloc(p, sz));
+  *cap = sz;
+  return GBS_OK;
+}
+
+static gbs_workspace *thread_ws() {
+  if (!t_ws) t_ws = new gbs_workspace();
+  return t_ws;
+}
+
+static int check_matrix(const gbs_matrix *m, const char *fn) {
+  if (!m) return fail(GBS_EINVAL, "%s: null matrix", fn);
+  return GBS_OK;
+}
+
+// Shared body of gbs_gemm / gbs_gemm_async.
+static int gemm_enqueue(gbs_workspace *ws, const gbs_matrix *l, const gbs_matrix *r,
+                        int32_t *d_res, gbs_algo algo, hipStream_t s,
+                        bool may_allocate) {
+  if (l->ncols != r->ncols)
+    return fail(GBS_EINVAL, "gemm: lhs has %llu columns, rhs %llu (depths must match)",
+                (unsigned long long)l->ncols, (unsigned long long)r->ncols);
+  Operand lo = view(l), ro = view(r);
+  const uint64_t K = l->ncols;
+  const uint64_t kwords = (K + 63) / 64;
+  gbs_algo eng = algo == GBS_ALGO_AUTO ? select_algo(lo, ro, K) : algo;
+  if (eng == GBS_ALGO_MFMA) {
+    if (!mfma_supported(lo, ro, K))
+      return fail(GBS_ENOTSUP,
+                  "gemm: the int8 MFMA engine cannot hold this product exactly "
+                  "(bitwidths %llu x %llu, depth %llu); use the popcount engine",
+                  (unsigned long long)l->nbits, (unsigned long long)r->nbits,
+                  (unsigned long long)K);
+    int rc = ensure_vals(const_cast<gbs_matrix *>(l), s, may_allocate);
+    if (!rc) rc = ensure_vals(const_cast<gbs_matrix *>(r), s, may_allocate);
+    if (rc) return rc;
+    return launch_mfma_gemm(l, r, d_res, s);
+  }
+  if (eng != GBS_ALGO_POPCOUNT) return fail(GBS_EINVAL, "gemm: unknown algo %d", (int)algo);
+  const int32_t *d_rowsum = nullptr;
+  if (lo.bipolar != ro.bipolar && l->nrows && r->nrows && kwords) {
+    const Operand &reg = lo.bipolar ? ro : lo;
+    const size_t need = reg.nrows * sizeof(int32_t);
+    if (ws->rowsum_cap < need) {
+      if (!may_allocate)
+        return fail(GBS_ERANGE, "gemm_async: workspace too small, call gbs_workspace_reserve");
+      int rc = grow((void **)&ws->d_rowsum, &ws->rowsum_cap, need);
+      if (rc) return rc;
+    }
+    int rc = launch_sum_rows(reg, kwords, K, ws->d_rowsum, s);
+    if (rc) return rc;
+    d_rowsum = ws->d_rowsum;
+  }
+  return launch_popc_gemm(lo, ro, kwords, K, d_res, d_rowsum, s);
+}
+
+}  // namespace gbs
+
+using namespace gbs;
+
+extern "C" {
+
+int gbs_init(int device) {
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n == 0)
+    return fail(GBS_EDEVICE, "gbs_init: no HIP device (%s)", hipGetErrorString(e));
+  if (device < 0 || device >= n)
+    return fail(GBS_EINVAL, "gbs_init: device %d out of range [0, %d)", device, n);
+  t_device = device;
+  GBS_HIP(hipSetDevice(device));
+  if (!thread_stream()) return fail(GBS_EDEVICE, "gbs_init: stream creation failed");
+  return GBS_OK;
+}
+
+const char *gbs_last_error(void) { return t_err.c_str(); }
+
+int gbs_version(void) { return 10000; }
+
+int gbs_host_alloc(size_t bytes, void **out) {
+  if (!out) return fail(GBS_EINVAL, "gbs_host_alloc: null out");
+  *out = nullptr;
+  int rc = ensure_device();
+  if (rc) return rc;
+  GBS_HIP(hipHostMalloc(out, bytes ? bytes : 8, hipHostMallocDefault));
+  return GBS_OK;
+}
+
+int gbs_host_free(void *p) {
+  if (p) GBS_HIP(hipHostFree(p));
+  return GBS_OK;
+}
+
+int gbs_matrix_create(uint64_t nbits, uint64_t nrows, uint64_t ncols, int issigned,
+                      uint64_t nrows_a, uint64_t ncols_a, gbs_matrix **out) {
+  if (!out) return fail(GBS_EINVAL, "gbs_matrix_create: null out");
+  *out = nullptr;
+  if (nbits < 1 || nbits > 8)
+    return fail(GBS_EINVAL, "gbs_matrix_create: nbits %llu outside [1, 8]",
+                (unsigned long long)nbits);
+  if (ncols_a % 64 || nrows_a < nrows || ncols_a < ncols)
+    return fail(GBS_EINVAL,
+                "gbs_matrix_create: allocation %llu x %llu cannot hold %llu x %llu "
+                "(ncols_a must be a multiple of 64)",
+                (unsigned long long)nrows_a, (unsigned long long)ncols_a,
+                (unsigned long long)nrows, (unsigned long long)ncols);
+  int rc = ensure_device();
+  if (rc) return rc;
+  gbs_matrix *m = new gbs_matrix();
+  m->nbits = nbits;
+  m->nrows = nrows;
+  m->ncols = ncols;
+  m->nrows_a = nrows_a;
+  m->ncols_a = ncols_a;
+  m->issigned = issigned != 0;
+  m->device = t_device;
+  m->d_planes = nullptr;
+  const size_t bytes = m->words() * 8;
+  hipError_t e = hipMalloc((void **)&m->d_planes, bytes ? bytes : 8);
+  if (e != hipSuccess) {
+    delete m;
+    return hip_fail(e, "gbs_matrix_create: hipMalloc");
+  }
+  e = hipMemsetAsync(m->d_planes, 0, bytes ? bytes : 8, thread_stream());
+  if (e == hipSuccess) e = hipStreamSynchronize(thread_stream());
+  if (e != hipSuccess) {
+    (void)hipFree(m->d_planes);
+    delete m;
+    return hip_fail(e, "gbs_matrix_create: clear");
+  }
+  *out = m;
+  return GBS_OK;
+}
+
+int gbs_matrix_destroy(gbs_matrix *m) {
+  if (!m) return GBS_OK;
+  int rc = ensure_device();
+  if (rc) return rc;
+  GBS_HIP(hipStreamSynchronize(thread_stream()));
+  GBS_HIP(hipFree(m->d_planes));
+  if (m->d_vals) GBS_HIP(hipFree(m->d_vals));
+  delete m;
+  return GBS_OK;
+}
+
+int gbs_matrix_device_planes(const gbs_matrix *m, uint64_t **d_planes) {
+  int rc = check_matrix(m, "gbs_matrix_device_planes");
+  if (rc) return rc;
+  if (!d_planes) return fail(GBS_EINVAL, "gbs_matrix_device_planes: null out");
+  *d_planes = m->d_planes;
+  return GBS_OK;
+}
+
+// Stage a host source in the thread workspace; returns the device pointer.
+static int stage_host(const void *src, size_t bytes, const void **d_src, hipStream_t s) {
+  gbs_workspace *ws = thread_ws();
+  int rc = grow(&ws->d_stage, &ws->stage_cap, bytes);
+  if (rc) return rc;
+  GBS_HIP(hipMemcpyAsync(ws->d_stage, src, bytes, hipMemcpyHostToDevice, s));
+  *d_src = ws->d_stage;
+  return GBS_OK;
+}
+
+int gbs_import(gbs_matrix *m, const void *src, gbs_dtype dt, int read_col_major,
+               int src_on_device, uint64_t *host_planes) {
+  int rc = check_matrix(m, "gbs_import");
+  if (rc) return rc;
+  const size_t es = dtype_size(dt);
+  if (es == 0) return fail(GBS_EINVAL, "gbs_import: unknown dtype %d", (int)dt);
+  const size_t bytes = (size_t)(m->nrows * m->ncols) * es;
+  if (!src && bytes) return fail(GBS_EINVAL, "gbs_import: null source");
+  if ((rc = ensure_device())) return rc;
+  hipStream_t s = thread_stream();
+  const void *d_src = src;
+  if (!src_on_device && bytes) {
+    if ((rc = stage_host(src, bytes, &d_src, s))) return rc;
+  }
+  if ((rc = launch_pack(m, d_src, dt, read_col_major != 0, s))) return rc;
+  m->planes_version++;
+  if (host_planes && m->words())
+    GBS_HIP(hipMemcpyAsync(host_planes, m->d_planes, m->words() * 8,
+                           hipMemcpyDeviceToHost, s));
+  GBS_HIP(hipStreamSynchronize(s));
+  return GBS_OK;
+}
+
+int gbs_import_quantize(gbs_matrix *m, const int32_t *src, const int32_t *thresholds,
+                        int nthres, int read_col_major, int src_on_device,
+                        uint64_t *host_planes) {
+  int rc = check_matrix(m, "gbs_import_quantize");
+  if (rc) return rc;
+  if (m->issigned)
+    return fail(GBS_EINVAL, "gbs_import_quantize: threshold quantization needs an "
+                            "unsigned matrix (gemmbitserial.hpp:283)");
+  if (nthres < 0) return fail(GBS_EINVAL, "gbs_import_quantize: nthres < 0");
+  const size_t bytes = (size_t)(m->nrows * m->ncols) * 4;
+  const size_t tbytes = (size_t)nthres * m->nrows * 4;
+  if ((bytes && !src) || (tbytes && !thresholds))
+    return fail(GBS_EINVAL, "gbs_import_quantize: null source or thresholds");
+  if ((rc = ensure_device())) return rc;
+  hipStream_t s = thread_stream();
+  const int32_t *d_src = src, *d_thr = thresholds;
+  void *tmp = nullptr;
+  if (!src_on_device && (bytes || tbytes)) {
+    GBS_HIP(hipMalloc(&tmp, bytes + tbytes + 8));
+    if (bytes) GBS_HIP(hipMemcpyAsync(tmp, src, bytes, hipMemcpyHostToDevice, s));
+    if (tbytes)
+      GBS_HIP(hipMemcpyAsync((char *)tmp + bytes, thresholds, tbytes,
+                             hipMemcpyHostToDevice, s));
+    d_src = (const int32_t *)tmp;
+    d_thr = (const int32_t *)((char *)tmp + bytes);
+  }
+  rc = launch_pack_quantize(m, d_src, d_thr, nthres, read_col_major != 0, s);
+  m->planes_version++;
+  if (!rc && host_planes && m->words()) {
+    hipError_t e = hipMemcpyAsync(host_planes, m->d_planes, m->words() * 8,
+                                  hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) rc = hip_fail(e, "gbs_import_quantize: copy back");
+  }
+  hipError_t e = hipStreamSynchronize(s);
+  if (tmp) (void)hipFree(tmp);
+  if (rc) return rc;
+  if (e != hipSuccess) return hip_fail(e, "gbs_import_quantize");
+  return GBS_OK;
+}
+
+int gbs_planes_from_host(gbs_matrix *m, const uint64_t *host_planes) {
+  int rc = check_matrix(m, "gbs_planes_from_host");
+  if (rc) return rc;
+  if (!m->words()) return GBS_OK;
+  if (!host_planes) return fail(GBS_EINVAL, "gbs_planes_from_host: null source");
+  if ((rc = ensure_device())) return rc;
+  hipStream_t s = thread_stream();
+  GBS_HIP(hipMemcpyAsync(m->d_planes, host_planes, m->words() * 8,
+                         hipMemcpyHostToDevice, s));
+  GBS_HIP(hipStreamSynchronize(s));
+  m->planes_version++;
+  return GBS_OK;
+}
+
+int gbs_matrix_mark_modified(gbs_matrix *m) {
+  int rc = check_matrix(m, "gbs_matrix_mark_modified");
+  if (rc) return rc;
+  m->planes_version++;
+  return GBS_OK;
+}
+
+int gbs_planes_to_host(const gbs_matrix *m, uint64_t *host_planes) {
+  int rc = check_matrix(m, "gbs_planes_to_host");
+  if (rc) return rc;
+  if (!m->words()) return GBS_OK;
+  if (!host_planes) return fail(GBS_EINVAL, "gbs_planes_to_host: null destination");
+  if ((rc = ensure_device())) return rc;
+  hipStream_t s = thread_stream();
+  GBS_HIP(hipMemcpyAsync(host_planes, m->d_planes, m->words() * 8,
+                         hipMemcpyDeviceToHost, s));
+  GBS_HIP(hipStreamSynchronize(s));
+  return GBS_OK;
+}
+
+// Run a kernel that writes n int32 to dst (device or host).
+}  // extern "C"
+template <typename F>
+static int to_i32(size_t n, int32_t *dst, int dst_on_device, F &&launch) {
+  if (n == 0) return GBS_OK;
+  if (!dst) return fail(GBS_EINVAL, "null destination");
+  int rc = ensure_device();
+  if (rc) return rc;
+  hipStream_t s = thread_stream();
+  int32_t *d = dst;
+  gbs_workspace *ws = thread_ws();
+  if (!dst_on_device) {
+    if ((rc = grow((void **)&ws->d_res, &ws->res_cap, n * 4))) return rc;
+    d = ws->d_res;
+  }
+  if ((rc = launch(d, s))) return rc;
+  if (!dst_on_device) GBS_HIP(hipMemcpyAsync(dst, d, n * 4, hipMemcpyDeviceToHost, s));
+  GBS_HIP(hipStreamSynchronize(s));
+  return GBS_OK;
+}
+extern "C" {
+
+int gbs_export(const gbs_matrix *m, int32_t *dst, int dst_on_device) {
+  int rc = check_matrix(m, "gbs_export");
+  if (rc) return rc;
+  return to_i32(m->nrows * m->ncols, dst, dst_on_device,
+                [&](int32_t *d, hipStream_t s) { return launch_export(m, d, s); });
+}
+
+int gbs_sum_rows(const gbs_matrix *m, int32_t *dst, int dst_on_device) {
+  int rc = check_matrix(m, "gbs_sum_rows");
+  if (rc) return rc;
+  return to_i32(m->nrows, dst, dst_on_device, [&](int32_t *d, hipStream_t s) {
+    return launch_sum_rows(view(m), m->kwords(), m->ncols, d, s);
+  });
+}
+
+int gbs_workspace_create(gbs_workspace **out) {
+  if (!out) return fail(GBS_EINVAL, "gbs_workspace_create: null out");
+  int rc = ensure_device();
+  if (rc) return rc;
+  *out = new gbs_workspace();
+  (*out)->device = t_device;
+  return GBS_OK;
+}
+
+int gbs_workspace_destroy(gbs_workspace *ws) {
+  if (!ws) return GBS_OK;
+  int rc = ensure_device();
+  if (rc) return rc;
+  GBS_HIP(hipDeviceSynchronize());
+  if (ws->d_res) GBS_HIP(hipFree(ws->d_res));
+  if (ws->d_rowsum) GBS_HIP(hipFree(ws->d_rowsum));
+  if (ws->d_stage) GBS_HIP(hipFree(ws->d_stage));
+  delete ws;
+  return GBS_OK;
+}
+
+int gbs_workspace_reserve(gbs_workspace *ws, const gbs_matrix *lhs, const gbs_matrix *rhs) {
+  if (!ws) ws = thread_ws();
+  int rc = check_matrix(lhs, "gbs_workspace_reserve");
+  if (!rc) rc = check_matrix(rhs, "gbs_workspace_reserve");
+  if (rc) return rc;
+  if ((rc = ensure_device())) return rc;
+  const size_t rows = lhs->nrows > rhs->nrows ? lhs->nrows : rhs->nrows;
+  if ((rc = grow((void **)&ws->d_rowsum, &ws->rowsum_cap, rows * 4))) return rc;
+  if (lhs->ncols == rhs->ncols && mfma_supported(view(lhs), view(rhs), lhs->ncols)) {
+    hipStream_t s = thread_stream();
+    if ((rc = ensure_vals(const_cast<gbs_matrix *>(lhs), s, true))) return rc;
+    if ((rc = ensure_vals(const_cast<gbs_matrix *>(rhs), s, true))) return rc;
+    GBS_HIP(hipStreamSynchronize(s));
+  }
+  return GBS_OK;
+}
+
+int gbs_gemm(gbs_workspace *ws, const gbs_matrix *lhs, const gbs_matrix *rhs,
+             int32_t *res, int res_on_device, gbs_algo algo) {
+  int rc = check_matrix(lhs, "gbs_gemm");
+  if (!rc) rc = check_matrix(rhs, "gbs_gemm");
+  if (rc) return rc;
+  if (!ws) ws = thread_ws();
+  const size_t n = (size_t)(lhs->nrows * rhs->nrows);
+  if (n && !res) return fail(GBS_EINVAL, "gbs_gemm: null result pointer");
+  if ((rc = ensure_device())) return rc;
+  hipStream_t s = thread_stream();
+  int32_t *d = res;
+  if (!res_on_device && n) {
+    if ((rc = grow((void **)&ws->d_res, &ws->res_cap, n * 4))) return rc;
+    d = ws->d_res;
+  }
+  if ((rc = gemm_enqueue(ws, lhs, rhs, d, algo, s, true))) return rc;
+  if (!res_on_device && n) GBS_HIP(hipMemcpyAsync(res, d, n * 4, hipMemcpyDeviceToHost, s));
+  GBS_HIP(hipStreamSynchronize(s));
+  return GBS_OK;
+}
+
+int gbs_gemm_async(gbs_workspace *ws, const gbs_matrix *lhs, const gbs_matrix *rhs,
+                   int32_t *d_res, gbs_algo algo, void *stream) {
+  int rc = check_matrix(lhs, "gbs_gemm_async");
+  if (!rc) rc = check_matrix(rhs, "gbs_gemm_async");
+  if (rc) return rc;
+  if (!ws) ws = thread_ws();
+  if (lhs->nrows * rhs->nrows && !d_res)
+    return fail(GBS_EINVAL, "gbs_gemm_async: null result pointer");
+  if ((rc = ensure_device())) return rc;
+  hipStream_t s = stream ? (hipStream_t)stream : thread_stream();
+  return gemm_enqueue(ws, lhs, rhs, d_res, algo, s, false);
+}
+
+int gbs_gemm_select(const gbs_matrix *lhs, const gbs_matrix *rhs, gbs_algo *chosen) {
+  int rc = check_matrix(lhs, "gbs_gemm_select");
+  if (!rc) rc = check_matrix(rhs, "gbs_gemm_select");
+  if (rc) return rc;
+  if (!chosen) return fail(GBS_EINVAL, "gbs_gemm_select: null out");
+  *chosen = select_algo(view(lhs), view(rhs), lhs->ncols);
+  return GBS_OK;
+}
+
+int gbs_stream(void **stream) {
+  if (!stream) return fail(GBS_EINVAL, "gbs_stream: null out");
+  int rc = ensure_device();
+  if (rc) return rc;
+  *stream = (void *)thread_stream();
+  return *stream ? GBS_OK : fail(GBS_EDEVICE, "gbs_stream: creation failed");
+}
+
+int gbs_stream_sync(void *stream) {
+  int rc = ensure_device();
+  if (rc) return rc;
+  GBS_HIP(hipStreamSynchronize(stream ? (hipStream_t)stream : thread_stream()));
+  return GBS_OK;
+}
+
+int gbs_event_create(void **ev) {
+  if (!ev) return fail(GBS_EINVAL, "gbs_event_create: null out");
+  int rc = ensure_device();
+  if (rc) return rc;
+  hipEvent_t e;
+  GBS_HIP(hipEventCreate(&e));
+  *ev = (void *)e;
+  return GBS_OK;
+}
+
+int gbs_event_destroy(void *ev) {
+  if (ev) GBS_HIP(hipEventDestroy((hipEvent_t)ev));
+  return GBS_OK;
+}
+
+int gbs_event_record(void *ev, void *stream) {
+  int rc = ensure_device();
+  if (rc) return rc;
+  GBS_HIP(hipEventRecord((hipEvent_t)ev, stream ? (hipStream_t)stream : thread_stream()));
+  return GBS_OK;
+}
+
+int gbs_event_elapsed_ms(void *start, void *stop, float *ms) {
+  if (!ms) return fail(GBS_EINVAL, "gbs_event_elapsed_ms: null out");
+  GBS_HIP(hipEventSynchronize((hipEvent_t)stop));
+  GBS_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+  return GBS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,197 @@
+/*
+ * gbs_hip.h -- C ABI of libgbs_hip.so, the MI355X (gfx950) bit-serial GEMM.
+ *
+ * Plain C: pointers, sizes and status codes; no C++ or torch types.  The C++
+ * drop-in header include/gemmbitserial.hpp is written on top of it, and any
+ * FFI (ctypes, cgo, JNI) can bind it directly.
+ *
+ * Each entry point names the reference interface it replaces (paths are
+ * relative to the gemmbitserial checkout).  The reference is a header-only
+ * C++ library with no FFI of its own; the functions below are what its
+ * BitSerialMatrix / GEMMContext methods call in the drop-in header.
+ *
+ * Conventions
+ *   - Every function returns GBS_OK (0) or a positive gbs_status; the message
+ *     of the last failure on the calling thread is gbs_last_error().
+ *   - Plane layout, on the host and on the device: uint64 words
+ *     [nbits][nrows_a][ncols_a/64]; column c of a row is bit (c & 63) of word
+ *     (c >> 6); every bit outside the real nrows x ncols is zero
+ *     (gemmbitserial.hpp:44-50, 89-129).
+ *   - rhs is given as rhsRows x depth ("transposed"); the result is
+ *     rhsRows x lhsRows row-major int32, res[n*M + m] (README.md "Note that the
+ *     right-hand-side matrix must be provided in transposed form").
+ *   - Synchronous entry points return with their results in place.  The
+ *     *_async entry points only enqueue on the given HIP stream (NULL = the
+ *     calling thread's library stream).
+ *   - Thread safety: calls on distinct objects from distinct threads are
+ *     safe; each thread owns its own library stream.  Calls that share a
+ *     gbs_workspace must not overlap.
+ */
+#ifndef GBS_HIP_H
+#define GBS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum gbs_status {
+  GBS_OK = 0,
+  GBS_EINVAL = 1,   /* bad argument: shape mismatch, null pointer, bitwidth */
+  GBS_ENOMEM = 2,   /* device or pinned allocation failed */
+  GBS_EDEVICE = 3,  /* HIP runtime or launch error */
+  GBS_ERANGE = 4,   /* request exceeds a stated limit */
+  GBS_ENOTSUP = 5   /* combination this build does not implement */
+} gbs_status;
+
+/* Element type of a regular (unpacked) matrix handed to gbs_import.
+ * GBS_U8 follows the reference's template importRegular<T> with
+ * T = (const) uint8_t; GBS_U8_RAW follows its non-template
+ * importRegular(uint8_t*) overload, which takes the raw low bits without the
+ * bipolar "> 0" test (gemmbitserial.hpp:212-245 vs 249-275). */
+typedef enum gbs_dtype {
+  GBS_I8 = 0, GBS_U8 = 1, GBS_U8_RAW = 2, GBS_I16 = 3, GBS_U16 = 4,
+  GBS_I32 = 5, GBS_U32 = 6, GBS_I64 = 7, GBS_U64 = 8
+} gbs_dtype;
+
+/* GEMM engine.  AUTO picks per shape from the measured crossover; both
+ * engines return identical int32 results. */
+typedef enum gbs_algo {
+  GBS_ALGO_AUTO = 0,
+  GBS_ALGO_POPCOUNT = 1, /* fused AND/XOR + popcount over bit-planes */
+  GBS_ALGO_MFMA = 2      /* planes expanded to int8, v_mfma_i32_*_i8 */
+} gbs_algo;
+
+typedef struct gbs_matrix gbs_matrix;       /* device-resident bit-serial matrix */
+typedef struct gbs_workspace gbs_workspace; /* device result buffer + scratch */
+
+/* Library / device ---------------------------------------------------- */
+
+/* Select and initialise the HIP device used by this thread's calls
+ * (default 0).  No reference counterpart: the reference runs on the host. */
+int gbs_init(int device);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *gbs_last_error(void);
+
+/* Library version, major*10000 + minor*100 + patch. */
+int gbs_version(void);
+
+/* Pinned host memory for plane mirrors and results (hipHostMalloc).
+ * Replaces the `new uint64_t[]` / `new int32_t[]` of
+ * gemmbitserial.hpp:33 and :450. */
+int gbs_host_alloc(size_t bytes, void **out);
+int gbs_host_free(void *p);
+
+/* Bit-serial matrices -------------------------------------------------- */
+
+/* Allocate a device matrix of nbits planes, nrows x ncols real elements,
+ * padded to nrows_a x ncols_a (ncols_a % 64 == 0, nrows_a >= nrows,
+ * ncols_a >= ncols).  Planes start zero.  1 <= nbits <= 8.
+ * Replaces BitSerialMatrix::alloc (gemmbitserial.hpp:24-35). */
+int gbs_matrix_create(uint64_t nbits, uint64_t nrows, uint64_t ncols,
+                      int issigned, uint64_t nrows_a, uint64_t ncols_a,
+                      gbs_matrix **out);
+
+/* Free a matrix.  Replaces BitSerialMatrix::dealloc (gemmbitserial.hpp:38-40). */
+int gbs_matrix_destroy(gbs_matrix *m);
+
+/* Device address of the plane words (layout above). */
+int gbs_matrix_device_planes(const gbs_matrix *m, uint64_t **d_planes);
+
+/* Tell the library that the plane words were written directly on the
+ * device (through gbs_matrix_device_planes), so derived copies -- the int8
+ * values of the MFMA engine -- are rebuilt before the next GEMM.  No
+ * reference counterpart (the reference has no derived copies). */
+int gbs_matrix_mark_modified(gbs_matrix *m);
+
+/* Pack a regular matrix into the planes on the GPU (one __ballot per plane
+ * word).  src holds nrows x ncols elements of type dt, row-major, or
+ * column-major (src[c*nrows + r]) when read_col_major; src_on_device says
+ * whether src is a device pointer.  When host_planes is non-NULL the packed
+ * words (all nbits*nrows_a*ncols_a/64 of them) are copied there too.
+ * Replaces BitSerialMatrix::importRegular<T> / importRegular(uint8_t*) /
+ * importRegular_naive (gemmbitserial.hpp:135-164, 192-275). */
+int gbs_import(gbs_matrix *m, const void *src, gbs_dtype dt, int read_col_major,
+               int src_on_device, uint64_t *host_planes);
+
+/* Threshold-quantizing import for unsigned matrices: element e of row r
+ * becomes the index of the first threshold t with e <= thr[t*nrows + r]
+ * (nthres when none); int32 inputs and thresholds.
+ * Replaces BitSerialMatrix::importRegularAndQuantize
+ * (gemmbitserial.hpp:281-308). */
+int gbs_import_quantize(gbs_matrix *m, const int32_t *src,
+                        const int32_t *thresholds, int nthres,
+                        int read_col_major, int src_on_device,
+                        uint64_t *host_planes);
+
+/* Copy plane words between a host buffer (nbits*nrows_a*ncols_a/64 words)
+ * and the device.  Used by the drop-in header to keep BitSerialMatrix::data
+ * (gemmbitserial.hpp:50) and the device copy coherent after host-side
+ * set/unset/word/clearAll (gemmbitserial.hpp:95-114). */
+int gbs_planes_from_host(gbs_matrix *m, const uint64_t *host_planes);
+int gbs_planes_to_host(const gbs_matrix *m, uint64_t *host_planes);
+
+/* Unpack to nrows x ncols int32 values, row-major: bipolar -> +-1, else the
+ * two's-complement or unsigned value.
+ * Replaces BitSerialMatrix::exportRegular (gemmbitserial.hpp:312-333). */
+int gbs_export(const gbs_matrix *m, int32_t *dst, int dst_on_device);
+
+/* Row sums of the values (nrows int32).
+ * Replaces sumRows_generic (arch-generic.hpp:6-37). */
+int gbs_sum_rows(const gbs_matrix *m, int32_t *dst, int dst_on_device);
+
+/* GEMM ------------------------------------------------------------------- */
+
+/* A workspace holds the device result buffer and the row-sum scratch one
+ * GEMM needs; it grows on demand.  NULL workspaces mean the calling
+ * thread's own.  No reference counterpart (the reference GEMMContext owns
+ * a host result, gemmbitserial.hpp:379-411). */
+int gbs_workspace_create(gbs_workspace **out);
+int gbs_workspace_destroy(gbs_workspace *ws);
+
+/* res[n*M + m] = sum_k val(lhs[m,k]) * val(rhs[n,k]) in int32 (wrapping),
+ * M = lhs->nrows, N = rhs->nrows; lhs->ncols must equal rhs->ncols.
+ * Values: bipolar (1-bit signed) +-1; signed two's complement with the top
+ * plane weighted -2^(n-1); unsigned otherwise.  res is a host pointer
+ * (copied back before return) or, with res_on_device, a device pointer.
+ * Replaces gemmBitSerial_generic and everything it dispatches to
+ * (arch-generic.hpp:39-76, 106-193, 239-345). */
+int gbs_gemm(gbs_workspace *ws, const gbs_matrix *lhs, const gbs_matrix *rhs,
+             int32_t *res, int res_on_device, gbs_algo algo);
+
+/* Same product, enqueued on `stream` into the device buffer d_res
+ * (N*M int32) without synchronising.  Nothing is allocated when ws already
+ * has room (see gbs_workspace_reserve).  For timing loops and graph
+ * capture. */
+int gbs_gemm_async(gbs_workspace *ws, const gbs_matrix *lhs,
+                   const gbs_matrix *rhs, int32_t *d_res, gbs_algo algo,
+                   void *stream);
+
+/* Grow ws for a product with these two operands, so that gbs_gemm_async
+ * allocates nothing. */
+int gbs_workspace_reserve(gbs_workspace *ws, const gbs_matrix *lhs,
+                          const gbs_matrix *rhs);
+
+/* The engine GBS_ALGO_AUTO resolves to for these operands. */
+int gbs_gemm_select(const gbs_matrix *lhs, const gbs_matrix *rhs,
+                    gbs_algo *chosen);
+
+/* Streams and timing (HIP events on the stream the kernels run on). ---- */
+
+/* The calling thread's library stream (a hipStream_t). */
+int gbs_stream(void **stream);
+int gbs_stream_sync(void *stream);
+/* Elapsed milliseconds between two points recorded on a stream. */
+int gbs_event_create(void **ev);
+int gbs_event_destroy(void *ev);
+int gbs_event_record(void *ev, void *stream);
+int gbs_event_elapsed_ms(void *start, void *stop, float *ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GBS_HIP_H */

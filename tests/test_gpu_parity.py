@@ -1,0 +1,265 @@
+"""GPU parity: every HIP kernel against the oracle (and, where built, the
+reference compiled from source) on the same seeded inputs.  Bit-exact."""
+import itertools
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import mnist, rand_values
+from oracle import orc as O
+
+pytestmark = pytest.mark.gpu
+DIMS = [3, 5, 7, 16, 17, 18, 30, 31, 32, 100, 177, 256]  # test.cpp:112
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+# ------------------------------------------------------------------ import
+@pytest.mark.parametrize("nbits,issigned", [(1, True), (1, False), (2, True), (2, False),
+                                            (3, True), (4, False), (7, True), (8, False)])
+@pytest.mark.parametrize("col_major", [False, True])
+def test_pack_words_match_oracle(gbs, orc, nbits, issigned, col_major):
+    rng = np.random.default_rng(200 + nbits)
+    for (r, c, ra, ca) in [(4, 4, 4, 64), (17, 77, 18, 128), (33, 130, 40, 192),
+                           (5, 1023, 5, 1024), (64, 4096, 70, 4096), (1, 3, 2, 256)]:
+        say("pack", nbits, issigned, col_major, r, c)
+        vals = rng.integers(-128, 128, (r, c)).astype(np.int8)
+        src = vals.T.copy() if col_major else vals
+        want = O.import_regular(src, nbits, issigned, nrows_a=ra, ncols_a=ca,
+                                col_major=col_major)
+        m = gbs.DeviceMatrix(nbits, r, c, issigned, ra, ca)
+        got = m.import_regular(src, col_major=col_major, want_planes=True)
+        assert np.array_equal(got, want.planes)
+        assert np.array_equal(m.planes(), want.planes)
+
+
+@pytest.mark.parametrize("dtype", [np.int8, np.uint8, np.int16, np.uint16, np.int32,
+                                   np.uint32, np.int64, np.uint64])
+def test_pack_dtypes(gbs, orc, dtype):
+    rng = np.random.default_rng(8)
+    info = np.iinfo(dtype)
+    vals = rng.integers(max(info.min, -1000), min(info.max, 1000), (9, 70)).astype(dtype)
+    for nbits, issigned in [(1, True), (3, True), (3, False), (8, True)]:
+        say("dtype", dtype, nbits, issigned)
+        m = gbs.DeviceMatrix(nbits, 9, 70, issigned)
+        got = m.import_regular(vals, want_planes=True)
+        assert np.array_equal(got, O.import_regular(vals, nbits, issigned).planes)
+
+
+def test_pack_raw_uint8(gbs, orc):
+    vals = np.random.default_rng(3).integers(0, 256, (11, 90)).astype(np.uint8)
+    for nbits, issigned in [(1, True), (2, False), (3, True)]:
+        m = gbs.DeviceMatrix(nbits, 11, 90, issigned)
+        got = m.import_regular(vals, raw_u8=True, want_planes=True)
+        assert np.array_equal(got, O.import_regular(vals, nbits, issigned, raw_u8=True).planes)
+
+
+def test_pack_overwrites_previous_content(gbs, orc):
+    m = gbs.DeviceMatrix(2, 7, 100, True, 8, 128)
+    m.import_regular(np.full((7, 100), -1, np.int8))
+    vals = np.random.default_rng(1).integers(-2, 2, (7, 100)).astype(np.int8)
+    got = m.import_regular(vals, want_planes=True)
+    assert np.array_equal(got, O.import_regular(vals, 2, True, nrows_a=8, ncols_a=128).planes)
+
+
+def test_import_quantize(gbs, orc):
+    rng = np.random.default_rng(2)
+    for col_major in [False, True]:
+        vals = rng.integers(-50, 50, (6, 90))
+        thr = np.sort(rng.integers(-40, 40, (3, 6)), axis=0)
+        src = vals.T.copy() if col_major else vals
+        want = O.import_and_quantize(src, thr, 2, col_major=col_major)
+        m = gbs.DeviceMatrix(2, 6, 90, False)
+        got = m.import_quantize(src, thr, col_major=col_major, want_planes=True)
+        assert np.array_equal(got, want.planes)
+
+
+@pytest.mark.parametrize("nbits,issigned", list(itertools.product([1, 2, 3, 4, 8], [True, False])))
+def test_export_and_sum_rows(gbs, orc, nbits, issigned):
+    rng = np.random.default_rng(nbits)
+    for d in [4, 17, 77, 1023]:
+        vals = rng.integers(-100, 100, (d, d)).astype(np.int8)
+        m = gbs.from_values(vals, nbits, issigned)
+        want = O.import_regular(vals, nbits, issigned)
+        assert np.array_equal(m.export_regular(), O.export_regular(want))
+        assert np.array_equal(m.sum_rows(), O.sum_rows(want))
+
+
+# --------------------------------------------------------------------- gemm
+def run_gemm(gbs, lhs, rhs, lb, rb, ls, rs, algo="popcount", geom=None):
+    M, K = lhs.shape
+    N = rhs.shape[0]
+    if geom is None:
+        geom = O.alloc_geometry(M, K, N)
+    l = gbs.DeviceMatrix(lb, M, K, ls, geom[2], geom[3])
+    r = gbs.DeviceMatrix(rb, N, K, rs, geom[4], geom[5])
+    l.import_regular(lhs)
+    r.import_regular(rhs)
+    return gbs.gemm(l, r, algo)
+
+
+def expected(lhs, rhs, lb, rb, ls, rs):
+    lp = O.import_regular(lhs, lb, ls)
+    rp = O.import_regular(rhs, rb, rs)
+    return O.gemm(lp, rp)
+
+
+@pytest.mark.parametrize("lb,rb", list(itertools.product([1, 2, 3, 4], repeat=2)))
+@pytest.mark.parametrize("ls,rs", list(itertools.product([False, True], repeat=2)))
+def test_gemm_all_encodings(gbs, orc, lb, rb, ls, rs):
+    rng = np.random.default_rng(lb * 100 + rb * 10 + 2 * ls + rs)
+    for (M, K, N) in [(3, 6, 9), (17, 34, 1), (31, 62, 93), (18, 200, 5), (2, 70, 3),
+                      (130, 700, 67), (64, 1030, 200)]:
+        say("gemm", lb, rb, ls, rs, M, K, N)
+        lhs = rand_values(rng, (M, K), lb, ls)
+        rhs = rand_values(rng, (N, K), rb, rs)
+        got = run_gemm(gbs, lhs, rhs, lb, rb, ls, rs)
+        assert np.array_equal(got, expected(lhs, rhs, lb, rb, ls, rs))
+
+
+def test_gemm_reference_grids(gbs, orc):
+    """test_matrix_matrix, test_bipolar_times_regular, test_bipolar_times_bipolar
+    (test.cpp:110-277) at the reference's dimension list."""
+    rng = np.random.default_rng(1)
+    for b, d, mv in itertools.product([2, 3, 4], DIMS, [0, 1]):
+        lhs = rand_values(rng, (d, 2 * d), b, False)
+        rhs = rand_values(rng, (1 if mv else 3 * d, 2 * d), b, False)
+        assert np.array_equal(run_gemm(gbs, lhs, rhs, b, b, False, False),
+                              O.wrap_i32(O.naive_gemm_i64(lhs, rhs)))
+    for sw, rbits, d, sg in itertools.product([0, 1], [2, 3, 4], DIMS, [True, False]):
+        bip = rand_values(rng, (d, d), 1, True)
+        reg = rand_values(rng, (d, d), rbits, sg)
+        if sw:
+            got = run_gemm(gbs, reg, bip, rbits, 1, sg, True)
+            want = O.naive_gemm_i64(reg, bip)
+        else:
+            got = run_gemm(gbs, bip, reg, 1, rbits, True, sg)
+            want = O.naive_gemm_i64(bip, reg)
+        assert np.array_equal(got, O.wrap_i32(want))
+    for d, mv in itertools.product(DIMS, [0, 1]):
+        lhs = rand_values(rng, (d, d), 1, True)
+        rhs = rand_values(rng, (1 if mv else d, d), 1, True)
+        assert np.array_equal(run_gemm(gbs, lhs, rhs, 1, 1, True, True),
+                              O.wrap_i32(O.naive_gemm_i64(lhs, rhs)))
+
+
+@pytest.mark.parametrize("lb,rb", [(5, 3), (8, 8), (6, 1), (1, 7)])
+def test_gemm_wide_bitwidths(gbs, orc, lb, rb):
+    rng = np.random.default_rng(lb + rb)
+    for ls, rs in itertools.product([False, True], repeat=2):
+        lhs = rand_values(rng, (37, 300), lb, ls, np.int16)
+        rhs = rand_values(rng, (45, 300), rb, rs, np.int16)
+        got = run_gemm(gbs, lhs, rhs, lb, rb, ls, rs)
+        assert np.array_equal(got, expected(lhs, rhs, lb, rb, ls, rs))
+
+
+def test_gemm_wraps_like_int32(gbs, orc):
+    K = 40000
+    lhs = np.full((2, K), 255, np.uint8)
+    rhs = np.full((9, K), 255, np.uint8)
+    got = run_gemm(gbs, lhs, rhs, 8, 8, False, False)
+    assert np.array_equal(got, O.wrap_i32(np.full((9, 2), 255 * 255 * K)))
+
+
+def test_gemm_empty_and_degenerate(gbs, orc):
+    for (M, K, N) in [(5, 0, 3), (0, 10, 3), (4, 10, 0), (1, 1, 1)]:
+        lhs = np.ones((M, K), np.int8)
+        rhs = np.ones((N, K), np.int8)
+        l = gbs.DeviceMatrix(2, M, K, False)
+        r = gbs.DeviceMatrix(2, N, K, False)
+        if M * K:
+            l.import_regular(lhs)
+        if N * K:
+            r.import_regular(rhs)
+        got = gbs.gemm(l, r, "popcount")
+        assert got.shape == (N, M) and np.array_equal(got, np.full((N, M), K, np.int32))
+
+
+def test_gemm_depth_mismatch_is_an_error(gbs):
+    l = gbs.DeviceMatrix(2, 4, 10, False)
+    r = gbs.DeviceMatrix(2, 4, 11, False)
+    with pytest.raises(gbs.GbsError, match="depths must match"):
+        gbs.gemm(l, r)
+
+
+def test_mnist_known_answer(gbs, orc):
+    w, x, golden = mnist()
+    geom = O.alloc_geometry(256, 784, 1)
+    got = run_gemm(gbs, w, x.reshape(1, -1), 1, 2, True, False, geom=geom)
+    assert np.array_equal(got.reshape(-1), golden)
+
+
+def test_split_k_path(gbs, orc):
+    # few tiles, deep K: exercises the K-split atomic path
+    rng = np.random.default_rng(12)
+    for lb, rb, ls, rs in [(1, 1, False, False), (1, 1, True, True), (2, 3, True, False),
+                           (1, 2, True, False)]:
+        lhs = rand_values(rng, (100, 20000), lb, ls)
+        rhs = rand_values(rng, (40, 20000), rb, rs)
+        got = run_gemm(gbs, lhs, rhs, lb, rb, ls, rs)
+        lv, rv = O.values_of(lhs, lb, ls), O.values_of(rhs, rb, rs)
+        assert np.array_equal(got, O.wrap_i32(O.blas_gemm_exact(lv, rv)))
+
+
+# ------------------------------------------------------------ int8 MFMA engine
+@pytest.mark.parametrize("lb,rb", list(itertools.product([1, 2, 3, 4], repeat=2)))
+@pytest.mark.parametrize("ls,rs", list(itertools.product([False, True], repeat=2)))
+def test_mfma_all_encodings(gbs, orc, lb, rb, ls, rs):
+    rng = np.random.default_rng(7000 + lb * 100 + rb * 10 + 2 * ls + rs)
+    for (M, K, N) in [(3, 6, 9), (17, 34, 1), (130, 700, 67), (64, 1030, 200),
+                      (257, 3000, 129)]:
+        say("mfma", lb, rb, ls, rs, M, K, N)
+        lhs = rand_values(rng, (M, K), lb, ls)
+        rhs = rand_values(rng, (N, K), rb, rs)
+        got = run_gemm(gbs, lhs, rhs, lb, rb, ls, rs, algo="mfma")
+        assert np.array_equal(got, expected(lhs, rhs, lb, rb, ls, rs))
+
+
+def test_mfma_split_k_and_large(gbs, orc):
+    rng = np.random.default_rng(77)
+    for (M, K, N, lb, rb, ls, rs) in [(100, 20000, 40, 1, 1, True, True),
+                                      (300, 8192, 520, 4, 4, True, False),
+                                      (4096, 512, 256, 2, 2, False, True)]:
+        lhs = rand_values(rng, (M, K), lb, ls)
+        rhs = rand_values(rng, (N, K), rb, rs)
+        got = run_gemm(gbs, lhs, rhs, lb, rb, ls, rs, algo="mfma")
+        want = O.wrap_i32(O.blas_gemm_exact(O.values_of(lhs, lb, ls), O.values_of(rhs, rb, rs)))
+        assert np.array_equal(got, want)
+
+
+def test_mfma_sees_reimported_and_host_written_planes(gbs, orc):
+    rng = np.random.default_rng(5)
+    a = rand_values(rng, (70, 300), 2, True)
+    b = rand_values(rng, (50, 300), 3, False)
+    l, r = gbs.from_values(a, 2, True), gbs.from_values(b, 3, False)
+    assert np.array_equal(gbs.gemm(l, r, "mfma"), expected(a, b, 2, 3, True, False))
+    b2 = rand_values(rng, (50, 300), 3, False)
+    r.import_regular(b2)
+    assert np.array_equal(gbs.gemm(l, r, "mfma"), expected(a, b2, 2, 3, True, False))
+    a2 = rand_values(rng, (70, 300), 2, True)
+    l.set_planes(O.import_regular(a2, 2, True).planes)
+    assert np.array_equal(gbs.gemm(l, r, "mfma"), expected(a2, b2, 2, 3, True, False))
+
+
+def test_mfma_refuses_inexact(gbs):
+    l = gbs.DeviceMatrix(8, 4, 64, False)   # unsigned 8-bit codes do not fit int8
+    r = gbs.DeviceMatrix(2, 4, 64, False)
+    with pytest.raises(gbs.GbsError, match="MFMA"):
+        gbs.gemm(l, r, "mfma")
+    assert gbs.select(l, r) == "popcount"
+
+
+def test_auto_matches_both_engines(gbs, orc):
+    rng = np.random.default_rng(8)
+    for (M, K, N, lb, rb, ls, rs) in [(512, 2048, 512, 1, 1, True, True),
+                                      (256, 2304, 1024, 2, 3, True, False),
+                                      (1000, 1000, 3, 4, 4, True, True)]:
+        lhs = rand_values(rng, (M, K), lb, ls)
+        rhs = rand_values(rng, (N, K), rb, rs)
+        l, r = gbs.from_values(lhs, lb, ls), gbs.from_values(rhs, rb, rs)
+        ra, rp, rm = gbs.gemm(l, r, "auto"), gbs.gemm(l, r, "popcount"), gbs.gemm(l, r, "mfma")
+        assert np.array_equal(ra, rp) and np.array_equal(rp, rm)
