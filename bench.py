@@ -1,16 +1,29 @@
 """Benchmark of the MI355X bit-serial GEMM (libgbs_hip.so).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--configs c0,c1,c2]
+                    [--engines auto|all] [--no-cpu]
 
-Workload of the headline line: the binary layers -- lhs 4096 x 8192 1-bit,
-rhs 512 x 8192 1-bit, int32 result -- run twice per step, once bipolar
-{-1,+1} (Bernoulli 0.5) and once unsigned {0,1} with 30 % ones.  Inputs are
-packed into bit-planes in HBM before timing; a step is one gemmBitSerial of
-each encoding, result left in HBM.  value = effective TOPS = 2*M*K*N per
-GEMM / device time, averaged over the timed steps (HIP events on the stream
-the kernels run on).  The reference CPU path (gemmBitSerial compiled from the
-reference sources, oracle/_ref) is timed on a bounded sample on the host.
-Prints one JSON line.
+Headline workload (value): the binary layers -- lhs 4096 x 8192 1-bit, rhs
+512 x 8192 1-bit, int32 result -- run once bipolar {-1,+1} (Bernoulli 0.5)
+and once unsigned {0,1} (30 % ones) per step.  Inputs are packed into
+bit-planes in HBM before timing; a step is one gemmBitSerial per encoding
+with the result left in HBM; value = effective TOPS = sum of 2*M*K*N over
+the step / device time per step (HIP events on the stream the kernels run
+on).  The other configurations are reported beside it under "configs":
+  c0  plain GEMM 256 x 2048 x 256, lhs 2-bit unsigned, rhs 2-bit signed
+  c2  conv-as-GEMM: weights 256 x 2304 2-bit signed, activations
+      50176 x 2304 3-bit unsigned (50 % zeros), popcount vs MFMA engine,
+      repeated at 4 x 4 bits for the crossover
+  c3  512 uneven products (M 64..4096, K 256..8192, N 1..256 log-uniform,
+      1..4-bit operands, random signedness) in one persistent launch,
+      replayed from a hipGraph, against one launch per product
+  c4  serving: 8-layer MLP, 4-bit signed weights resident in HBM, Poisson
+      arrivals of 1..64-row 4-bit requests, continuous batching; p50/p99
+      latency and TOPS
+Every GPU result is checked bit-exact against an exact product before it is
+reported.  The reference CPU path (gemmBitSerial compiled from the
+reference sources, oracle/_ref, single thread) is timed on a bounded sample
+of each configuration.  Prints one JSON line on stdout; progress on stderr.
 """
 import argparse
 import json
@@ -35,8 +48,16 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def binary_layers(seed):
-    rng = np.random.default_rng(seed)
+# ------------------------------------------------------------- workloads
+def u(rng, shape, lo, hi):
+    return rng.integers(lo, hi + 1, shape).astype(np.int8)
+
+
+def cfg_c0(rng):
+    return [("2u x 2s", u(rng, (256, 2048), 0, 3), u(rng, (256, 2048), -2, 1), 2, 2, False, True)]
+
+
+def cfg_c1(rng):
     M, K, N = 4096, 8192, 512
     a_bip = np.where(rng.random((M, K)) < 0.5, 1, -1).astype(np.int8)
     b_bip = np.where(rng.random((N, K)) < 0.5, 1, -1).astype(np.int8)
@@ -46,15 +67,57 @@ def binary_layers(seed):
             ("unsigned01", a_u, b_u, 1, 1, False, False)]
 
 
+def sparse_act(rng, shape, hi):
+    v = rng.integers(1, hi + 1, shape).astype(np.int8)
+    v[rng.random(shape) < 0.5] = 0
+    return v
+
+
+def cfg_c2(rng):
+    w = u(rng, (256, 2304), -2, 1)
+    act = sparse_act(rng, (50176, 2304), 7)
+    w4 = u(rng, (256, 2304), -8, 7)
+    act4 = sparse_act(rng, (50176, 2304), 15)
+    return [("2s x 3u", w, act, 2, 3, True, False),
+            ("4s x 4u", w4, act4, 4, 4, True, False)]
+
+
+CONFIGS = {"c0": ("plain GEMM 256x2048x256, lhs 2-bit unsigned, rhs 2-bit signed", cfg_c0),
+           "c1": ("binary layers: lhs 4096x8192 1-bit, rhs 512x8192 1-bit, bipolar "
+                  "(Bernoulli 0.5) + unsigned {0,1} (30% ones)", cfg_c1),
+           "c2": ("conv-as-GEMM: weights 256x2304, activations 50176x2304 (16x56x56 im2col, "
+                  "50% zeros), 2s x 3u and 4s x 4u", cfg_c2)}
+
+
 def valu_lane_ops(M, K, N, lb, rb):
     """AND/XOR + v_bcnt per 32-bit word per plane pair."""
     return M * N * ((K + 31) // 32) * 2 * lb * rb
 
 
-def hbm_bytes(M, K, N, lb, rb):
+def plane_bytes(M, K, N, lb, rb):
     return (M * K * lb + N * K * rb) // 8 + 4 * M * N
 
 
+def int8_bytes(M, K, N):
+    return (M + N) * K + 4 * M * N
+
+
+def roofline_of(algo, M, K, N, lb, rb, sec):
+    if algo == "mfma":
+        ach = 2.0 * M * K * N / sec
+        return {"bound": "mfma", "achieved": round(ach / 1e12, 2),
+                "peak": MFMA_I8_PEAK / 1e12, "unit": "TOP/s (int8)",
+                "frac": round(ach / MFMA_I8_PEAK, 4)}
+    ach = valu_lane_ops(M, K, N, lb, rb) / sec
+    hbm = plane_bytes(M, K, N, lb, rb) / sec
+    if hbm / HBM_PEAK > ach / VALU_PEAK:
+        return {"bound": "hbm", "achieved": round(hbm / 1e9, 1), "peak": HBM_PEAK / 1e9,
+                "unit": "GB/s", "frac": round(hbm / HBM_PEAK, 4)}
+    return {"bound": "valu", "achieved": round(ach / 1e12, 3), "peak": round(VALU_PEAK / 1e12, 3),
+            "unit": "Tlane-op/s (int32 VALU)", "frac": round(ach / VALU_PEAK, 4)}
+
+
+# ---------------------------------------------------------------- timing
 def time_gemm(gbs, ws, l, r, d_res, algo, steps, warmup, stream):
     for _ in range(warmup):
         ws.gemm_async(l, r, d_res.ptr, algo, stream)
@@ -68,25 +131,168 @@ def time_gemm(gbs, ws, l, r, d_res, algo, steps, warmup, stream):
     return e0.elapsed_ms(e1) / steps
 
 
-def cpu_baseline(runs, budget_s):
-    """Reference gemmBitSerial (oracle/_ref, -O3, single thread) on the same
-    inputs; a bounded number of calls per encoding."""
-    from oracle import orc as O
-    if not O.ref_available():
-        return None
-    total_ops, total_ns, samples = 0.0, 0.0, []
-    for name, a, b, lb, rb, ls, rs in runs:
+def run_config(gbs, O, name, runs, engines, steps, warmup, stream, ws, cpu_secs):
+    out = []
+    for enc, a, b, lb, rb, ls, rs in runs:
         M, K = a.shape
         N = b.shape[0]
-        ns, reps, _ = O.ref_bench_gemm(a, b, lb, rb, ls, rs,
-                                       min_secs=budget_s / len(runs), max_reps=50)
-        total_ops += 2.0 * M * K * N
-        total_ns += ns
-        samples.append(f"{name}: {reps} timed calls after 1 warm-up, {ns / 1e6:.1f} ms/call")
-    return {"value": total_ops / total_ns / 1e3, "unit": "TOPS", "cores": 1,
-            "kind": "reference",
-            "sample": "full binary-layer shapes, " + "; ".join(samples) +
-                      f"; host nproc={os.cpu_count()}"}
+        log(f"[bench] {name} {enc}: pack lhs {M}x{K} ({lb}b), rhs {N}x{K} ({rb}b)")
+        l = gbs.from_values(a, lb, ls)
+        r = gbs.from_values(b, rb, rs)
+        ws.reserve(l, r)
+        d_res = gbs.DeviceBuffer(4 * M * N)
+        auto = gbs.select(l, r)
+        want = None
+        row = {"encoding": enc, "M": M, "K": K, "N": N, "bits": [lb, rb],
+               "signed": [ls, rs], "auto": auto, "engines": {}}
+        for eng in (["popcount", "mfma"] if engines == "all" else [auto]):
+            log(f"[bench] {name} {enc}: {eng} x {steps}")
+            try:
+                ms = time_gemm(gbs, ws, l, r, d_res, eng, steps, warmup, stream)
+            except gbs.GbsError as e:  # e.g. MFMA cannot hold this product exactly
+                row["engines"][eng] = {"error": str(e)}
+                continue
+            got = d_res.to_host(np.int32, (N, M))
+            if want is None:
+                want = O.wrap_i32(O.blas_gemm_exact(O.values_of(a, lb, ls),
+                                                    O.values_of(b, rb, rs)))
+            if not np.array_equal(got, want):
+                raise SystemExit(f"{name} {enc} {eng}: result differs from the exact product")
+            sec = ms * 1e-3
+            row["engines"][eng] = {"us": round(ms * 1e3, 2),
+                                   "tops": round(2.0 * M * K * N / sec / 1e12, 2),
+                                   "roofline": roofline_of(eng, M, K, N, lb, rb, sec),
+                                   "bit_exact": True}
+            log(f"[bench] {name} {enc}: {eng} {ms * 1e3:.1f} us, "
+                f"{2.0 * M * K * N / sec / 1e12:.1f} TOPS, bit-exact")
+        if cpu_secs > 0 and O.ref_available():
+            ns, reps, res = O.ref_bench_gemm(a, b, lb, rb, ls, rs, min_secs=cpu_secs,
+                                             max_reps=200)
+            if not np.array_equal(res, want):
+                raise SystemExit(f"{name} {enc}: reference CPU result differs")
+            row["cpu_reference"] = {"tops": round(2.0 * M * K * N / ns / 1e3, 4),
+                                    "ms_per_call": round(ns / 1e6, 3), "timed_calls": reps,
+                                    "cores": 1}
+            log(f"[bench] {name} {enc}: reference CPU {ns / 1e6:.1f} ms/call ({reps} calls)")
+        for m in (l, r):
+            m.close()
+        d_res.close()
+        out.append(row)
+    return out
+
+
+def bench_batched(gbs, O, steps, warmup, stream, log):
+    from gemmbitserial_amd import workloads as W
+    log("[bench] c3: generate and pack 512 products")
+    probs = W.batched_uneven()
+    items = []
+    for p in probs:
+        l = gbs.from_values(p["lhs"], p["lb"], p["ls"])
+        r = gbs.from_values(p["rhs"], p["rb"], p["rs"])
+        items.append((l, r, gbs.DeviceBuffer(4 * p["M"] * p["N"])))
+    b = gbs.Batch([(l, r, d.ptr) for l, r, d in items])
+    info = b.info()
+    ops = sum(2.0 * p["M"] * p["K"] * p["N"] for p in probs)
+
+    def timed(fn, n):
+        for _ in range(warmup):
+            fn(stream)
+        gbs.sync(stream)
+        e0, e1 = gbs.Event(), gbs.Event()
+        e0.record(stream)
+        for _ in range(n):
+            fn(stream)
+        e1.record(stream)
+        gbs.sync(stream)
+        return e0.elapsed_ms(e1) / n
+
+    out = {"workload": "512 uneven products in one launch", "problems": len(probs),
+           "tiles": info["tiles"], "persistent_products": info["persistent"],
+           "separate_products": info["separate"], "ops": ops}
+    for name, fn, n in [("graph", b.run_graph, steps), ("persistent", b.run, steps),
+                        ("per_call", b.run_per_call, max(3, steps // 5))]:
+        log(f"[bench] c3: {name} x {n}")
+        ms = timed(fn, n)
+        bad = 0
+        for (l, r, d), p in zip(items, probs):
+            got = d.to_host(np.int32, (p["N"], p["M"]))
+            want = O.wrap_i32(O.blas_gemm_exact(O.values_of(p["lhs"], p["lb"], p["ls"]),
+                                                O.values_of(p["rhs"], p["rb"], p["rs"])))
+            bad += not np.array_equal(got, want)
+            if name != "graph":
+                break  # full check once; spot-check the other runs
+        if bad:
+            raise SystemExit(f"c3 {name}: {bad} products differ from the exact result")
+        out[name] = {"ms": round(ms, 4), "tops": round(ops / (ms * 1e-3) / 1e12, 2)}
+        log(f"[bench] c3: {name} {ms * 1e3:.1f} us, {ops / (ms * 1e-3) / 1e12:.1f} TOPS")
+        d0 = items[0][2]
+        d0.fill_zero()
+    out["speedup_graph_vs_per_call"] = round(out["per_call"]["ms"] / out["graph"]["ms"], 2)
+    out["bit_exact"] = True
+    b.close()
+    for l, r, d in items:
+        l.close(), r.close(), d.close()
+    return out
+
+
+def bench_serving(gbs, rate, count, log):
+    from gemmbitserial_amd import workloads as W
+    h, Ws, shifts = W.serving_mlp()
+    mats = [gbs.from_values(w, 4, True) for w in Ws]
+    srv = gbs.Server(mats, shifts, act_bits=4, max_requests=64, max_rows=64)
+    reqs = W.serving_requests(h[0], count, rate)
+    # warm-up requests (not measured)
+    for i in range(8):
+        srv.submit(10**6 + i, reqs[i]["x"])
+    pend = {10**6 + i for i in range(8)}
+    while pend:
+        srv.step()
+        for i in list(pend):
+            if srv.take(i) is not None:
+                pend.discard(i)
+    st0 = srv.stats()
+    log(f"[bench] c4: serving {count} requests at {rate:.0f}/s, hidden sizes {h}")
+    t0 = gbs.clock_ns() + 1_000_000
+    arrivals = [t0 + int(r["t"] * 1e9) for r in reqs]
+    nxt, lat, res, pending = 0, [], {}, set()
+    while len(res) < count:
+        now = gbs.clock_ns()
+        while nxt < count and arrivals[nxt] <= now:
+            srv.submit(reqs[nxt]["id"], reqs[nxt]["x"], arrivals[nxt])
+            pending.add(reqs[nxt]["id"])
+            nxt += 1
+        if not pending:
+            continue  # idle until the next arrival
+        if srv.step():
+            for i in list(pending):
+                t = srv.take(i)
+                if t is not None:
+                    res[i] = t
+                    lat.append((t[2] - t[1]) / 1e6)
+                    pending.discard(i)
+    t_end = max(t[2] for t in res.values())
+    st = srv.stats()
+    ops = st["ops"] - st0["ops"]
+    wall = (t_end - arrivals[0]) / 1e9
+    # bit-exactness of a sample of requests against the exact forward pass
+    sample = list(range(0, count, max(1, count // 16)))
+    for i in sample:
+        if not np.array_equal(res[i][0], W.mlp_reference(reqs[i]["x"], Ws, shifts)):
+            raise SystemExit(f"c4: request {i} differs from the exact forward pass")
+    lat = np.array(lat)
+    out = {"workload": "8-layer MLP serving, continuous batching", "hidden": h,
+           "shifts": shifts, "requests": count, "rate_per_s": rate,
+           "rows_total": int(sum(r["x"].shape[0] for r in reqs)),
+           "p50_ms": round(float(np.percentile(lat, 50)), 3),
+           "p99_ms": round(float(np.percentile(lat, 99)), 3),
+           "tops": round(ops / wall / 1e12, 3), "steps": st["steps"] - st0["steps"],
+           "wall_s": round(wall, 4), "row_capacity": st["row_capacity"],
+           "bit_exact_sample": len(sample)}
+    log(f"[bench] c4: p50 {out['p50_ms']} ms, p99 {out['p99_ms']} ms, {out['tops']} TOPS")
+    srv.close()
+    for m in mats:
+        m.close()
+    return out
 
 
 def main():
@@ -94,14 +300,19 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--algo", default="auto", choices=["auto", "popcount", "mfma"])
-    ap.add_argument("--cpu-secs", type=float, default=12.0)
+    ap.add_argument("--configs", default="c1,c0,c2,c3,c4")
+    ap.add_argument("--serve-rate", type=float, default=2000.0)
+    ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--engines", default="all", choices=["auto", "all"])
+    ap.add_argument("--cpu-secs", type=float, default=4.0,
+                    help="reference CPU timing budget per run (0 = skip)")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); this build "
-              "has no multi-GPU path: every problem fits one MI355X.", file=sys.stderr)
+              "has no multi-GPU path: every problem here fits one MI355X.", file=sys.stderr)
         sys.exit(2)
+    cpu_secs = 0.0 if args.no_cpu else args.cpu_secs
 
     import gemmbitserial_amd as gbs
     from oracle import orc as O
@@ -109,57 +320,48 @@ def main():
     stream = gbs.stream()
     ws = gbs.Workspace()
 
-    runs = binary_layers(SEED0 + 1)
-    results, total_ops, total_ms = [], 0.0, 0.0
-    for name, a, b, lb, rb, ls, rs in runs:
-        M, K = a.shape
-        N = b.shape[0]
-        log(f"[bench] {name}: pack {M}x{K} and {N}x{K}")
-        l = gbs.DeviceMatrix(lb, M, K, ls)
-        r = gbs.DeviceMatrix(rb, N, K, rs)
-        l.import_regular(a)
-        r.import_regular(b)
-        ws.reserve(l, r)
-        d_res = gbs.DeviceBuffer(4 * M * N)
-        algo = gbs.select(l, r) if args.algo == "auto" else args.algo
-        log(f"[bench] {name}: time {args.steps} steps ({algo})")
-        ms = time_gemm(gbs, ws, l, r, d_res, algo, args.steps, args.warmup, stream)
-        got = d_res.to_host(np.int32, (N, M))
-        want = O.wrap_i32(O.blas_gemm_exact(O.values_of(a, lb, ls), O.values_of(b, rb, rs)))
-        exact = bool(np.array_equal(got, want))
-        log(f"[bench] {name}: {ms * 1e3:.1f} us/call, bit-exact={exact}")
-        if not exact:
-            raise SystemExit(f"{name}: result differs from the exact product")
-        ops = 2.0 * M * K * N
-        total_ops += ops
-        total_ms += ms
-        results.append({"encoding": name, "M": M, "K": K, "N": N, "algo": algo,
-                        "us_per_call": round(ms * 1e3, 2),
-                        "tops": round(ops / (ms * 1e-3) / 1e12, 2),
-                        "valu_lane_ops": valu_lane_ops(M, K, N, lb, rb),
-                        "valu_frac": round(valu_lane_ops(M, K, N, lb, rb) / (ms * 1e-3) / VALU_PEAK, 4),
-                        "bit_exact": exact})
-    value = total_ops / (total_ms * 1e-3) / 1e12
-    # dominant kernel: the popcount GEMM; achieved = algorithmic VALU lane-ops
-    # per launch / measured launch duration
-    lane_ops = sum(x["valu_lane_ops"] for x in results)
-    achieved = lane_ops / (total_ms * 1e-3)
-    roofline = {"bound": "valu", "achieved": round(achieved / 1e12, 3),
-                "peak": round(VALU_PEAK / 1e12, 3), "unit": "Tlane-op/s",
-                "frac": round(achieved / VALU_PEAK, 4), "traffic": None}
+    names = [c.strip() for c in args.configs.split(",") if c.strip()]
+    if "c1" not in names:
+        names = ["c1"] + names
+    results = {}
+    for i, name in enumerate(names):
+        if name == "c3":
+            results[name] = bench_batched(gbs, O, args.steps, args.warmup, stream, log)
+            continue
+        if name == "c4":
+            results[name] = bench_serving(gbs, args.serve_rate, args.serve_requests, log)
+            continue
+        desc, make = CONFIGS[name]
+        runs = make(np.random.default_rng(SEED0 + int(name[1])))
+        results[name] = {"workload": desc, "seed": SEED0 + int(name[1]),
+                         "runs": run_config(gbs, O, name, runs, args.engines, args.steps,
+                                            args.warmup, stream, ws, cpu_secs)}
+    # headline: the binary layers through gemmBitSerial's own engine choice
+    c1 = results["c1"]["runs"]
+    step_ms = sum(r["engines"][r["auto"]]["us"] for r in c1) / 1e3
+    ops = sum(2.0 * r["M"] * r["K"] * r["N"] for r in c1)
+    value = ops / (step_ms * 1e-3) / 1e12
+    # dominant kernel of the step: the engine AUTO picked (same for both runs)
+    eng = c1[0]["auto"]
+    r0 = c1[0]
+    roof = roofline_of(eng, r0["M"], r0["K"], r0["N"], 1, 1, step_ms * 1e-3 / len(c1))
+    roof["traffic"] = None
     cpu = None
-    if not args.no_cpu:
-        log(f"[bench] cpu baseline (~{args.cpu_secs:.0f} s)")
-        cpu = cpu_baseline(runs, args.cpu_secs)
-    line = {"metric": METRIC, "value": round(value, 3), "unit": "TOPS",
+    if cpu_secs > 0 and all("cpu_reference" in r for r in c1):
+        cpu_ms = sum(r["cpu_reference"]["ms_per_call"] for r in c1)
+        cpu = {"value": round(ops / (cpu_ms * 1e-3) / 1e12, 4), "unit": "TOPS", "cores": 1,
+               "kind": "reference",
+               "sample": "the two full binary-layer GEMMs, " + ", ".join(
+                   f"{r['encoding']}: {r['cpu_reference']['timed_calls']} timed calls after 1 "
+                   f"warm-up" for r in c1) + f"; host logical CPUs {os.cpu_count()}"}
+    line = {"metric": METRIC, "value": round(value, 2), "unit": "TOPS",
             "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-            "ms_per_step": round(total_ms, 5), "higher_is_better": True,
+            "ms_per_step": round(step_ms, 5), "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "int32",
             "data": "synthetic (seeded numpy), packed to bit-planes in HBM before timing",
-            "config": {"workload": "binary layers: lhs 4096x8192 1-bit, rhs 512x8192 1-bit, "
-                                   "bipolar (Bernoulli 0.5) + unsigned {0,1} (30% ones)",
-                       "seed": SEED0 + 1, "runs": results},
-            "roofline": roofline, "cpu_baseline": cpu}
+            "config": {"workload": CONFIGS["c1"][0], "engine": eng,
+                       "configs": results},
+            "roofline": roof, "cpu_baseline": cpu}
     print(json.dumps(line), flush=True)
 
 

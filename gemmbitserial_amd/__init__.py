@@ -36,6 +36,12 @@ EXPORTED = [
     "gbs_gemm", "gbs_gemm_async", "gbs_workspace_reserve", "gbs_gemm_select",
     "gbs_stream", "gbs_stream_sync", "gbs_event_create", "gbs_event_destroy",
     "gbs_event_record", "gbs_event_elapsed_ms",
+    "gbs_batch_create", "gbs_batch_destroy", "gbs_batch_run", "gbs_batch_run_graph",
+    "gbs_batch_run_per_call", "gbs_batch_info",
+    "gbs_tune_classes", "gbs_tune_class_of", "gbs_tune_set", "gbs_tune_load",
+    "gbs_clock_ns", "gbs_server_create", "gbs_server_destroy", "gbs_server_submit",
+    "gbs_server_step", "gbs_server_take", "gbs_server_stats",
+    "gbs_gemm_async_config",
 ]
 
 
@@ -73,6 +79,17 @@ def lib():
         "gbs_stream": [_p], "gbs_stream_sync": [_p], "gbs_event_create": [_p],
         "gbs_event_destroy": [_p], "gbs_event_record": [_p, _p],
         "gbs_event_elapsed_ms": [_p, _p, _p],
+        "gbs_batch_create": [_p, _u64, _p], "gbs_batch_destroy": [_p],
+        "gbs_batch_run": [_p, _p], "gbs_batch_run_graph": [_p, _p],
+        "gbs_batch_run_per_call": [_p, _p], "gbs_batch_info": [_p, _p, _p, _p],
+        "gbs_tune_classes": [_p, _p], "gbs_tune_class_of": [_u64, _u64, _u64, _p],
+        "gbs_tune_set": [_p, _p, _i], "gbs_tune_load": [C.c_char_p],
+        "gbs_gemm_async_config": [_p, _p, _p, _p, _i, _i, _p],
+        "gbs_clock_ns": [_p], "gbs_server_create": [_p, _p, _i, _i, _p, _p],
+        "gbs_server_destroy": [_p], "gbs_server_submit": [_p, _u64, _p, _i, C.c_int64],
+        "gbs_server_step": [_p, _p],
+        "gbs_server_take": [_p, _u64, _p, _u64, _p, _p, _p, _p],
+        "gbs_server_stats": [_p, _p],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -213,6 +230,7 @@ class DeviceBuffer:
             h.hipMalloc.argtypes = [_p, C.c_size_t]
             h.hipFree.argtypes = [_p]
             h.hipMemcpy.argtypes = [_p, _p, C.c_size_t, _i]
+            h.hipMemset.argtypes = [_p, _i, C.c_size_t]
             h.hipDeviceSynchronize.argtypes = []
             cls._hip = h
         return cls._hip
@@ -231,6 +249,11 @@ class DeviceBuffer:
         if rc != 0:
             raise GbsError(3, f"hipMemcpy D2H failed: {rc}")
         return out
+
+    def fill_zero(self):
+        rc = self.hip().hipMemset(C.c_void_p(self.ptr), 0, max(self.nbytes, 8))
+        if rc != 0:
+            raise GbsError(3, f"hipMemset failed: {rc}")
 
     def close(self):
         if self.ptr:
@@ -286,3 +309,143 @@ class Event:
         ms = C.c_float()
         check(lib().gbs_event_elapsed_ms(self.e, end.e, C.byref(ms)))
         return ms.value
+
+
+class _Problem(C.Structure):
+    _fields_ = [("lhs", _p), ("rhs", _p), ("d_res", _p)]
+
+
+class Batch:
+    """gbs_batch: many products in one persistent launch.  problems is a
+    list of (lhs DeviceMatrix, rhs DeviceMatrix, device result pointer)."""
+
+    def __init__(self, problems):
+        self._keep = problems
+        arr = (_Problem * max(len(problems), 1))()
+        for i, (l, r, d) in enumerate(problems):
+            arr[i] = _Problem(l.h, r.h, d)
+        h = C.c_void_p()
+        check(lib().gbs_batch_create(arr, len(problems), C.byref(h)))
+        self.h = h
+
+    def run(self, stream=None):
+        check(lib().gbs_batch_run(self.h, stream))
+
+    def run_graph(self, stream=None):
+        check(lib().gbs_batch_run_graph(self.h, stream))
+
+    def run_per_call(self, stream=None):
+        check(lib().gbs_batch_run_per_call(self.h, stream))
+
+    def info(self):
+        t, a, b = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        check(lib().gbs_batch_info(self.h, C.byref(t), C.byref(a), C.byref(b)))
+        return {"tiles": t.value, "persistent": a.value, "separate": b.value}
+
+    def close(self):
+        if self.h:
+            check(lib().gbs_batch_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def tune_classes():
+    n, s = C.c_int(), C.c_int()
+    check(lib().gbs_tune_classes(C.byref(n), C.byref(s)))
+    return n.value, s.value
+
+
+def tune_class_of(M, N, K):
+    c = C.c_int()
+    check(lib().gbs_tune_class_of(M, N, K, C.byref(c)))
+    return c.value
+
+
+def tune_set(shapes, splits=None):
+    if shapes is None:
+        check(lib().gbs_tune_set(None, None, 0))
+        return
+    a = (C.c_int * len(shapes))(*shapes)
+    b = (C.c_int * len(shapes))(*(splits or [0] * len(shapes)))
+    check(lib().gbs_tune_set(a, b, len(shapes)))
+
+
+def tune_load(path):
+    check(lib().gbs_tune_load(path.encode()))
+
+
+class _ServerConfig(C.Structure):
+    _fields_ = [("max_requests", C.c_uint32), ("max_rows", C.c_uint32),
+                ("mem_budget", C.c_uint64)]
+
+
+class _ServerCounters(C.Structure):
+    _fields_ = [("steps", C.c_uint64), ("done_requests", C.c_uint64),
+                ("done_rows", C.c_uint64), ("ops", C.c_double),
+                ("row_capacity", C.c_uint32), ("bytes_reserved", C.c_uint64),
+                ("waiting", C.c_uint32), ("in_flight", C.c_uint32)]
+
+
+def clock_ns():
+    t = C.c_int64()
+    check(lib().gbs_clock_ns(C.byref(t)))
+    return t.value
+
+
+class Server:
+    """gbs_server: continuous batching over resident bit-serial layers.
+    weights: list of DeviceMatrix (h_{l+1} x h_l); shifts: requant shift per
+    layer but the last."""
+
+    def __init__(self, weights, shifts, act_bits=4, max_requests=64, max_rows=64,
+                 mem_budget=0):
+        L = lib()
+        self.weights = list(weights)
+        self.h = [self.weights[0].ncols] + [w.nrows for w in self.weights]
+        wa = (C.c_void_p * len(self.weights))(*[w.h.value for w in self.weights])
+        sh = (C.c_uint32 * max(1, len(shifts)))(*(list(shifts) or [0]))
+        cfg = _ServerConfig(max_requests, max_rows, mem_budget)
+        h = C.c_void_p()
+        check(L.gbs_server_create(wa, sh, len(self.weights), act_bits, C.byref(cfg), C.byref(h)))
+        self.s = h
+
+    def submit(self, rid, x, arrival_ns=0):
+        x = np.ascontiguousarray(x, dtype=np.uint8)
+        assert x.shape[1] == self.h[0]
+        check(lib().gbs_server_submit(self.s, rid, _ptr(x), x.shape[0], int(arrival_ns)))
+
+    def step(self):
+        n = C.c_uint32()
+        check(lib().gbs_server_step(self.s, C.byref(n)))
+        return n.value
+
+    def take(self, rid, max_rows=64):
+        out = np.zeros((max_rows, self.h[-1]), np.int32)
+        ready, rows = C.c_int(), C.c_uint32()
+        arr, done = C.c_int64(), C.c_int64()
+        check(lib().gbs_server_take(self.s, rid, _ptr(out), out.size, C.byref(ready),
+                                    C.byref(rows), C.byref(arr), C.byref(done)))
+        if not ready.value:
+            return None
+        return out[: rows.value].copy(), arr.value, done.value
+
+    def stats(self):
+        st = _ServerCounters()
+        check(lib().gbs_server_stats(self.s, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in _ServerCounters._fields_}
+
+    def close(self):
+        if self.s:
+            check(lib().gbs_server_destroy(self.s))
+            self.s = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

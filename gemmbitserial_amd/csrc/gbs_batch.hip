@@ -1,0 +1,352 @@
+// gbs_batch.hip -- many independent bit-serial GEMMs (uneven M/K/N, mixed
+// bitwidths and signedness) in one persistent kernel launch.
+//
+// Plan (gbs_batch_create): every product that the int8 MFMA engine holds
+// exactly becomes a run of work items -- output tiles of the shape its
+// shape class maps to in the tuning table, times K splits (deep products
+// are cut into pieces of at most 2 KiB of K, whose partial sums are added
+// with integer atomics into results zeroed first).  Products are ordered by
+// the cost of one item, largest first, so the long items start early and
+// the short ones fill the tail.  A device array holds one descriptor per product and
+// the prefix sum of tile counts.  Products the MFMA engine cannot hold
+// exactly (unsigned 8-bit codes, very deep K) go to the popcount engine as
+// separate launches after the persistent kernel.
+//
+// Run: a small kernel zeroes the queue counter, then a grid of
+// 2 blocks x 256 CUs loops: thread 0 takes the next tile index with one
+// atomicAdd, the block finds its product by binary search over the prefix
+// sums and computes that tile with the shape's code path.  Every block
+// leaves when the index passes the last tile.  The whole sequence can be
+// captured once into a hipGraph and replayed.
+#include <algorithm>
+#include <vector>
+
+#include "gbs_mfma_tile.h"
+
+struct gbs_batch {
+  int device = 0;
+  std::vector<gbs_problem> probs;          // as given
+  std::vector<uint32_t> mfma_idx;          // persistent products, in run order
+  std::vector<uint32_t> separate_idx;      // popcount products
+  void *d_desc = nullptr;                  // BatchProb[]
+  void *d_zero = nullptr;                  // ZeroSpan[] of split products
+  uint32_t n_zero = 0;
+  unsigned int *d_counter = nullptr;
+  uint64_t total_tiles = 0;
+  uint32_t grid = 0;
+  gbs_workspace *ws = nullptr;             // for separate products
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  hipStream_t graph_stream = nullptr;
+};
+
+namespace gbs {
+namespace {
+
+struct BatchProb {
+  MfmaOperands ops;
+  MfmaPlan plan;
+  uint64_t tile_begin;
+};
+
+struct ZeroSpan {
+  int32_t *p;
+  uint64_t n;
+};
+
+__global__ void reset_counter_kernel(unsigned int *c) { *c = 0; }
+
+// Zero the results of split products (their partial sums are added
+// atomically); blockIdx.y picks the span.
+__global__ __launch_bounds__(256) void zero_spans_kernel(const ZeroSpan *spans) {
+  const ZeroSpan z = spans[blockIdx.y];
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < z.n;
+       i += (uint64_t)gridDim.x * 256)
+    z.p[i] = 0;
+}
+
+__global__ __launch_bounds__(256, 2) void batch_kernel(const BatchProb *probs, uint32_t nprobs,
+                                                       uint64_t total, unsigned int *counter) {
+  __shared__ __attribute__((aligned(16))) int8_t lds[kMfmaLdsMax];
+  __shared__ uint32_t s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(counter, 1u);
+    __syncthreads();
+    const uint64_t item = s_item;
+    __syncthreads();
+    if (item >= total) break;
+    // largest p with tile_begin <= item
+    uint32_t lo = 0, hi = nprobs - 1;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) / 2;
+      if (probs[mid].tile_begin <= item) lo = mid;
+      else hi = mid - 1;
+    }
+    const MfmaOperands ops = probs[lo].ops;
+    const MfmaPlan pl = probs[lo].plan;
+    uint32_t pt, qt, sp;
+    plan_decode(pl, item - probs[lo].tile_begin, pt, qt, sp);
+    const uint32_t kbeg = sp * pl.kchunk;
+    const uint32_t kend = min(ops.kbytes, kbeg + pl.kchunk);
+    mfma_tile_by_shape(pl.shape, ops, pt, qt, kbeg, kend, pl.splits > 1, lds);
+    __syncthreads();
+  }
+}
+
+}  // namespace
+
+// Plan of one product inside a persistent launch: the tuned plan, with the
+// widest tile (not carried by the persistent kernel) replaced by 128 x 128,
+// and deep products cut into pieces of at most 16 K units so that no item
+// is much longer than the rest.
+static MfmaPlan batch_plan(uint32_t M, uint32_t N, uint32_t kbytes) {
+  MfmaPlan pl = mfma_plan(M, N, kbytes);
+  if (pl.shape != batch_shape(pl.shape)) pl = mfma_plan(M, N, kbytes, 0, (int)pl.splits);
+  const uint32_t units = kbytes / kMfmaKS;
+  const uint32_t want = (units + 15) / 16;
+  if (want > pl.splits) pl = mfma_plan(M, N, kbytes, pl.shape, (int)want);
+  return pl;
+}
+
+// Plan and launch products given as int8 operand views (the serving loop's
+// per-step batch).  host_desc must be pinned and hold batch_desc_bytes(n);
+// it is reused only after the stream has passed this launch.
+int launch_views(const MfmaOperands *views, int n, void *host_desc, void *d_desc,
+                 unsigned int *d_counter, hipStream_t s) {
+  BatchProb *h = (BatchProb *)host_desc;
+  ZeroSpan *hz = (ZeroSpan *)(h + (n > 0 ? n : 1));
+  uint64_t tiles = 0;
+  int m = 0, nz = 0;
+  for (int i = 0; i < n; i++) {
+    const MfmaOperands &v = views[i];
+    if (v.M == 0 || v.N == 0) continue;
+    BatchProb d{};
+    d.ops = v;
+    d.plan = batch_plan(v.M, v.N, v.kbytes);
+    d.tile_begin = tiles;
+    tiles += d.plan.tiles();
+    if (d.plan.splits > 1) hz[nz++] = ZeroSpan{v.res, (uint64_t)v.M * v.N};
+    h[m++] = d;
+  }
+  if (tiles == 0) return GBS_OK;
+  GBS_HIP(hipMemcpyAsync(d_desc, h, batch_desc_bytes(n), hipMemcpyHostToDevice, s));
+  if (nz) {
+    hipLaunchKernelGGL(zero_spans_kernel, dim3(64, nz), dim3(256), 0, s,
+                       (const ZeroSpan *)((BatchProb *)d_desc + (n > 0 ? n : 1)));
+  }
+  hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, d_counter);
+  const uint32_t grid = (uint32_t)(tiles < 512 ? tiles : 512);
+  hipLaunchKernelGGL(batch_kernel, dim3(grid), dim3(256), 0, s, (const BatchProb *)d_desc,
+                     (uint32_t)m, tiles, d_counter);
+  GBS_HIP(hipGetLastError());
+  return GBS_OK;
+}
+
+size_t batch_desc_bytes(int n) {
+  return (size_t)(n > 0 ? n : 1) * (sizeof(BatchProb) + sizeof(ZeroSpan));
+}
+
+static int batch_launch(gbs_batch *b, hipStream_t s) {
+  if (b->total_tiles) {
+    if (b->n_zero)
+      hipLaunchKernelGGL(zero_spans_kernel, dim3(64, b->n_zero), dim3(256), 0, s,
+                         (const ZeroSpan *)b->d_zero);
+    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, b->d_counter);
+    GBS_HIP(hipGetLastError());
+    hipLaunchKernelGGL(batch_kernel, dim3(b->grid), dim3(256), 0, s,
+                       (const BatchProb *)b->d_desc, (uint32_t)b->mfma_idx.size(),
+                       b->total_tiles, b->d_counter);
+    GBS_HIP(hipGetLastError());
+  }
+  for (uint32_t i : b->separate_idx) {
+    const gbs_problem &p = b->probs[i];
+    int rc = gemm_enqueue_public(b->ws, p.lhs, p.rhs, p.d_res, GBS_ALGO_POPCOUNT, s, false);
+    if (rc) return rc;
+  }
+  return GBS_OK;
+}
+
+}  // namespace gbs
+
+using namespace gbs;
+
+extern "C" {
+
+int gbs_batch_create(const gbs_problem *problems, uint64_t count, gbs_batch **out) {
+  if (!out) return fail(GBS_EINVAL, "gbs_batch_create: null out");
+  *out = nullptr;
+  if (count && !problems) return fail(GBS_EINVAL, "gbs_batch_create: null problems");
+  int rc = ensure_device();
+  if (rc) return rc;
+  hipStream_t s = thread_stream();
+  gbs_batch *b = new gbs_batch();
+  b->probs.assign(problems, problems + count);
+  if ((rc = gbs_workspace_create(&b->ws))) {
+    delete b;
+    return rc;
+  }
+  struct Item { uint32_t idx; double item_cost; MfmaPlan plan; };
+  std::vector<Item> items;
+  for (uint64_t i = 0; i < count; i++) {
+    const gbs_problem &p = problems[i];
+    if (!p.lhs || !p.rhs) {
+      gbs_batch_destroy(b);
+      return fail(GBS_EINVAL, "gbs_batch_create: problem %llu has a null matrix",
+                  (unsigned long long)i);
+    }
+    if (p.lhs->ncols != p.rhs->ncols) {
+      gbs_batch_destroy(b);
+      return fail(GBS_EINVAL, "gbs_batch_create: problem %llu depths differ (%llu vs %llu)",
+                  (unsigned long long)i, (unsigned long long)p.lhs->ncols,
+                  (unsigned long long)p.rhs->ncols);
+    }
+    if (p.lhs->nrows * p.rhs->nrows && !p.d_res) {
+      gbs_batch_destroy(b);
+      return fail(GBS_EINVAL, "gbs_batch_create: problem %llu has no result buffer",
+                  (unsigned long long)i);
+    }
+    if (p.lhs->nrows == 0 || p.rhs->nrows == 0) continue;
+    // room for any engine the per-call baseline may pick
+    if ((rc = gbs_workspace_reserve(b->ws, p.lhs, p.rhs))) {
+      gbs_batch_destroy(b);
+      return rc;
+    }
+    const Operand lo = view(p.lhs), ro = view(p.rhs);
+    if (mfma_supported(lo, ro, p.lhs->ncols) && p.lhs->ncols > 0) {
+      rc = ensure_vals(const_cast<gbs_matrix *>(p.lhs), s, true);
+      if (!rc) rc = ensure_vals(const_cast<gbs_matrix *>(p.rhs), s, true);
+      if (rc) {
+        gbs_batch_destroy(b);
+        return rc;
+      }
+      const uint32_t kb = (uint32_t)((p.lhs->ncols + kMfmaKS - 1) / kMfmaKS * kMfmaKS);
+      const MfmaPlan pl = batch_plan((uint32_t)p.lhs->nrows, (uint32_t)p.rhs->nrows, kb);
+      const MfmaTileShape sh = kMfmaShape[pl.shape];
+      items.push_back({(uint32_t)i, (double)sh.P * sh.Q * (double)pl.kchunk, pl});
+    } else {
+      b->separate_idx.push_back((uint32_t)i);
+    }
+  }
+  // longest items first: they start early and the short ones fill the tail
+  std::stable_sort(items.begin(), items.end(),
+                   [](const Item &x, const Item &y) { return x.item_cost > y.item_cost; });
+  std::vector<BatchProb> desc;
+  std::vector<ZeroSpan> zero;
+  uint64_t tiles = 0;
+  for (const Item &it : items) {
+    const gbs_problem &p = problems[it.idx];
+    BatchProb d{};
+    d.ops.R = p.rhs->d_vals;
+    d.ops.L = p.lhs->d_vals;
+    d.ops.r_ld = p.rhs->vals_ld;
+    d.ops.l_ld = p.lhs->vals_ld;
+    d.ops.res = p.d_res;
+    d.ops.M = (uint32_t)p.lhs->nrows;
+    d.ops.N = (uint32_t)p.rhs->nrows;
+    d.ops.kbytes = (uint32_t)((p.lhs->ncols + kMfmaKS - 1) / kMfmaKS * kMfmaKS);
+    d.plan = it.plan;
+    d.tile_begin = tiles;
+    tiles += d.plan.tiles();
+    if (d.plan.splits > 1) zero.push_back(ZeroSpan{p.d_res, (uint64_t)d.ops.M * d.ops.N});
+    desc.push_back(d);
+    b->mfma_idx.push_back(it.idx);
+  }
+  b->total_tiles = tiles;
+  const uint64_t slots = 256ull * 2;
+  b->grid = (uint32_t)(tiles < slots ? tiles : slots);
+  if (!desc.empty()) {
+    hipError_t e = hipMalloc(&b->d_desc, desc.size() * sizeof(BatchProb));
+    if (e == hipSuccess) e = hipMalloc((void **)&b->d_counter, 256);
+    if (e == hipSuccess && !zero.empty()) {
+      e = hipMalloc(&b->d_zero, zero.size() * sizeof(ZeroSpan));
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(b->d_zero, zero.data(), zero.size() * sizeof(ZeroSpan),
+                           hipMemcpyHostToDevice, s);
+      b->n_zero = (uint32_t)zero.size();
+    }
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(b->d_desc, desc.data(), desc.size() * sizeof(BatchProb),
+                         hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+      gbs_batch_destroy(b);
+      return hip_fail(e, "gbs_batch_create");
+    }
+  } else {
+    hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+      gbs_batch_destroy(b);
+      return hip_fail(e, "gbs_batch_create");
+    }
+  }
+  b->device = 0;
+  *out = b;
+  return GBS_OK;
+}
+
+int gbs_batch_destroy(gbs_batch *b) {
+  if (!b) return GBS_OK;
+  int rc = ensure_device();
+  if (rc) return rc;
+  (void)hipDeviceSynchronize();
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  if (b->d_desc) (void)hipFree(b->d_desc);
+  if (b->d_counter) (void)hipFree(b->d_counter);
+  if (b->d_zero) (void)hipFree(b->d_zero);
+  if (b->ws) gbs_workspace_destroy(b->ws);
+  delete b;
+  return GBS_OK;
+}
+
+int gbs_batch_run(gbs_batch *b, void *stream) {
+  if (!b) return fail(GBS_EINVAL, "gbs_batch_run: null batch");
+  int rc = ensure_device();
+  if (rc) return rc;
+  return batch_launch(b, stream ? (hipStream_t)stream : thread_stream());
+}
+
+int gbs_batch_run_graph(gbs_batch *b, void *stream) {
+  if (!b) return fail(GBS_EINVAL, "gbs_batch_run_graph: null batch");
+  int rc = ensure_device();
+  if (rc) return rc;
+  hipStream_t s = stream ? (hipStream_t)stream : thread_stream();
+  if (!b->exec) {
+    GBS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    rc = batch_launch(b, s);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return hip_fail(e, "gbs_batch_run_graph: capture");
+    b->graph = g;
+    GBS_HIP(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+  }
+  GBS_HIP(hipGraphLaunch(b->exec, s));
+  return GBS_OK;
+}
+
+int gbs_batch_run_per_call(gbs_batch *b, void *stream) {
+  if (!b) return fail(GBS_EINVAL, "gbs_batch_run_per_call: null batch");
+  int rc = ensure_device();
+  if (rc) return rc;
+  hipStream_t s = stream ? (hipStream_t)stream : thread_stream();
+  for (const gbs_problem &p : b->probs) {
+    if ((rc = gemm_enqueue_public(b->ws, p.lhs, p.rhs, p.d_res, GBS_ALGO_AUTO, s, false)))
+      return rc;
+  }
+  return GBS_OK;
+}
+
+int gbs_batch_info(const gbs_batch *b, uint64_t *tiles, uint64_t *persistent,
+                   uint64_t *separate) {
+  if (!b) return fail(GBS_EINVAL, "gbs_batch_info: null batch");
+  if (tiles) *tiles = b->total_tiles;
+  if (persistent) *persistent = b->mfma_idx.size();
+  if (separate) *separate = b->separate_idx.size();
+  return GBS_OK;
+}
+
+}  // extern "C"

@@ -84,12 +84,20 @@ int launch_popc_gemm(const Operand &l, const Operand &r, uint64_t kwords,
 // (allocating it on first use when may_allocate).
 int ensure_vals(gbs_matrix *m, hipStream_t s, bool may_allocate);
 int launch_mfma_gemm(const gbs_matrix *l, const gbs_matrix *r, int32_t *d_res,
-                     hipStream_t s);
+                     hipStream_t s, int forced_shape = -1, bool allow_split = true,
+                     int forced_splits = 0);
+// tuning table of the MFMA engine: tile shape and K split per shape class
+// (shapes == nullptr restores the built-in heuristic; splits 0 = automatic)
+int set_mfma_table(const int *shapes, const int *splits, int n);
 bool mfma_supported(const Operand &l, const Operand &r, uint64_t K);
 gbs_algo select_algo(const Operand &l, const Operand &r, uint64_t K);
 constexpr uint64_t kValsRowAlign = 256;  // rows of the int8 copy
 constexpr uint64_t kValsColAlign = 256;  // bytes per row of the int8 copy
 
 size_t dtype_size(gbs_dtype dt);
+
+// the GEMM enqueue shared by gbs_gemm, gbs_gemm_async and the batch engine
+int gemm_enqueue_public(gbs_workspace *ws, const gbs_matrix *l, const gbs_matrix *r,
+                        int32_t *d_res, gbs_algo algo, hipStream_t s, bool may_allocate);
 
 }  // namespace gbs

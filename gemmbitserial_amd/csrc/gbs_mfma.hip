@@ -12,139 +12,27 @@
 // K * max|val(lhs)| * max|val(rhs)| < 2^31 (checked by mfma_supported), in
 // which case it equals the reference's int32 result bit for bit.
 //
-// Kernel: 256 threads = 2 x 2 waves over a P (rhs rows) x Q (lhs rows)
-// tile; each wave owns (P/2) x (Q/2) outputs as 16 x 16 fragments of
-// v_mfma_i32_16x16x64_i8.  K is staged 128 bytes at a time in LDS
-// (row stride 144 B, so the 16 rows a fragment load touches fall in
-// distinct banks), double-buffered through registers with 16-byte loads.
-// The rhs operand goes to the MFMA A port so that a fragment's output
-// lanes run along m, the contiguous dimension of res.  Both operand
-// fragments take 16 contiguous bytes of a row at k-offset 16*(lane/16):
-// the same k-to-(lane, byte) assignment on both ports, so the dot product
-// over k is the same whatever order the instruction sums in.  Small grids
-// split K and add with integer atomics.
-#include "gbs_internal.h"
+// The tile itself (gbs_mfma_tile.h) comes in eight shapes; a product runs
+// the shape and K split its (M, N, K) class maps to in the tuning table
+// (written by gemmbitserial_amd/tune.py and loaded through gbs_tune_load,
+// else a size heuristic).  Split products zero the result first and add
+// their partial sums with integer atomics, which is exact in any order.
+#include "gbs_mfma_tile.h"
 
 namespace gbs {
 namespace {
 
-typedef int v4i __attribute__((ext_vector_type(4)));
-
-constexpr int KS = 128;       // K bytes per LDS stage
-constexpr int LDR = KS + 16;  // LDS row stride (bytes)
-
-struct MfmaArgs {
-  const int8_t *R, *L;  // rhs / lhs int8 values
-  uint64_t r_ld, l_ld;  // row strides (bytes)
-  int32_t *res;
-  uint32_t M, N;        // lhs rows, rhs rows
-  uint32_t kbytes;      // padded depth (multiple of KS)
-  uint32_t kchunk;      // depth per split (multiple of KS)
-  uint32_t ptiles, qtiles;
-  int atomic;
-};
-
-template <int P, int Q>
-__global__ __launch_bounds__(256) void mfma_kernel(MfmaArgs a) {
-  constexpr int FP = P / 32, FQ = Q / 32;         // fragments per wave
-  constexpr int CPR = KS / 16;                    // 16-byte chunks per row
-  constexpr int RLD = P * CPR / 256, LLD = Q * CPR / 256;
-  static_assert(RLD * 256 == P * CPR && LLD * 256 == Q * CPR, "tile/thread mismatch");
-  __shared__ __attribute__((aligned(16))) int8_t sR[2][P * LDR];
-  __shared__ __attribute__((aligned(16))) int8_t sL[2][Q * LDR];
-
+template <int P, int Q, int WP, int KSB, int DEPTH>
+__global__ __launch_bounds__(256, 2) void mfma_kernel(MfmaOperands a, MfmaPlan pl) {
+  __shared__ __attribute__((aligned(16))) int8_t lds[mfma_lds_bytes<P, Q, KSB>()];
   const uint32_t T = gridDim.x, b = blockIdx.x;
-  uint32_t t = (T % 8 == 0) ? (b % 8) * (T / 8) + b / 8 : b;  // XCD-aware order
-  const uint32_t qt = t % a.qtiles;
-  t /= a.qtiles;
-  const uint32_t pt = t % a.ptiles;
-  const uint32_t sp = t / a.ptiles;
-  const uint32_t p0 = pt * P, q0 = qt * Q;
-  const uint32_t kbeg = sp * a.kchunk;
-  const uint32_t kend = min(a.kbytes, kbeg + a.kchunk);
-
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wp = wave & 1, wq = wave >> 1;
-
-  v4i gr[RLD], gl[LLD];
-  auto gload = [&](uint32_t k0) {
-#pragma unroll
-    for (int i = 0; i < RLD; i++) {
-      const int c = tid + 256 * i, row = c / CPR, ch = c % CPR;
-      gr[i] = *reinterpret_cast<const v4i *>(a.R + (uint64_t)(p0 + row) * a.r_ld + k0 + ch * 16);
-    }
-#pragma unroll
-    for (int i = 0; i < LLD; i++) {
-      const int c = tid + 256 * i, row = c / CPR, ch = c % CPR;
-      gl[i] = *reinterpret_cast<const v4i *>(a.L + (uint64_t)(q0 + row) * a.l_ld + k0 + ch * 16);
-    }
-  };
-  auto swrite = [&](int buf) {
-#pragma unroll
-    for (int i = 0; i < RLD; i++) {
-      const int c = tid + 256 * i, row = c / CPR, ch = c % CPR;
-      *reinterpret_cast<v4i *>(&sR[buf][row * LDR + ch * 16]) = gr[i];
-    }
-#pragma unroll
-    for (int i = 0; i < LLD; i++) {
-      const int c = tid + 256 * i, row = c / CPR, ch = c % CPR;
-      *reinterpret_cast<v4i *>(&sL[buf][row * LDR + ch * 16]) = gl[i];
-    }
-  };
-
-  v4i acc[FP][FQ];
-#pragma unroll
-  for (int i = 0; i < FP; i++)
-#pragma unroll
-    for (int j = 0; j < FQ; j++) acc[i][j] = v4i{0, 0, 0, 0};
-
-  if (kbeg < kend) {
-    gload(kbeg);
-    swrite(0);
-  }
-  __syncthreads();
-  int buf = 0;
-  const int fr = lane & 15, fk = (lane >> 4) * 16;
-  for (uint32_t k0 = kbeg; k0 < kend; k0 += KS) {
-    const bool more = k0 + KS < kend;
-    if (more) gload(k0 + KS);
-#pragma unroll
-    for (int kk = 0; kk < KS / 64; kk++) {
-      v4i af[FP], bf[FQ];
-#pragma unroll
-      for (int i = 0; i < FP; i++)
-        af[i] = *reinterpret_cast<const v4i *>(
-            &sR[buf][(wp * (P / 2) + i * 16 + fr) * LDR + kk * 64 + fk]);
-#pragma unroll
-      for (int j = 0; j < FQ; j++)
-        bf[j] = *reinterpret_cast<const v4i *>(
-            &sL[buf][(wq * (Q / 2) + j * 16 + fr) * LDR + kk * 64 + fk]);
-#pragma unroll
-      for (int i = 0; i < FP; i++)
-#pragma unroll
-        for (int j = 0; j < FQ; j++)
-          acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
-    }
-    if (more) swrite(buf ^ 1);
-    __syncthreads();
-    buf ^= 1;
-  }
-
-  // fragment output: lane holds rows 4*(lane/16)+r (rhs, n) of column lane%16 (lhs, m)
-#pragma unroll
-  for (int i = 0; i < FP; i++)
-#pragma unroll
-    for (int j = 0; j < FQ; j++)
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const uint32_t n = p0 + wp * (P / 2) + i * 16 + 4 * (lane >> 4) + r;
-        const uint32_t m = q0 + wq * (Q / 2) + j * 16 + fr;
-        if (n < a.N && m < a.M) {
-          const uint64_t idx = (uint64_t)n * a.M + m;
-          if (a.atomic) atomicAdd(a.res + idx, acc[i][j][r]);
-          else a.res[idx] = acc[i][j][r];
-        }
-      }
+  // consecutive logical tiles run on one XCD (blocks are dealt round-robin)
+  const uint64_t t = (T % 8 == 0) ? (uint64_t)(b % 8) * (T / 8) + b / 8 : b;
+  uint32_t pt, qt, sp;
+  plan_decode(pl, t, pt, qt, sp);
+  const uint32_t kbeg = sp * pl.kchunk;
+  const uint32_t kend = min(a.kbytes, kbeg + pl.kchunk);
+  mfma_tile<P, Q, WP, KSB, DEPTH>(a, pt, qt, kbeg, kend, pl.splits > 1, lds);
 }
 
 // planes -> int8 values, 4 bytes per thread.  Zero outside the real matrix.
@@ -180,33 +68,42 @@ uint64_t max_abs(const Operand &o) {
   return (1ull << o.nbits) - 1;
 }
 
-template <int P, int Q>
-int run_mfma(MfmaArgs a, hipStream_t s) {
-  a.ptiles = (a.N + P - 1) / P;
-  a.qtiles = (a.M + Q - 1) / Q;
-  const uint64_t tiles = (uint64_t)a.ptiles * a.qtiles;
-  uint32_t splits = 1;
-  const uint32_t ksteps = a.kbytes / KS;
-  if (tiles < 512 && ksteps >= 16) {
-    uint64_t want = (512 + tiles - 1) / tiles;
-    uint64_t maxs = ksteps / 8;
-    splits = (uint32_t)(want < maxs ? want : maxs);
-    if (splits < 1) splits = 1;
-  }
-  uint32_t per = (ksteps + splits - 1) / splits;
-  splits = (ksteps + per - 1) / per;
-  a.kchunk = per * KS;
-  a.atomic = splits > 1;
-  if (a.atomic) {
+template <int P, int Q, int WP, int KSB, int DEPTH>
+int run_plan(const MfmaOperands &a, const MfmaPlan &pl, hipStream_t s) {
+  if (pl.splits > 1) {
     int rc = launch_fill_i32(a.res, (uint64_t)a.M * a.N, 0, s);
     if (rc) return rc;
   }
-  const uint64_t grid = tiles * splits;
+  const uint64_t grid = pl.tiles();
   if (grid == 0) return GBS_OK;
   if (grid > 0x7fffffffull) return fail(GBS_ERANGE, "mfma grid too large");
-  hipLaunchKernelGGL((mfma_kernel<P, Q>), dim3((uint32_t)grid), dim3(256), 0, s, a);
+  hipLaunchKernelGGL((mfma_kernel<P, Q, WP, KSB, DEPTH>), dim3((uint32_t)grid), dim3(256), 0,
+                     s, a, pl);
   GBS_HIP(hipGetLastError());
   return GBS_OK;
+}
+
+int run_plan_id(const MfmaOperands &a, const MfmaPlan &pl, hipStream_t s) {
+  switch (pl.shape) {
+    case 0: return run_plan<128, 128, 2, 128, 2>(a, pl, s);
+    case 1: return run_plan<64, 128, 2, 128, 2>(a, pl, s);
+    case 2: return run_plan<32, 256, 1, 64, 2>(a, pl, s);
+    case 3: return run_plan<16, 256, 1, 128, 2>(a, pl, s);
+    case 4: return run_plan<64, 64, 2, 128, 2>(a, pl, s);
+    case 5: return run_plan<128, 64, 4, 128, 2>(a, pl, s);
+    case 6: return run_plan<128, 128, 2, 64, 1>(a, pl, s);
+    case 7: return run_plan<128, 256, 2, 64, 2>(a, pl, s);
+  }
+  return fail(GBS_EINVAL, "mfma: unknown tile shape %d", pl.shape);
+}
+
+int g_shape[kClasses], g_splits[kClasses];
+bool g_table_set = false;
+
+int default_shape(uint64_t M, uint64_t N) {
+  (void)M;
+  (void)N;
+  return 4;  // 64 x 64: the best single shape over the tuner's classes
 }
 
 }  // namespace
@@ -231,8 +128,12 @@ gbs_algo select_algo(const Operand &l, const Operand &r, uint64_t K) {
   const double t_popc = (valu > pbytes / (0.6 * 8e12) ? valu : pbytes / (0.6 * 8e12));
   const double mops = 2 * M * N * Kd / (0.35 * 5e15);
   const double ibytes = (M + N) * Kd + 4 * M * N;
-  // the MFMA tile is 128 x 128: skinny products waste most of it
-  const double waste = (N < 128 ? 128 / (N > 1 ? N : 1) : 1) * (M < 128 ? 128 / (M > 1 ? M : 1) : 1);
+  // a tile row/column only partly filled still costs the whole tile
+  int shape_id, splits;
+  mfma_config_for(l.nrows, r.nrows, K, &shape_id, &splits);
+  const MfmaTileShape sh = kMfmaShape[shape_id];
+  const double waste = (N < sh.P ? sh.P / (N > 1 ? N : 1) : 1) *
+                       (M < sh.Q ? sh.Q / (M > 1 ? M : 1) : 1);
   const double t_mfma_c = mops * waste;
   const double t_mfma = (t_mfma_c > ibytes / (0.6 * 8e12) ? t_mfma_c : ibytes / (0.6 * 8e12));
   return t_mfma < t_popc ? GBS_ALGO_MFMA : GBS_ALGO_POPCOUNT;
@@ -260,14 +161,14 @@ int ensure_vals(gbs_matrix *m, hipStream_t s, bool may_allocate) {
 }
 
 int launch_mfma_gemm(const gbs_matrix *l, const gbs_matrix *r, int32_t *d_res,
-                     hipStream_t s) {
+                     hipStream_t s, int forced_shape, bool allow_split, int forced_splits) {
   const uint64_t M = l->nrows, N = r->nrows;
   if (M == 0 || N == 0) return GBS_OK;
   if (M > 0x7fffffffull || N > 0x7fffffffull)
     return fail(GBS_ERANGE, "gemm dimensions exceed 2^31");
   if (l->vals_ld != r->vals_ld || !l->d_vals || !r->d_vals)
     return fail(GBS_EINVAL, "mfma: operand int8 copies are not current");
-  MfmaArgs a{};
+  MfmaOperands a{};
   a.R = r->d_vals;
   a.L = l->d_vals;
   a.r_ld = r->vals_ld;
@@ -275,9 +176,69 @@ int launch_mfma_gemm(const gbs_matrix *l, const gbs_matrix *r, int32_t *d_res,
   a.res = d_res;
   a.M = (uint32_t)M;
   a.N = (uint32_t)N;
-  a.kbytes = (uint32_t)((l->ncols + KS - 1) / KS * KS);
+  a.kbytes = (uint32_t)((l->ncols + kMfmaKS - 1) / kMfmaKS * kMfmaKS);
   if (a.kbytes == 0) return launch_fill_i32(d_res, M * N, 0, s);
-  return run_mfma<128, 128>(a, s);
+  const MfmaPlan pl = mfma_plan(a.M, a.N, a.kbytes, forced_shape,
+                                forced_splits > 0 ? forced_splits : (allow_split ? 0 : 1));
+  return run_plan_id(a, pl, s);
+}
+
+void mfma_config_for(uint64_t M, uint64_t N, uint64_t K, int *shape, int *splits) {
+  *shape = default_shape(M, N);
+  *splits = 0;
+  if (g_table_set) {
+    const int c = shape_class(M, N, K);
+    if (g_shape[c] >= 0 && g_shape[c] < kMfmaShapes) {
+      *shape = g_shape[c];
+      *splits = g_splits[c];
+    }
+  }
+}
+
+MfmaPlan mfma_plan(uint32_t M, uint32_t N, uint32_t kbytes, int forced_shape,
+                   int forced_splits) {
+  int shape, splits;
+  mfma_config_for(M, N, kbytes, &shape, &splits);
+  if (forced_shape >= 0) {
+    shape = forced_shape;
+    splits = 0;
+  }
+  if (forced_splits > 0) splits = forced_splits;
+  const MfmaTileShape sh = kMfmaShape[shape];
+  MfmaPlan pl{};
+  pl.shape = shape;
+  pl.ptiles = (N + sh.P - 1) / sh.P;
+  pl.qtiles = (M + sh.Q - 1) / sh.Q;
+  // XCD groups share the larger operand: lhs larger -> rhs tiles fastest
+  pl.order = (uint64_t)M > (uint64_t)N ? 1 : 0;
+  const uint32_t ksteps = kbytes / kMfmaKS;  // 128-byte units
+  const uint64_t tiles = (uint64_t)pl.ptiles * pl.qtiles;
+  if (splits <= 0) {
+    // fill two blocks per CU, keeping at least 8 units of K per split
+    uint64_t want = tiles >= 512 ? 1 : (512 + tiles - 1) / tiles;
+    uint64_t maxs = ksteps / 8 ? ksteps / 8 : 1;
+    splits = (int)(want < maxs ? want : maxs);
+  }
+  if (splits < 1) splits = 1;
+  if ((uint32_t)splits > (ksteps ? ksteps : 1)) splits = ksteps ? ksteps : 1;
+  const uint32_t per = ksteps ? (ksteps + splits - 1) / splits : 1;
+  pl.splits = ksteps ? (ksteps + per - 1) / per : 1;
+  pl.kchunk = per * kMfmaKS;
+  return pl;
+}
+
+int set_mfma_table(const int *shapes, const int *splits, int n) {
+  if (!shapes) {
+    g_table_set = false;
+    return GBS_OK;
+  }
+  if (n != kClasses) return fail(GBS_EINVAL, "tile table needs %d entries, got %d", kClasses, n);
+  for (int i = 0; i < kClasses; i++) {
+    g_shape[i] = shapes[i];
+    g_splits[i] = splits ? splits[i] : 0;
+  }
+  g_table_set = true;
+  return GBS_OK;
 }
 
 }  // namespace gbs

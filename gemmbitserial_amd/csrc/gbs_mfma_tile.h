@@ -1,0 +1,249 @@
+// gbs_mfma_tile.h -- one output tile of the exact int8 MFMA GEMM, shared by
+// the single-product kernel (gbs_mfma.hip) and the batched persistent kernel
+// (gbs_batch.hip).
+//
+// Tile = P rhs rows (n) x Q lhs rows (m) of res; 256 threads = 4 waves laid
+// out WP x (4/WP) over it; each wave owns (P/WP) x (Q/(4/WP)) outputs as
+// 16 x 16 fragments of v_mfma_i32_16x16x64_i8.  The rhs operand goes to the
+// MFMA A port so that a fragment's output lanes run along m, the contiguous
+// dimension of res.  Both operand fragments take 16 contiguous bytes of a
+// row at k-offset 16*(lane/16) -- the same k-to-(lane, byte) assignment on
+// both ports, so the sum over k does not depend on the order the
+// instruction adds in.  K is staged KSB = 64 or 128 bytes at a time in LDS
+// (row stride KSB + 16 B: the 16 rows one fragment load touches fall in
+// distinct banks), double-buffered in LDS, with 16-byte global loads held in
+// registers one or two stages ahead of the stage being multiplied.
+#pragma once
+
+#include <type_traits>
+
+#include "gbs_internal.h"
+
+namespace gbs {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+constexpr int kMfmaKS = 128;  // padding unit of K (bytes): every stage size divides it
+
+// Operands of one product: int8 value copies [rows >= tile-aligned][ld].
+struct MfmaOperands {
+  const int8_t *R, *L;   // rhs / lhs values
+  uint64_t r_ld, l_ld;   // row strides (bytes)
+  int32_t *res;          // N x M
+  uint32_t M, N;         // lhs rows, rhs rows
+  uint32_t kbytes;       // depth rounded up to kMfmaKS (zero padded)
+};
+
+// Tile shapes the engine is compiled for: P x Q outputs, WP waves along P,
+// KSB bytes of K per LDS stage, DEPTH stages of global loads in flight.
+struct MfmaTileShape { int P, Q, WP, KSB, DEPTH; };
+constexpr int kMfmaShapes = 8;
+constexpr MfmaTileShape kMfmaShape[kMfmaShapes] = {
+    {128, 128, 2, 128, 2}, {64, 128, 2, 128, 2}, {32, 256, 1, 64, 2},
+    {16, 256, 1, 128, 2},  {64, 64, 2, 128, 2},  {128, 64, 4, 128, 2},
+    {128, 128, 2, 64, 1},  {128, 256, 2, 64, 2}};
+
+template <int P, int Q, int KSB>
+constexpr size_t mfma_lds_bytes() { return (size_t)2 * (P + Q) * (KSB + 16); }
+constexpr size_t kMfmaLdsMax = (size_t)2 * 272 * (128 + 16);  // largest over the shapes (16 x 256)
+
+template <int P, int Q, int WP, int KSB, int DEPTH>
+__device__ __forceinline__ void mfma_tile(const MfmaOperands &a, uint32_t pt, uint32_t qt,
+                                          uint32_t kbeg, uint32_t kend, bool atomic,
+                                          int8_t *lds) {
+  constexpr int WQ = 4 / WP;
+  constexpr int PW = P / WP, QW = Q / WQ;       // wave tile
+  constexpr int FP = PW / 16, FQ = QW / 16;     // fragments per wave
+  constexpr int LDR = KSB + 16;                 // LDS row stride (bytes)
+  constexpr int CPR = KSB / 16;                 // 16-byte chunks per row and stage
+  constexpr int RCH = P * CPR, LCH = Q * CPR;   // chunks per stage
+  constexpr int RLD = (RCH + 255) / 256, LLD = (LCH + 255) / 256;
+  static_assert(FP >= 1 && FQ >= 1 && PW % 16 == 0 && QW % 16 == 0, "bad tile");
+  static_assert(KSB % 64 == 0 && kMfmaKS % KSB == 0, "bad stage size");
+  static_assert(DEPTH == 1 || DEPTH == 2, "prefetch depth");
+  static_assert(mfma_lds_bytes<P, Q, KSB>() <= kMfmaLdsMax, "LDS");
+  int8_t *sR0 = lds, *sL0 = lds + P * LDR;
+  constexpr int stage = (P + Q) * LDR;
+
+  const uint32_t p0 = pt * P, q0 = qt * Q;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wp = wave % WP, wq = wave / WP;
+
+  v4i gr[DEPTH][RLD], gl[DEPTH][LLD];
+  auto gload = [&](v4i(&r)[RLD], v4i(&l)[LLD], uint32_t k0) {
+#pragma unroll
+    for (int i = 0; i < RLD; i++) {
+      const int c = tid + 256 * i, row = c / CPR, ch = c % CPR;
+      if (RCH % 256 == 0 || c < RCH)
+        r[i] = *reinterpret_cast<const v4i *>(a.R + (uint64_t)(p0 + row) * a.r_ld + k0 + ch * 16);
+    }
+#pragma unroll
+    for (int i = 0; i < LLD; i++) {
+      const int c = tid + 256 * i, row = c / CPR, ch = c % CPR;
+      if (LCH % 256 == 0 || c < LCH)
+        l[i] = *reinterpret_cast<const v4i *>(a.L + (uint64_t)(q0 + row) * a.l_ld + k0 + ch * 16);
+    }
+  };
+  auto swrite = [&](const v4i(&r)[RLD], const v4i(&l)[LLD], int buf) {
+    int8_t *sR = sR0 + buf * stage, *sL = sL0 + buf * stage;
+#pragma unroll
+    for (int i = 0; i < RLD; i++) {
+      const int c = tid + 256 * i, row = c / CPR, ch = c % CPR;
+      if (RCH % 256 == 0 || c < RCH) *reinterpret_cast<v4i *>(&sR[row * LDR + ch * 16]) = r[i];
+    }
+#pragma unroll
+    for (int i = 0; i < LLD; i++) {
+      const int c = tid + 256 * i, row = c / CPR, ch = c % CPR;
+      if (LCH % 256 == 0 || c < LCH) *reinterpret_cast<v4i *>(&sL[row * LDR + ch * 16]) = l[i];
+    }
+  };
+
+  v4i acc[FP][FQ];
+#pragma unroll
+  for (int i = 0; i < FP; i++)
+#pragma unroll
+    for (int j = 0; j < FQ; j++) acc[i][j] = v4i{0, 0, 0, 0};
+
+  const int fr = lane & 15, fk = (lane >> 4) * 16;
+  auto compute = [&](int buf) {
+    const int8_t *sR = sR0 + buf * stage, *sL = sL0 + buf * stage;
+#pragma unroll
+    for (int kk = 0; kk < KSB / 64; kk++) {
+      v4i af[FP], bf[FQ];
+#pragma unroll
+      for (int i = 0; i < FP; i++)
+        af[i] = *reinterpret_cast<const v4i *>(&sR[(wp * PW + i * 16 + fr) * LDR + kk * 64 + fk]);
+#pragma unroll
+      for (int j = 0; j < FQ; j++)
+        bf[j] = *reinterpret_cast<const v4i *>(&sL[(wq * QW + j * 16 + fr) * LDR + kk * 64 + fk]);
+#pragma unroll
+      for (int i = 0; i < FP; i++)
+#pragma unroll
+        for (int j = 0; j < FQ; j++)
+          acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
+    }
+  };
+
+  const uint32_t nst = kend > kbeg ? (kend - kbeg) / KSB : 0;  // stages
+  if (DEPTH == 1) {
+    if (nst) {
+      gload(gr[0], gl[0], kbeg);
+      swrite(gr[0], gl[0], 0);
+    }
+    __syncthreads();
+    for (uint32_t i = 0; i < nst; i++) {
+      const bool more = i + 1 < nst;
+      if (more) gload(gr[0], gl[0], kbeg + (i + 1) * KSB);
+      compute(i & 1);
+      if (more) swrite(gr[0], gl[0], (i + 1) & 1);
+      __syncthreads();
+    }
+  } else {
+    // stage i sits in LDS buffer i%2 and, before that, in register set i%2;
+    // loads run two stages ahead.  The loop is unrolled by two so that the
+    // register sets are indexed statically.
+    if (nst) gload(gr[0], gl[0], kbeg);
+    if (nst > 1) gload(gr[DEPTH - 1], gl[DEPTH - 1], kbeg + KSB);
+    if (nst) swrite(gr[0], gl[0], 0);
+    __syncthreads();
+    auto step = [&](uint32_t i, auto cur) {
+      constexpr int c = decltype(cur)::value, n = c ^ 1;
+      if (i + 2 < nst) gload(gr[c], gl[c], kbeg + (i + 2) * KSB);
+      compute(c);
+      if (i + 1 < nst) swrite(gr[n], gl[n], n);
+      __syncthreads();
+    };
+    for (uint32_t i = 0; i < nst; i += 2) {
+      step(i, std::integral_constant<int, 0>());
+      if (i + 1 < nst) step(i + 1, std::integral_constant<int, DEPTH - 1>());
+    }
+  }
+
+  // fragment output: lane holds rows 4*(lane/16)+r (rhs, n) of column
+  // lane%16 (lhs, m)
+#pragma unroll
+  for (int i = 0; i < FP; i++)
+#pragma unroll
+    for (int j = 0; j < FQ; j++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const uint32_t n = p0 + wp * PW + i * 16 + 4 * (lane >> 4) + r;
+        const uint32_t m = q0 + wq * QW + j * 16 + fr;
+        if (n < a.N && m < a.M) {
+          const uint64_t idx = (uint64_t)n * a.M + m;
+          if (atomic) atomicAdd(a.res + idx, acc[i][j][r]);
+          else a.res[idx] = acc[i][j][r];
+        }
+      }
+}
+
+// Shapes the persistent kernel carries (all but the 128 x 256 one, whose
+// registers would make the shared kernel spill); others map to shape 0.
+__host__ __device__ constexpr int batch_shape(int s) { return s == 7 ? 0 : s; }
+
+// Dispatch one tile by shape id (the persistent kernel's per-product switch).
+__device__ __forceinline__ void mfma_tile_by_shape(int shape, const MfmaOperands &a,
+                                                   uint32_t pt, uint32_t qt, uint32_t kbeg,
+                                                   uint32_t kend, bool atomic, int8_t *lds) {
+  switch (shape) {
+    case 0: mfma_tile<128, 128, 2, 128, 2>(a, pt, qt, kbeg, kend, atomic, lds); break;
+    case 1: mfma_tile<64, 128, 2, 128, 2>(a, pt, qt, kbeg, kend, atomic, lds); break;
+    case 2: mfma_tile<32, 256, 1, 64, 2>(a, pt, qt, kbeg, kend, atomic, lds); break;
+    case 3: mfma_tile<16, 256, 1, 128, 2>(a, pt, qt, kbeg, kend, atomic, lds); break;
+    case 4: mfma_tile<64, 64, 2, 128, 2>(a, pt, qt, kbeg, kend, atomic, lds); break;
+    case 5: mfma_tile<128, 64, 4, 128, 2>(a, pt, qt, kbeg, kend, atomic, lds); break;
+    default: mfma_tile<128, 128, 2, 64, 1>(a, pt, qt, kbeg, kend, atomic, lds); break;
+  }
+}
+
+// Launch geometry of one product: tile shape, K split and tile order.  The
+// tiles of a product are numbered split-major; inside a split, the tiles
+// that share the larger operand are consecutive (order 0: lhs tiles
+// fastest, so consecutive tiles share an rhs tile; order 1: rhs tiles
+// fastest), and consecutive tiles land on one XCD, whose L2 then holds that
+// operand's rows.
+struct MfmaPlan {
+  int shape;
+  uint32_t splits, kchunk, ptiles, qtiles, order;
+  uint64_t tiles() const { return (uint64_t)ptiles * qtiles * splits; }
+};
+
+__host__ __device__ inline void plan_decode(const MfmaPlan &pl, uint64_t t, uint32_t &pt,
+                                            uint32_t &qt, uint32_t &sp) {
+  const uint64_t per = (uint64_t)pl.ptiles * pl.qtiles;
+  sp = (uint32_t)(t / per);
+  const uint64_t r = t - (uint64_t)sp * per;
+  if (pl.order == 0) {
+    qt = (uint32_t)(r % pl.qtiles);
+    pt = (uint32_t)(r / pl.qtiles);
+  } else {
+    pt = (uint32_t)(r % pl.ptiles);
+    qt = (uint32_t)(r / pl.ptiles);
+  }
+}
+
+// Plan for a product (uses the tuned table; see mfma_config_for).
+MfmaPlan mfma_plan(uint32_t M, uint32_t N, uint32_t kbytes, int forced_shape = -1,
+                   int forced_splits = 0);
+
+// Shape class of a product, for the tile autotuner's table: buckets of
+// log2 of M, N and K.  kClassM x kClassN x kClassK classes.
+constexpr int kClassM = 4, kClassN = 5, kClassK = 3;
+constexpr int kClasses = kClassM * kClassN * kClassK;
+inline int shape_class(uint64_t M, uint64_t N, uint64_t K) {
+  const int cm = M < 256 ? 0 : M < 1024 ? 1 : M < 4096 ? 2 : 3;
+  const int cn = N <= 16 ? 0 : N <= 32 ? 1 : N <= 64 ? 2 : N <= 256 ? 3 : 4;
+  const int ck = K < 1024 ? 0 : K < 4096 ? 1 : 2;
+  return (cm * kClassN + cn) * kClassK + ck;
+}
+
+// Tile shape and K split chosen for a class (table set by gbs_tune_load;
+// else a size heuristic).  splits 0 = decide from the tile count.
+void mfma_config_for(uint64_t M, uint64_t N, uint64_t K, int *shape, int *splits);
+
+// Persistent launch over products given as operand views (gbs_batch.hip).
+int launch_views(const MfmaOperands *views, int n, void *host_desc, void *d_desc,
+                 unsigned int *d_counter, hipStream_t s);
+size_t batch_desc_bytes(int n);
+
+}  // namespace gbs

@@ -218,7 +218,15 @@ __global__ __launch_bounds__(256) void sum_rows_kernel(
 __global__ void fill_i32_kernel(int32_t *d, uint64_t n, int32_t v) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t j = i; j < n; j += stride) d[j] = v;
+  // 16-byte stores over the aligned body, scalar stores for the ends
+  const uint64_t head = ((16 - ((uintptr_t)d & 15)) & 15) / 4;
+  const uint64_t h = head < n ? head : n;
+  const uint64_t nv = (n - h) / 4;
+  int4 *dv = reinterpret_cast<int4 *>(d + h);
+  const int4 vv = make_int4(v, v, v, v);
+  for (uint64_t j = i; j < nv; j += stride) dv[j] = vv;
+  for (uint64_t j = i; j < h; j += stride) d[j] = v;
+  for (uint64_t j = h + 4 * nv + i; j < n; j += stride) d[j] = v;
 }
 
 }  // namespace
@@ -292,8 +300,8 @@ int launch_sum_rows(const Operand &o, uint64_t kwords, uint64_t ncols,
 
 int launch_fill_i32(int32_t *d, uint64_t n, int32_t v, hipStream_t s) {
   if (n == 0) return GBS_OK;
-  uint64_t blocks = (n + 255) / 256;
-  if (blocks > 4096) blocks = 4096;
+  uint64_t blocks = (n / 4 + 255) / 256 + 1;
+  if (blocks > 2048) blocks = 2048;
   hipLaunchKernelGGL(fill_i32_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, d, n, v);
   GBS_HIP(hipGetLastError());
   return GBS_OK;

@@ -1,14 +1,16 @@
 // gbs_runtime.hip -- the C ABI of libgbs_hip.so (include/gbs_hip.h):
 // device selection, per-thread streams, matrices, workspaces and the
 // synchronous / asynchronous GEMM entry points.
+#include <dlfcn.h>
 #include <stdarg.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <mutex>
 #include <string>
 
-#include "gbs_internal.h"
+#include "gbs_mfma_tile.h"
 
 namespace gbs {
 
@@ -35,7 +37,33 @@ int hip_fail(hipError_t e, const char *what) {
               hipGetErrorString(e));
 }
 
+// The tuned tile table: $GBS_TUNE_FILE, else tune_table.txt beside the
+// library; the built-in heuristic when neither exists.
+static void load_default_table() {
+  static std::once_flag once;
+  std::call_once(once, [] {
+    std::string path;
+    if (const char *env = getenv("GBS_TUNE_FILE")) {
+      path = env;
+    } else {
+      Dl_info info;
+      if (dladdr((void *)&load_default_table, &info) && info.dli_fname) {
+        path = info.dli_fname;
+        const size_t slash = path.rfind('/');
+        path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) +
+               "/tune_table.txt";
+      }
+    }
+    FILE *f = path.empty() ? nullptr : fopen(path.c_str(), "r");
+    if (!f) return;
+    fclose(f);
+    const std::string saved = t_err;
+    if (gbs_tune_load(path.c_str()) != GBS_OK) t_err = saved;
+  });
+}
+
 int ensure_device() {
+  load_default_table();
   if (t_device < 0) {
     int n = 0;
     hipError_t e = hipGetDeviceCount(&n);
@@ -82,8 +110,8 @@ static int check_matrix(const gbs_matrix *m, const char *fn) {
   return GBS_OK;
 }
 
-// Shared body of gbs_gemm / gbs_gemm_async.
-static int gemm_enqueue(gbs_workspace *ws, const gbs_matrix *l, const gbs_matrix *r,
+// Shared body of gbs_gemm / gbs_gemm_async / the batch fallbacks.
+int gemm_enqueue(gbs_workspace *ws, const gbs_matrix *l, const gbs_matrix *r,
                         int32_t *d_res, gbs_algo algo, hipStream_t s,
                         bool may_allocate) {
   if (l->ncols != r->ncols)
@@ -123,6 +151,11 @@ static int gemm_enqueue(gbs_workspace *ws, const gbs_matrix *l, const gbs_matrix
   return launch_popc_gemm(lo, ro, kwords, K, d_res, d_rowsum, s);
 }
 
+int gemm_enqueue_public(gbs_workspace *ws, const gbs_matrix *l, const gbs_matrix *r,
+                        int32_t *d_res, gbs_algo algo, hipStream_t s, bool may_allocate) {
+  return gemm_enqueue(ws, l, r, d_res, algo, s, may_allocate);
+}
+
 }  // namespace gbs
 
 using namespace gbs;
@@ -130,6 +163,7 @@ using namespace gbs;
 extern "C" {
 
 int gbs_init(int device) {
+  load_default_table();
   int n = 0;
   hipError_t e = hipGetDeviceCount(&n);
   if (e != hipSuccess || n == 0)
@@ -487,6 +521,67 @@ int gbs_event_elapsed_ms(void *start, void *stop, float *ms) {
   GBS_HIP(hipEventSynchronize((hipEvent_t)stop));
   GBS_HIP(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
   return GBS_OK;
+}
+
+int gbs_gemm_async_config(gbs_workspace *ws, const gbs_matrix *lhs, const gbs_matrix *rhs,
+                          int32_t *d_res, int shape, int splits, void *stream) {
+  int rc = check_matrix(lhs, "gbs_gemm_async_config");
+  if (!rc) rc = check_matrix(rhs, "gbs_gemm_async_config");
+  if (rc) return rc;
+  (void)ws;
+  if (lhs->ncols != rhs->ncols) return fail(GBS_EINVAL, "gbs_gemm_async_config: depths differ");
+  if (!mfma_supported(view(lhs), view(rhs), lhs->ncols))
+    return fail(GBS_ENOTSUP, "gbs_gemm_async_config: not exact on the MFMA engine");
+  if (shape < 0 || shape >= kMfmaShapes || splits < 1)
+    return fail(GBS_EINVAL, "gbs_gemm_async_config: shape %d / splits %d out of range", shape,
+                splits);
+  if ((rc = ensure_device())) return rc;
+  hipStream_t s = stream ? (hipStream_t)stream : thread_stream();
+  if ((rc = ensure_vals(const_cast<gbs_matrix *>(lhs), s, false))) return rc;
+  if ((rc = ensure_vals(const_cast<gbs_matrix *>(rhs), s, false))) return rc;
+  return launch_mfma_gemm(lhs, rhs, d_res, s, shape, true, splits > 0 ? splits : 1);
+}
+
+int gbs_tune_classes(int *n_classes, int *n_shapes) {
+  if (n_classes) *n_classes = kClasses;
+  if (n_shapes) *n_shapes = kMfmaShapes;
+  return GBS_OK;
+}
+
+int gbs_tune_class_of(uint64_t M, uint64_t N, uint64_t K, int *cls) {
+  if (!cls) return fail(GBS_EINVAL, "gbs_tune_class_of: null out");
+  *cls = shape_class(M, N, K);
+  return GBS_OK;
+}
+
+int gbs_tune_set(const int *shapes, const int *splits, int n) {
+  return set_mfma_table(shapes, splits, n);
+}
+
+int gbs_tune_load(const char *path) {
+  if (!path) return fail(GBS_EINVAL, "gbs_tune_load: null path");
+  FILE *f = fopen(path, "r");
+  if (!f) return fail(GBS_EINVAL, "gbs_tune_load: cannot open %s", path);
+  int shapes[kClasses], splits[kClasses];
+  for (int i = 0; i < kClasses; i++) {
+    shapes[i] = -1;
+    splits[i] = 0;
+  }
+  char line[512];
+  int n = 0;
+  while (fgets(line, sizeof(line), f)) {
+    int c, sh, sp = 0;
+    if (line[0] == '#') continue;
+    const int got = sscanf(line, "%d %d %d", &c, &sh, &sp);
+    if (got >= 2 && c >= 0 && c < kClasses && sh >= 0 && sh < kMfmaShapes) {
+      shapes[c] = sh;
+      splits[c] = (got == 3 && sp > 0) ? sp : 0;
+      n++;
+    }
+  }
+  fclose(f);
+  if (n == 0) return fail(GBS_EINVAL, "gbs_tune_load: no entries in %s", path);
+  return set_mfma_table(shapes, splits, kClasses);
 }
 
 }  // extern "C"

@@ -1,0 +1,422 @@
+// gbs_serve.hip -- continuous-batching serving loop for a stack of
+// bit-serial layers with weights resident in HBM.
+//
+// A server holds L weight matrices W_l (h_{l+1} x h_l, bit-planes plus their
+// int8 copy) and, per layer, an activation pool: bit-planes and int8 values
+// of up to R rows of width h_l.  Requests (1..max_rows rows of act_bits-bit
+// unsigned activations) wait in a FIFO.  Every step:
+//
+//   1. admission: waiting requests join a new cohort while its rows fit the
+//      pool capacity R (set from the memory budget at creation) and the
+//      number of requests in flight stays under max_requests; the cohort's
+//      inputs are packed on the GPU into pool 0;
+//   2. one persistent launch computes, for every cohort in flight (cohort l
+//      sits at layer l), acc_l = pool_l * W_l^T -- all layers at once, so
+//      requests that arrived at different times share the device;
+//   3. one launch requantizes acc_l -> min(2^act_bits - 1, max(0, acc >> s_l))
+//      and packs it (one __ballot per plane word, plus the int8 copy) into
+//      pool l+1, where cohort l sits next step;
+//   4. the last layer's int32 rows are copied back; those requests finish.
+//
+// Rows of different requests never mix inside a GEMM row, and integer
+// arithmetic is exact, so a request's result is the same alone or in any
+// mix.  A server is driven by one host thread.
+#include <string.h>
+
+#include <chrono>
+#include <algorithm>
+#include <deque>
+#include <map>
+#include <vector>
+
+#include "gbs_mfma_tile.h"
+
+namespace {
+
+int64_t now_ns() {
+  return std::chrono::duration_cast<std::chrono::nanoseconds>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+struct Req {
+  uint64_t id;
+  uint32_t rows;
+  uint32_t off;  // first row in its cohort's pool
+  int64_t arrival, done;
+  std::vector<uint8_t> x;
+  std::vector<int32_t> y;
+};
+
+constexpr int kMaxLayers = 16;
+
+// One activation pack job: rows [0, rows) of a source into a pool.
+struct ActJob {
+  const uint8_t *in;     // mode 0: input codes, rows x h (row-major)
+  const int32_t *acc;    // mode 1: int32 accumulators, rows x h
+  uint64_t *planes;      // pool planes [nbits][R][wpr]
+  int8_t *vals;          // pool int8 copy [.][ld]
+  uint64_t wpr, wpp, ld;
+  uint32_t rows, h, mode, shift;
+  uint64_t unit_begin;   // prefix of rows * wpr over the jobs
+};
+
+struct ActJobs {
+  ActJob j[kMaxLayers];
+  int n;
+  uint32_t nbits, maxv;
+};
+
+// One wave per (job, row, plane word): lane = column of that word.
+__global__ __launch_bounds__(256) void act_pack_kernel(ActJobs J, uint64_t units) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t u = (uint64_t)blockIdx.x * 4 + threadIdx.x / 64;
+  if (u >= units) return;
+  int k = 0;
+  while (k + 1 < J.n && J.j[k + 1].unit_begin <= u) k++;
+  const ActJob &jb = J.j[k];
+  const uint64_t local = u - jb.unit_begin;
+  const uint64_t r = local / jb.wpr, w = local - r * jb.wpr;
+  const uint64_t c = w * 64 + lane;
+  uint32_t code = 0;
+  if (c < jb.h) {
+    if (jb.mode == 0) {
+      code = jb.in[r * jb.h + c] & J.maxv;
+    } else {
+      const int32_t a = jb.acc[r * jb.h + c] >> jb.shift;  // arithmetic shift
+      code = a < 0 ? 0u : (a > (int32_t)J.maxv ? J.maxv : (uint32_t)a);
+    }
+    jb.vals[r * jb.ld + c] = (int8_t)code;
+  }
+  for (uint32_t b = 0; b < J.nbits; b++) {
+    const uint64_t m = __ballot((code >> b) & 1u);
+    if (lane == 0) jb.planes[b * jb.wpp + r * jb.wpr + w] = m;
+  }
+}
+
+}  // namespace
+
+struct gbs_server {
+  uint32_t L = 0, act_bits = 0, R = 0, max_requests = 0, max_rows = 0;
+  std::vector<const gbs_matrix *> W;
+  std::vector<uint32_t> h, shift;
+  std::vector<gbs_matrix *> pool;
+  std::vector<int32_t *> acc;
+  void *h_desc = nullptr, *d_desc = nullptr;
+  unsigned int *d_counter = nullptr;
+  uint8_t *h_in = nullptr, *d_in = nullptr;
+  int32_t *h_out = nullptr;
+  uint64_t bytes_reserved = 0;
+  std::deque<Req *> waiting;
+  std::vector<std::vector<Req *>> cohort;  // cohort[l]: requests at layer l
+  std::vector<uint32_t> cohort_rows;
+  std::map<uint64_t, Req *> finished;
+  uint64_t steps = 0, done_requests = 0, done_rows = 0;
+  double ops_done = 0;
+};
+
+using namespace gbs;
+
+static void server_free(gbs_server *s) {
+  if (!s) return;
+  (void)hipDeviceSynchronize();
+  for (gbs_matrix *m : s->pool) gbs_matrix_destroy(m);
+  for (int32_t *a : s->acc) (void)hipFree(a);
+  if (s->d_desc) (void)hipFree(s->d_desc);
+  if (s->d_counter) (void)hipFree(s->d_counter);
+  if (s->d_in) (void)hipFree(s->d_in);
+  if (s->h_desc) (void)hipHostFree(s->h_desc);
+  if (s->h_in) (void)hipHostFree(s->h_in);
+  if (s->h_out) (void)hipHostFree(s->h_out);
+  for (Req *r : s->waiting) delete r;
+  for (auto &c : s->cohort)
+    for (Req *r : c) delete r;
+  for (auto &kv : s->finished) delete kv.second;
+  delete s;
+}
+
+static uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
+
+extern "C" {
+
+int gbs_clock_ns(int64_t *now) {
+  if (!now) return fail(GBS_EINVAL, "gbs_clock_ns: null out");
+  *now = now_ns();
+  return GBS_OK;
+}
+
+int gbs_server_create(const gbs_matrix *const *weights, const uint32_t *shifts,
+                      uint32_t n_layers, uint32_t act_bits, const gbs_server_config *cfg,
+                      gbs_server **out) {
+  if (!out) return fail(GBS_EINVAL, "gbs_server_create: null out");
+  *out = nullptr;
+  if (!weights || !cfg || n_layers == 0 || n_layers > kMaxLayers)
+    return fail(GBS_EINVAL, "gbs_server_create: need 1..%d layers and a config", kMaxLayers);
+  if (act_bits < 1 || act_bits > 7)
+    return fail(GBS_EINVAL, "gbs_server_create: act_bits %u outside [1, 7]", act_bits);
+  if (n_layers > 1 && !shifts) return fail(GBS_EINVAL, "gbs_server_create: null shifts");
+  if (cfg->max_rows == 0 || cfg->max_requests == 0)
+    return fail(GBS_EINVAL, "gbs_server_create: max_rows and max_requests must be > 0");
+  int rc = ensure_device();
+  if (rc) return rc;
+  hipStream_t st = thread_stream();
+  gbs_server *s = new gbs_server();
+  s->L = n_layers;
+  s->act_bits = act_bits;
+  s->max_requests = cfg->max_requests;
+  s->max_rows = cfg->max_rows;
+  s->h.push_back((uint32_t)weights[0]->ncols);
+  for (uint32_t l = 0; l < n_layers; l++) {
+    const gbs_matrix *w = weights[l];
+    if (!w || w->ncols != s->h[l]) {
+      server_free(s);
+      return fail(GBS_EINVAL, "gbs_server_create: layer %u width does not chain", l);
+    }
+    const Operand wo = view(w);
+    Operand ao{};
+    ao.nbits = act_bits;
+    if (!mfma_supported(wo, ao, w->ncols)) {
+      server_free(s);
+      return fail(GBS_ENOTSUP, "gbs_server_create: layer %u not exact on the int8 engine", l);
+    }
+    if ((rc = ensure_vals(const_cast<gbs_matrix *>(w), st, true))) {
+      server_free(s);
+      return rc;
+    }
+    s->W.push_back(w);
+    s->h.push_back((uint32_t)w->nrows);
+    s->shift.push_back(l + 1 < n_layers ? shifts[l] : 0);
+  }
+  // bytes one row of capacity costs across every pool and buffer
+  uint64_t per_row = s->h[0] + 4ull * s->h[n_layers];  // input / output staging
+  for (uint32_t l = 0; l < n_layers; l++)
+    per_row += act_bits * align_up(s->h[l], 64) / 8 + align_up(s->h[l], kValsColAlign) +
+               4ull * s->h[l + 1];
+  uint64_t budget = cfg->mem_budget;
+  if (budget == 0) {
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) fr = 1ull << 30;
+    budget = fr / 4;
+  }
+  uint64_t R = budget / per_row;
+  if (R > 65536) R = 65536;
+  if (R < cfg->max_rows) {
+    server_free(s);
+    return fail(GBS_ERANGE,
+                "gbs_server_create: memory budget %llu B holds %llu rows, less than one "
+                "request of max_rows %u",
+                (unsigned long long)budget, (unsigned long long)R, cfg->max_rows);
+  }
+  s->R = (uint32_t)R;
+  s->cohort.resize(n_layers);
+  s->cohort_rows.assign(n_layers, 0);
+  const uint64_t Ra = align_up(R, kValsRowAlign);
+  for (uint32_t l = 0; l < n_layers && !rc; l++) {
+    gbs_matrix *p = nullptr;
+    rc = gbs_matrix_create(act_bits, R, s->h[l], 0, R, align_up(s->h[l], 64), &p);
+    if (rc) break;
+    s->pool.push_back(p);
+    const uint64_t ld = align_up(s->h[l], kValsColAlign);
+    hipError_t e = hipMalloc((void **)&p->d_vals, Ra * ld);
+    if (e == hipSuccess) e = hipMemset(p->d_vals, 0, Ra * ld);
+    if (e != hipSuccess) {
+      rc = hip_fail(e, "gbs_server_create: pool");
+      break;
+    }
+    p->vals_rows = Ra;
+    p->vals_ld = ld;
+    p->vals_version = p->planes_version;  // kept current by act_pack_kernel
+    int32_t *a = nullptr;
+    e = hipMalloc((void **)&a, R * s->h[l + 1] * 4ull);
+    if (e != hipSuccess) {
+      rc = hip_fail(e, "gbs_server_create: accumulators");
+      break;
+    }
+    s->acc.push_back(a);
+  }
+  if (!rc) {
+    hipError_t e = hipMalloc(&s->d_desc, batch_desc_bytes(kMaxLayers));
+    if (e == hipSuccess) e = hipMalloc((void **)&s->d_counter, 256);
+    if (e == hipSuccess) e = hipMalloc((void **)&s->d_in, R * s->h[0] + 16);
+    if (e == hipSuccess) e = hipHostMalloc(&s->h_desc, batch_desc_bytes(kMaxLayers));
+    if (e == hipSuccess) e = hipHostMalloc((void **)&s->h_in, R * s->h[0] + 16);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&s->h_out, R * s->h[n_layers] * 4ull + 16);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) rc = hip_fail(e, "gbs_server_create: buffers");
+  }
+  if (rc) {
+    server_free(s);
+    return rc;
+  }
+  s->bytes_reserved = R * per_row;
+  *out = s;
+  return GBS_OK;
+}
+
+int gbs_server_destroy(gbs_server *s) {
+  if (!s) return GBS_OK;
+  int rc = ensure_device();
+  if (rc) return rc;
+  server_free(s);
+  return GBS_OK;
+}
+
+int gbs_server_submit(gbs_server *s, uint64_t id, const uint8_t *x, uint32_t rows,
+                      int64_t arrival_ns) {
+  if (!s || !x) return fail(GBS_EINVAL, "gbs_server_submit: null argument");
+  if (rows == 0 || rows > s->max_rows)
+    return fail(GBS_ERANGE, "gbs_server_submit: %u rows outside [1, %u]", rows, s->max_rows);
+  Req *r = new Req();
+  r->id = id;
+  r->rows = rows;
+  r->off = 0;
+  r->arrival = arrival_ns ? arrival_ns : now_ns();
+  r->done = 0;
+  r->x.assign(x, x + (size_t)rows * s->h[0]);
+  s->waiting.push_back(r);
+  return GBS_OK;
+}
+
+int gbs_server_step(gbs_server *s, uint32_t *finished_now) {
+  if (!s) return fail(GBS_EINVAL, "gbs_server_step: null server");
+  int rc = ensure_device();
+  if (rc) return rc;
+  hipStream_t st = thread_stream();
+  const uint32_t L = s->L;
+  // Between steps cohort[l] holds the requests whose next GEMM is layer l
+  // (their activations sit in pool l); cohort[0] is empty.
+  // 1. admission into layer 0
+  size_t in_flight = 0;
+  for (uint32_t l = 1; l < L; l++) in_flight += s->cohort[l].size();
+  uint32_t n0 = 0;
+  while (!s->waiting.empty()) {
+    Req *r = s->waiting.front();
+    if (n0 + r->rows > s->R || in_flight + s->cohort[0].size() + 1 > s->max_requests) break;
+    s->waiting.pop_front();
+    r->off = n0;
+    std::copy(r->x.begin(), r->x.end(), s->h_in + (size_t)n0 * s->h[0]);
+    n0 += r->rows;
+    s->cohort[0].push_back(r);
+  }
+  s->cohort_rows[0] = n0;
+  if (n0) {
+    GBS_HIP(hipMemcpyAsync(s->d_in, s->h_in, (size_t)n0 * s->h[0], hipMemcpyHostToDevice, st));
+    ActJobs J{};
+    J.nbits = s->act_bits;
+    J.maxv = (1u << s->act_bits) - 1;
+    gbs_matrix *p = s->pool[0];
+    J.j[0] = ActJob{s->d_in, nullptr, p->d_planes, p->d_vals, p->wpr(), p->wpp(),
+                    p->vals_ld, n0, s->h[0], 0, 0, 0};
+    J.n = 1;
+    const uint64_t units = (uint64_t)n0 * p->wpr();
+    hipLaunchKernelGGL(act_pack_kernel, dim3((uint32_t)((units + 3) / 4)), dim3(256), 0, st,
+                       J, units);
+    GBS_HIP(hipGetLastError());
+  }
+  // 2. every layer's GEMM in one persistent launch
+  MfmaOperands views[kMaxLayers];
+  int nv = 0;
+  for (uint32_t l = 0; l < L; l++) {
+    const uint32_t n = s->cohort_rows[l];
+    if (!n) continue;
+    MfmaOperands v{};
+    v.R = s->pool[l]->d_vals;
+    v.r_ld = s->pool[l]->vals_ld;
+    v.L = s->W[l]->d_vals;
+    v.l_ld = s->W[l]->vals_ld;
+    v.res = s->acc[l];
+    v.M = s->h[l + 1];
+    v.N = n;
+    v.kbytes = (uint32_t)align_up(s->h[l], kMfmaKS);
+    views[nv++] = v;
+    s->ops_done += 2.0 * s->h[l + 1] * s->h[l] * n;
+  }
+  if (nv && (rc = launch_views(views, nv, s->h_desc, s->d_desc, s->d_counter, st))) return rc;
+  // 3. requantize + pack into the next layer's pool
+  ActJobs J{};
+  J.nbits = s->act_bits;
+  J.maxv = (1u << s->act_bits) - 1;
+  uint64_t units = 0;
+  for (uint32_t l = 0; l + 1 < L; l++) {
+    const uint32_t n = s->cohort_rows[l];
+    if (!n) continue;
+    gbs_matrix *p = s->pool[l + 1];
+    J.j[J.n++] = ActJob{nullptr, s->acc[l], p->d_planes, p->d_vals, p->wpr(), p->wpp(),
+                        p->vals_ld, n, s->h[l + 1], 1, s->shift[l], units};
+    units += (uint64_t)n * p->wpr();
+  }
+  if (units) {
+    hipLaunchKernelGGL(act_pack_kernel, dim3((uint32_t)((units + 3) / 4)), dim3(256), 0, st,
+                       J, units);
+    GBS_HIP(hipGetLastError());
+  }
+  // 4. last layer's rows back to the host
+  const uint32_t nl = s->cohort_rows[L - 1];
+  if (nl)
+    GBS_HIP(hipMemcpyAsync(s->h_out, s->acc[L - 1], (size_t)nl * s->h[L] * 4,
+                           hipMemcpyDeviceToHost, st));
+  GBS_HIP(hipStreamSynchronize(st));
+  const int64_t t = now_ns();
+  uint32_t fin = 0;
+  // the cohort at the last layer is complete now
+  for (Req *r : s->cohort[L - 1]) {
+    r->y.assign(s->h_out + (size_t)r->off * s->h[L],
+                s->h_out + (size_t)(r->off + r->rows) * s->h[L]);
+    r->done = t;
+    s->finished[r->id] = r;
+    s->done_requests++;
+    s->done_rows += r->rows;
+    fin++;
+  }
+  // every other cohort moves one layer on
+  for (uint32_t l = L - 1; l > 0; l--) {
+    s->cohort[l].swap(s->cohort[l - 1]);
+    s->cohort_rows[l] = s->cohort_rows[l - 1];
+  }
+  s->cohort[0].clear();
+  s->cohort_rows[0] = 0;
+  s->steps++;
+  if (finished_now) *finished_now = fin;
+  return GBS_OK;
+}
+
+int gbs_server_take(gbs_server *s, uint64_t id, int32_t *out, uint64_t cap_elems,
+                    int *ready, uint32_t *rows, int64_t *arrival_ns, int64_t *done_ns) {
+  if (!s || !ready) return fail(GBS_EINVAL, "gbs_server_take: null argument");
+  auto it = s->finished.find(id);
+  if (it == s->finished.end()) {
+    *ready = 0;
+    return GBS_OK;
+  }
+  Req *r = it->second;
+  if (out) {
+    if (cap_elems < r->y.size())
+      return fail(GBS_ERANGE, "gbs_server_take: output holds %llu of %zu values",
+                  (unsigned long long)cap_elems, r->y.size());
+    std::copy(r->y.begin(), r->y.end(), out);
+  }
+  *ready = 1;
+  if (rows) *rows = r->rows;
+  if (arrival_ns) *arrival_ns = r->arrival;
+  if (done_ns) *done_ns = r->done;
+  s->finished.erase(it);
+  delete r;
+  return GBS_OK;
+}
+
+int gbs_server_stats(const gbs_server *s, gbs_server_counters *st) {
+  if (!s || !st) return fail(GBS_EINVAL, "gbs_server_stats: null argument");
+  st->steps = s->steps;
+  st->done_requests = s->done_requests;
+  st->done_rows = s->done_rows;
+  st->ops = s->ops_done;
+  st->row_capacity = s->R;
+  st->bytes_reserved = s->bytes_reserved;
+  st->waiting = (uint32_t)s->waiting.size();
+  uint32_t act = 0;
+  for (const auto &c : s->cohort) act += (uint32_t)c.size();
+  st->in_flight = act;
+  return GBS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,83 @@
+"""Seeded synthetic workloads of the benchmark configurations (shapes,
+bitwidths, value distributions).  Values are int8 arrays in the regular
+layout the importer takes: lhs M x K, rhs N x K."""
+import numpy as np
+
+SEED0 = 0x67627300
+
+
+def values(rng, shape, nbits, issigned):
+    """Uniform over the encoding's range: bipolar +-1 for 1-bit signed,
+    [-2^(b-1), 2^(b-1)-1] signed, [0, 2^b - 1] unsigned."""
+    if nbits == 1 and issigned:
+        return np.where(rng.integers(0, 2, shape) == 1, 1, -1).astype(np.int8)
+    if issigned:
+        return rng.integers(-(1 << (nbits - 1)), 1 << (nbits - 1), shape).astype(np.int8)
+    return rng.integers(0, 1 << nbits, shape).astype(np.int8)
+
+
+def log_uniform_int(rng, lo, hi):
+    return int(round(float(np.exp(rng.uniform(np.log(lo), np.log(hi))))))
+
+
+def batched_uneven(count=512, seed=SEED0 + 3, scale=1.0):
+    """Batched uneven GEMMs: M in [64, 4096], K in [256, 8192] (multiple of
+    64), N in [1, 256], each log-uniform; bitwidths uniform in {1,2,3,4};
+    signedness random per side (1-bit signed = bipolar).  scale < 1 shrinks
+    M and K for quick tests.  Returns a list of problem dicts."""
+    rng = np.random.default_rng(seed)
+    probs = []
+    for _ in range(count):
+        M = max(1, int(log_uniform_int(rng, 64, 4096) * scale))
+        K = max(64, 64 * int(round(log_uniform_int(rng, 256, 8192) * scale / 64)))
+        N = log_uniform_int(rng, 1, 256)
+        lb, rb = int(rng.integers(1, 5)), int(rng.integers(1, 5))
+        ls, rs = bool(rng.integers(0, 2)), bool(rng.integers(0, 2))
+        probs.append(dict(M=M, K=K, N=N, lb=lb, rb=rb, ls=ls, rs=rs,
+                          lhs=values(rng, (M, K), lb, ls), rhs=values(rng, (N, K), rb, rs)))
+    return probs
+
+
+def mlp_shift(h_in, w_bits=4, a_bits=4):
+    """Requant shift for a layer of depth h_in: acc >> s lands the typical
+    accumulator (std of a sum of h_in products of uniform w_bits-bit signed
+    weights and a_bits-bit unsigned activations) in the activation range."""
+    ew2 = np.mean(np.arange(-(1 << (w_bits - 1)), 1 << (w_bits - 1)) ** 2.0)
+    ex2 = np.mean(np.arange(0, 1 << a_bits) ** 2.0)
+    std = np.sqrt(h_in * ew2 * ex2)
+    return max(0, int(np.floor(np.log2(std / (1 << (a_bits - 1))))))
+
+
+def serving_mlp(layers=8, lo=1024, hi=4096, seed=SEED0 + 4):
+    """8-layer MLP: hidden sizes log-uniform in [lo, hi] (unrounded), 4-bit
+    signed weights uniform in [-8, 7], requant shift per layer."""
+    rng = np.random.default_rng(seed)
+    h = [log_uniform_int(rng, lo, hi) for _ in range(layers + 1)]
+    W = [values(rng, (h[l + 1], h[l]), 4, True) for l in range(layers)]
+    shifts = [mlp_shift(h[l]) for l in range(layers - 1)]
+    return h, W, shifts
+
+
+def serving_requests(h0, count, rate_per_s, seed=SEED0 + 5, max_rows=64):
+    """Poisson arrivals (exponential gaps at rate_per_s): each request a batch
+    of 1..max_rows rows of 4-bit unsigned activations."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate_per_s, count)
+    t = np.cumsum(gaps)
+    out = []
+    for i in range(count):
+        rows = int(rng.integers(1, max_rows + 1))
+        out.append(dict(id=i, t=float(t[i]), x=rng.integers(0, 16, (rows, h0)).astype(np.uint8)))
+    return out
+
+
+def mlp_reference(x, W, shifts):
+    """Exact int64 forward pass of one request: acc = x W_l^T, then
+    min(15, max(0, acc >> s_l)) between layers; returns the last int32
+    accumulators."""
+    a = x.astype(np.float64)
+    for l, w in enumerate(W):
+        acc = (a @ w.astype(np.float64).T).astype(np.int64)
+        if l + 1 < len(W):
+            a = np.clip(acc >> shifts[l], 0, 15).astype(np.float64)
+    return acc.astype(np.int32)

@@ -190,6 +190,100 @@ int gbs_event_destroy(void *ev);
 int gbs_event_record(void *ev, void *stream);
 int gbs_event_elapsed_ms(void *start, void *stop, float *ms);
 
+/* Batched GEMMs ---------------------------------------------------------- */
+/* Many independent products res_i = lhs_i * rhs_i^T (uneven shapes, mixed
+ * bitwidths and signedness) run by one persistent kernel that takes output
+ * tiles from a device-side work queue.  No reference counterpart: the
+ * reference's callers loop over gemmBitSerial (benchmark/benchmark.cpp:
+ * 215-220, the CaffeNet layer list). */
+typedef struct gbs_batch gbs_batch;
+typedef struct gbs_problem {
+  const gbs_matrix *lhs, *rhs;
+  int32_t *d_res;               /* device, rhs->nrows x lhs->nrows int32 */
+} gbs_problem;
+
+/* Plan a batch (validates shapes, builds the tile list and the device
+ * descriptors, makes every operand's int8 copy current).  The matrices and
+ * result buffers must outlive the batch; re-importing an operand requires
+ * a new plan. */
+int gbs_batch_create(const gbs_problem *problems, uint64_t count, gbs_batch **out);
+int gbs_batch_destroy(gbs_batch *b);
+/* Enqueue the whole batch: one persistent launch (plus separate popcount
+ * launches for products the int8 engine cannot hold exactly). */
+int gbs_batch_run(gbs_batch *b, void *stream);
+/* Same, captured into a hipGraph on the first call and replayed after. */
+int gbs_batch_run_graph(gbs_batch *b, void *stream);
+/* Baseline: one gbs_gemm_async launch sequence per product. */
+int gbs_batch_run_per_call(gbs_batch *b, void *stream);
+int gbs_batch_info(const gbs_batch *b, uint64_t *tiles, uint64_t *persistent,
+                   uint64_t *separate);
+
+/* Tile autotuning of the MFMA engine -------------------------------------- */
+/* Products fall into shape classes (buckets of M, N and K); each class maps
+ * to one of the compiled tile shapes.  No reference counterpart (the
+ * reference sizes CPU cache blocks with computeBlockSize,
+ * gemmbitserial.hpp:343-377). */
+int gbs_tune_classes(int *n_classes, int *n_shapes);
+int gbs_tune_class_of(uint64_t M, uint64_t N, uint64_t K, int *cls);
+/* shapes[cls], splits[cls] for every class (shapes NULL: built-in
+ * heuristic; splits NULL or 0: K split chosen from the tile count). */
+int gbs_tune_set(const int *shapes, const int *splits, int n);
+/* Text table, one "class shape [splits]" line per class, '#' comments. */
+int gbs_tune_load(const char *path);
+/* One MFMA product with a forced tile shape and K split (the tuner's
+ * probe).  Operands must already have current int8 copies
+ * (gbs_workspace_reserve). */
+int gbs_gemm_async_config(gbs_workspace *ws, const gbs_matrix *lhs,
+                          const gbs_matrix *rhs, int32_t *d_res, int shape,
+                          int splits, void *stream);
+
+/* Serving -------------------------------------------------------------- */
+/* A continuous-batching request loop over a stack of layers whose weights
+ * stay resident in HBM in bit-serial form (plus their int8 copy).  Each
+ * request is 1..max_rows rows of act_bits-bit unsigned activations; between
+ * layers int32 results are requantized to min(2^act_bits - 1,
+ * max(0, acc >> shift_l)) and packed into bit-planes on the GPU.  Every step
+ * admits waiting requests as row capacity and request slots free up, and
+ * runs the GEMMs of all requests in flight (each at its own layer) in one
+ * persistent launch.  No reference counterpart: the reference ships only
+ * the single-GEMM call (its MNIST test, test/test.cpp:157-172, runs one
+ * layer).  A server is driven by one host thread. */
+typedef struct gbs_server gbs_server;
+typedef struct gbs_server_config {
+  uint32_t max_requests; /* requests in flight at once */
+  uint32_t max_rows;     /* rows per request */
+  uint64_t mem_budget;   /* bytes for activation pools and staging; 0 = a
+                            quarter of the free HBM at creation */
+} gbs_server_config;
+typedef struct gbs_server_counters {
+  uint64_t steps, done_requests, done_rows;
+  double ops;             /* 2*M*K*N summed over every GEMM row computed */
+  uint32_t row_capacity;  /* rows one cohort may hold (from mem_budget) */
+  uint64_t bytes_reserved;
+  uint32_t waiting, in_flight;
+} gbs_server_counters;
+
+/* Nanoseconds of the steady clock the server stamps requests with. */
+int gbs_clock_ns(int64_t *now);
+/* weights[l]: h_{l+1} x h_l, each must be exact on the int8 engine and must
+ * not change while the server lives; shifts[l] for l < n_layers - 1. */
+int gbs_server_create(const gbs_matrix *const *weights, const uint32_t *shifts,
+                      uint32_t n_layers, uint32_t act_bits,
+                      const gbs_server_config *cfg, gbs_server **out);
+int gbs_server_destroy(gbs_server *s);
+/* Queue a request: x is rows x h_0 codes (uint8, low act_bits bits used).
+ * arrival_ns (gbs_clock_ns time base) 0 = now.  Copies x. */
+int gbs_server_submit(gbs_server *s, uint64_t id, const uint8_t *x, uint32_t rows,
+                      int64_t arrival_ns);
+/* Admit, compute one layer for every request in flight, finish the ones
+ * leaving the last layer.  Synchronous. */
+int gbs_server_step(gbs_server *s, uint32_t *finished_now);
+/* Collect a finished request: *ready = 0 while it is not done.  out holds
+ * rows x h_L int32 (the last layer's accumulators). */
+int gbs_server_take(gbs_server *s, uint64_t id, int32_t *out, uint64_t cap_elems,
+                    int *ready, uint32_t *rows, int64_t *arrival_ns, int64_t *done_ns);
+int gbs_server_stats(const gbs_server *s, gbs_server_counters *st);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,105 @@
+"""Batched persistent launch: every product equals its own single-product
+result and the oracle, per-call and graph-replayed runs agree."""
+import numpy as np
+import pytest
+
+from gemmbitserial_amd import workloads as W
+from oracle import orc as O
+
+pytestmark = pytest.mark.gpu
+
+
+def build(gbs, probs):
+    items = []
+    for p in probs:
+        l = gbs.from_values(p["lhs"], p["lb"], p["ls"])
+        r = gbs.from_values(p["rhs"], p["rb"], p["rs"])
+        d = gbs.DeviceBuffer(4 * max(1, p["M"] * p["N"]))
+        items.append((l, r, d))
+    return items
+
+
+def exact(p):
+    return O.wrap_i32(O.blas_gemm_exact(O.values_of(p["lhs"], p["lb"], p["ls"]),
+                                        O.values_of(p["rhs"], p["rb"], p["rs"])))
+
+
+def results(items, probs):
+    return [d.to_host(np.int32, (p["N"], p["M"])) for (l, r, d), p in zip(items, probs)]
+
+
+def zero(items):
+    for _, _, d in items:
+        d.fill_zero()
+
+
+def test_batch_mixed_small(gbs, orc):
+    probs = W.batched_uneven(count=48, seed=11, scale=0.25)
+    # edge cases: an 8-bit unsigned product (popcount fallback), K = 0, N = 0
+    rng = np.random.default_rng(3)
+    probs.append(dict(M=33, K=200, N=7, lb=8, rb=2, ls=False, rs=True,
+                      lhs=rng.integers(0, 256, (33, 200)).astype(np.uint8),
+                      rhs=W.values(rng, (7, 200), 2, True)))
+    probs.append(dict(M=5, K=0, N=3, lb=2, rb=2, ls=False, rs=False,
+                      lhs=np.zeros((5, 0), np.int8), rhs=np.zeros((3, 0), np.int8)))
+    probs.append(dict(M=9, K=64, N=0, lb=1, rb=1, ls=True, rs=True,
+                      lhs=W.values(rng, (9, 64), 1, True), rhs=np.zeros((0, 64), np.int8)))
+    items = []
+    for p in probs:
+        l = gbs.DeviceMatrix(p["lb"], p["M"], p["K"], p["ls"])
+        r = gbs.DeviceMatrix(p["rb"], p["N"], p["K"], p["rs"])
+        if p["M"] * p["K"]:
+            l.import_regular(p["lhs"])
+        if p["N"] * p["K"]:
+            r.import_regular(p["rhs"])
+        items.append((l, r, gbs.DeviceBuffer(4 * max(1, p["M"] * p["N"]))))
+    b = gbs.Batch([(l, r, d.ptr) for l, r, d in items])
+    info = b.info()
+    assert info["separate"] >= 2 and info["persistent"] >= 40
+    want = [exact(p) if p["K"] else np.zeros((p["N"], p["M"]), np.int32) for p in probs]
+    b.run()
+    gbs.sync()
+    for g, w in zip(results(items, probs), want):
+        assert np.array_equal(g, w)
+    zero(items)
+    b.run_per_call()
+    gbs.sync()
+    for g, w in zip(results(items, probs), want):
+        assert np.array_equal(g, w)
+    # graph: captured once, replayed with eager work and syncs in between
+    for rep in range(3):
+        zero(items)
+        gbs.sync()
+        b.run_graph()
+        gbs.sync()
+        for g, w in zip(results(items, probs), want):
+            assert np.array_equal(g, w), f"graph replay {rep}"
+    b.close()
+
+
+def test_batch_order_does_not_matter(gbs, orc):
+    probs = W.batched_uneven(count=24, seed=5, scale=0.2)
+    items = build(gbs, probs)
+    perm = np.random.default_rng(0).permutation(len(items))
+    b1 = gbs.Batch([(l, r, d.ptr) for l, r, d in items])
+    b1.run()
+    gbs.sync()
+    first = results(items, probs)
+    zero(items)
+    b2 = gbs.Batch([(items[i][0], items[i][1], items[i][2].ptr) for i in perm])
+    b2.run()
+    gbs.sync()
+    for g, f, p in zip(results(items, probs), first, probs):
+        assert np.array_equal(g, f)
+        assert np.array_equal(g, exact(p))
+
+
+def test_batch_full_size_c3(gbs, orc):
+    """The 512-product uneven workload at full size: every result exact."""
+    probs = W.batched_uneven()
+    items = build(gbs, probs)
+    b = gbs.Batch([(l, r, d.ptr) for l, r, d in items])
+    b.run_graph()
+    gbs.sync()
+    for g, p in zip(results(items, probs), probs):
+        assert np.array_equal(g, exact(p))

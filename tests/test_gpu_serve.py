@@ -1,0 +1,116 @@
+"""Serving loop: every request's result equals the exact forward pass and is
+the same alone as in the mix; admission respects the row capacity."""
+import numpy as np
+import pytest
+
+from gemmbitserial_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def make_server(gbs, h, Ws, shifts, **kw):
+    mats = [gbs.from_values(w, 4, True) for w in Ws]
+    return gbs.Server(mats, shifts, act_bits=4, **kw), mats
+
+
+def drain(srv, ids, max_steps=10000):
+    out = {}
+    for _ in range(max_steps):
+        srv.step()
+        for i in list(ids):
+            r = srv.take(i)
+            if r is not None:
+                out[i] = r
+                ids.remove(i)
+        if not ids:
+            return out
+    raise AssertionError("requests did not finish")
+
+
+def small_mlp(seed, layers=3):
+    rng = np.random.default_rng(seed)
+    h = [int(x) for x in rng.integers(70, 300, layers + 1)]
+    Ws = [W.values(rng, (h[l + 1], h[l]), 4, True) for l in range(layers)]
+    shifts = [W.mlp_shift(h[l]) for l in range(layers - 1)]
+    return h, Ws, shifts
+
+
+def test_serve_mix_equals_alone_and_reference(gbs):
+    h, Ws, shifts = small_mlp(1)
+    reqs = W.serving_requests(h[0], 12, 1000.0, seed=2, max_rows=20)
+    srv, mats = make_server(gbs, h, Ws, shifts, max_requests=5, max_rows=20)
+    for r in reqs:
+        srv.submit(r["id"], r["x"])
+    mixed = drain(srv, [r["id"] for r in reqs])
+    st = srv.stats()
+    assert st["done_requests"] == len(reqs) and st["waiting"] == 0
+    srv.close()
+    for r in reqs:
+        want = W.mlp_reference(r["x"], Ws, shifts)
+        assert np.array_equal(mixed[r["id"]][0], want), r["id"]
+    # alone: a fresh server per request
+    for r in reqs[:4]:
+        s1, _ = make_server(gbs, h, Ws, shifts, max_requests=1, max_rows=20)
+        s1.submit(r["id"], r["x"])
+        alone = drain(s1, [r["id"]])
+        assert np.array_equal(alone[r["id"]][0], mixed[r["id"]][0])
+        s1.close()
+
+
+def test_serve_staggered_arrivals(gbs):
+    """Requests submitted between steps join while others are in flight."""
+    h, Ws, shifts = small_mlp(3, layers=4)
+    reqs = W.serving_requests(h[0], 16, 1000.0, seed=4, max_rows=8)
+    srv, mats = make_server(gbs, h, Ws, shifts, max_requests=8, max_rows=8)
+    pending = [r["id"] for r in reqs]
+    got = {}
+    for r in reqs:
+        srv.submit(r["id"], r["x"])
+        srv.step()
+        for i in list(pending):
+            t = srv.take(i)
+            if t is not None:
+                got[i] = t
+                pending.remove(i)
+    got.update(drain(srv, pending))
+    for r in reqs:
+        assert np.array_equal(got[r["id"]][0], W.mlp_reference(r["x"], Ws, shifts))
+        arr, done = got[r["id"]][1], got[r["id"]][2]
+        assert done >= arr
+    srv.close()
+
+
+def test_serve_admission_by_memory(gbs):
+    h, Ws, shifts = small_mlp(5)
+    # a budget that holds only ~10 rows per cohort: requests queue, not fail
+    per_row = sum(4 * ((x + 63) // 64 * 64) // 8 + (x + 255) // 256 * 256 for x in h[:-1]) \
+        + sum(4 * x for x in h[1:]) + h[0] + 4 * h[-1]
+    srv, mats = make_server(gbs, h, Ws, shifts, max_requests=64, max_rows=8,
+                            mem_budget=per_row * 10)
+    assert srv.stats()["row_capacity"] == 10
+    reqs = W.serving_requests(h[0], 10, 1000.0, seed=6, max_rows=8)
+    for r in reqs:
+        srv.submit(r["id"], r["x"])
+    srv.step()
+    st = srv.stats()
+    assert st["waiting"] > 0  # not everything fit into the first cohort
+    got = drain(srv, [r["id"] for r in reqs])
+    for r in reqs:
+        assert np.array_equal(got[r["id"]][0], W.mlp_reference(r["x"], Ws, shifts))
+    with pytest.raises(gbs.GbsError):
+        srv.submit(99, np.zeros((9, h[0]), np.uint8))  # more rows than max_rows
+    srv.close()
+    with pytest.raises(gbs.GbsError, match="memory budget"):
+        make_server(gbs, h, Ws, shifts, max_requests=4, max_rows=8, mem_budget=per_row * 4)
+
+
+def test_serve_full_size_mlp(gbs):
+    h, Ws, shifts = W.serving_mlp()
+    reqs = W.serving_requests(h[0], 24, 1000.0, seed=7)
+    srv, mats = make_server(gbs, h, Ws, shifts, max_requests=16, max_rows=64)
+    for r in reqs:
+        srv.submit(r["id"], r["x"])
+    got = drain(srv, [r["id"] for r in reqs])
+    for r in reqs:
+        assert np.array_equal(got[r["id"]][0], W.mlp_reference(r["x"], Ws, shifts))
+    srv.close()
