@@ -307,6 +307,9 @@ def main():
     ap.add_argument("--cpu-secs", type=float, default=4.0,
                     help="reference CPU timing budget per run (0 = skip)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibrate", type=int, default=0, metavar="BYTES",
+                    help="also run device-to-device copies of BYTES (a kernel of known "
+                         "HBM traffic, for scaling profiler byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); this build "
@@ -320,6 +323,12 @@ def main():
     stream = gbs.stream()
     ws = gbs.Workspace()
 
+    if args.copy_calibrate:
+        src, dst = gbs.DeviceBuffer(args.copy_calibrate), gbs.DeviceBuffer(args.copy_calibrate)
+        src.fill_zero()
+        for _ in range(3):
+            dst.copy_from(src)
+        src.close(), dst.close()
     names = [c.strip() for c in args.configs.split(",") if c.strip()]
     if "c1" not in names:
         names = ["c1"] + names

@@ -250,6 +250,12 @@ class DeviceBuffer:
             raise GbsError(3, f"hipMemcpy D2H failed: {rc}")
         return out
 
+    def copy_from(self, other):
+        rc = self.hip().hipMemcpy(C.c_void_p(self.ptr), C.c_void_p(other.ptr),
+                                  min(self.nbytes, other.nbytes), 3)
+        if rc != 0:
+            raise GbsError(3, f"hipMemcpy D2D failed: {rc}")
+
     def fill_zero(self):
         rc = self.hip().hipMemset(C.c_void_p(self.ptr), 0, max(self.nbytes, 8))
         if rc != 0:

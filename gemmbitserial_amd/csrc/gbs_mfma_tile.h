@@ -227,12 +227,12 @@ MfmaPlan mfma_plan(uint32_t M, uint32_t N, uint32_t kbytes, int forced_shape = -
                    int forced_splits = 0);
 
 // Shape class of a product, for the tile autotuner's table: buckets of
-// log2 of M, N and K.  kClassM x kClassN x kClassK classes.
-constexpr int kClassM = 4, kClassN = 5, kClassK = 3;
+// M, N and K.  kClassM x kClassN x kClassK classes.
+constexpr int kClassM = 5, kClassN = 6, kClassK = 3;
 constexpr int kClasses = kClassM * kClassN * kClassK;
 inline int shape_class(uint64_t M, uint64_t N, uint64_t K) {
-  const int cm = M < 256 ? 0 : M < 1024 ? 1 : M < 4096 ? 2 : 3;
-  const int cn = N <= 16 ? 0 : N <= 32 ? 1 : N <= 64 ? 2 : N <= 256 ? 3 : 4;
+  const int cm = M < 256 ? 0 : M < 1024 ? 1 : M < 4096 ? 2 : M < 16384 ? 3 : 4;
+  const int cn = N <= 16 ? 0 : N <= 32 ? 1 : N <= 64 ? 2 : N <= 256 ? 3 : N <= 2048 ? 4 : 5;
   const int ck = K < 1024 ? 0 : K < 4096 ? 1 : 2;
   return (cm * kClassN + cn) * kClassK + ck;
 }

@@ -24,10 +24,11 @@ from gemmbitserial_amd import workloads as W
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_OUT = os.path.join(HERE, "tune_table.txt")
 # representative sizes of the class buckets (see shape_class in
-# csrc/gbs_mfma_tile.h): M <256 / <1024 / <4096 / >=4096,
-# N <=16 / <=32 / <=64 / <=256 / >256, K <1024 / <4096 / >=4096
-REP_M = [160, 640, 2048, 4096]
-REP_N = [8, 24, 48, 160, 512]
+# csrc/gbs_mfma_tile.h): M <256 / <1024 / <4096 / <16384 / >=16384,
+# N <=16 / <=32 / <=64 / <=256 / <=2048 / >2048, K <1024 / <4096 / >=4096.
+# The products of the largest buckets are kept to a few hundred MB.
+REP_M = [160, 640, 2048, 4096, 32768]
+REP_N = [8, 24, 48, 160, 512, 16384]
 REP_K = [512, 2048, 8192]
 SHAPES = ["128x128", "64x128", "32x256/k64", "16x256", "64x64", "128x64", "128x128/k64",
           "128x256/k64"]
@@ -55,6 +56,8 @@ def tune(reps=10, log=print):
     shapes, splits = [-1] * n_classes, [0] * n_classes
     detail = {}
     for M, N, K in itertools.product(REP_M, REP_N, REP_K):
+        if M * N > 2 ** 26:  # 256 MB of result: left to the built-in heuristic
+            continue
         cls = gbs.tune_class_of(M, N, K)
         l = gbs.from_values(W.values(rng, (M, K), 2, True), 2, True)
         r = gbs.from_values(W.values(rng, (N, K), 2, False), 2, False)
