@@ -23,7 +23,9 @@ namespace gbs {
 namespace {
 
 template <int P, int Q, int WP, int KSB, int DEPTH>
-__global__ __launch_bounds__(256, 2) void mfma_kernel(MfmaOperands a, MfmaPlan pl) {
+__global__ __launch_bounds__(256, (P * Q >= 65536 || (P * Q >= 32768 && DEPTH == 0)) ? 1 : 2)
+void mfma_kernel(
+    MfmaOperands a, MfmaPlan pl) {
   __shared__ __attribute__((aligned(16))) int8_t lds[mfma_lds_bytes<P, Q, KSB>()];
   const uint32_t T = gridDim.x, b = blockIdx.x;
   // consecutive logical tiles run on one XCD (blocks are dealt round-robin)
@@ -32,7 +34,10 @@ __global__ __launch_bounds__(256, 2) void mfma_kernel(MfmaOperands a, MfmaPlan p
   plan_decode(pl, t, pt, qt, sp);
   const uint32_t kbeg = sp * pl.kchunk;
   const uint32_t kend = min(a.kbytes, kbeg + pl.kchunk);
-  mfma_tile<P, Q, WP, KSB, DEPTH>(a, pt, qt, kbeg, kend, pl.splits > 1, lds);
+  if constexpr (DEPTH == 0)
+    mfma_tile_pp<P, Q, WP>(a, pt, qt, kbeg, kend, pl.splits > 1, lds);
+  else
+    mfma_tile<P, Q, WP, KSB, DEPTH>(a, pt, qt, kbeg, kend, pl.splits > 1, lds);
 }
 
 // planes -> int8 values, 4 bytes per thread.  Zero outside the real matrix.
@@ -93,6 +98,11 @@ int run_plan_id(const MfmaOperands &a, const MfmaPlan &pl, hipStream_t s) {
     case 5: return run_plan<128, 64, 4, 128, 2>(a, pl, s);
     case 6: return run_plan<128, 128, 2, 64, 1>(a, pl, s);
     case 7: return run_plan<128, 256, 2, 64, 2>(a, pl, s);
+    case 8: return run_plan<128, 128, 2, 64, 0>(a, pl, s);
+    case 9: return run_plan<128, 256, 2, 64, 0>(a, pl, s);
+    case 10: return run_plan<64, 128, 2, 64, 0>(a, pl, s);
+    case 11: return run_plan<256, 256, 2, 64, 1>(a, pl, s);
+    case 12: return run_plan<256, 256, 2, 128, 1>(a, pl, s);
   }
   return fail(GBS_EINVAL, "mfma: unknown tile shape %d", pl.shape);
 }

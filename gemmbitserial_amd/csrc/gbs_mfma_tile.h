@@ -36,16 +36,33 @@ struct MfmaOperands {
 
 // Tile shapes the engine is compiled for: P x Q outputs, WP waves along P,
 // KSB bytes of K per LDS stage, DEPTH stages of global loads in flight.
+// DEPTH 0 marks the software-pipelined tile (mfma_tile_pp).
 struct MfmaTileShape { int P, Q, WP, KSB, DEPTH; };
-constexpr int kMfmaShapes = 8;
+constexpr int kMfmaShapes = 13;
 constexpr MfmaTileShape kMfmaShape[kMfmaShapes] = {
     {128, 128, 2, 128, 2}, {64, 128, 2, 128, 2}, {32, 256, 1, 64, 2},
     {16, 256, 1, 128, 2},  {64, 64, 2, 128, 2},  {128, 64, 4, 128, 2},
-    {128, 128, 2, 64, 1},  {128, 256, 2, 64, 2}};
+    {128, 128, 2, 64, 1},  {128, 256, 2, 64, 2}, {128, 128, 2, 64, 0},
+    {128, 256, 2, 64, 0},  {64, 128, 2, 64, 0},  {256, 256, 2, 64, 1},
+    {256, 256, 2, 128, 1}};
+
+// LDS row layout of a stage.  128- and 256-byte stages use rows of exactly
+// KSB bytes with the 16-byte chunks of row r XOR-permuted by (r / (256/KSB))
+// mod (KSB/16): any 16 consecutive rows at one logical chunk, and any 256
+// consecutive bytes written, then fall in 64 distinct banks.  64-byte
+// stages pad each row by 16 bytes instead.
+template <int KSB>
+constexpr int lds_row_bytes() { return KSB >= 128 ? KSB : KSB + 16; }
+template <int KSB>
+__device__ __forceinline__ int lds_chunk(int row, int ch) {
+  if constexpr (KSB >= 128) return ch ^ ((row / (256 / KSB)) % (KSB / 16));
+  else return ch;
+}
 
 template <int P, int Q, int KSB>
-constexpr size_t mfma_lds_bytes() { return (size_t)2 * (P + Q) * (KSB + 16); }
-constexpr size_t kMfmaLdsMax = (size_t)2 * 272 * (128 + 16);  // largest over the shapes (16 x 256)
+constexpr size_t mfma_lds_bytes() { return (size_t)2 * (P + Q) * lds_row_bytes<KSB>(); }
+constexpr size_t kLdsPerCU = 160 * 1024;
+constexpr size_t kMfmaLdsMax = (size_t)2 * 272 * 128;  // persistent kernel: largest carried shape (16 x 256)
 
 template <int P, int Q, int WP, int KSB, int DEPTH>
 __device__ __forceinline__ void mfma_tile(const MfmaOperands &a, uint32_t pt, uint32_t qt,
@@ -54,14 +71,14 @@ __device__ __forceinline__ void mfma_tile(const MfmaOperands &a, uint32_t pt, ui
   constexpr int WQ = 4 / WP;
   constexpr int PW = P / WP, QW = Q / WQ;       // wave tile
   constexpr int FP = PW / 16, FQ = QW / 16;     // fragments per wave
-  constexpr int LDR = KSB + 16;                 // LDS row stride (bytes)
+  constexpr int LDR = lds_row_bytes<KSB>();     // LDS row stride (bytes)
   constexpr int CPR = KSB / 16;                 // 16-byte chunks per row and stage
   constexpr int RCH = P * CPR, LCH = Q * CPR;   // chunks per stage
   constexpr int RLD = (RCH + 255) / 256, LLD = (LCH + 255) / 256;
   static_assert(FP >= 1 && FQ >= 1 && PW % 16 == 0 && QW % 16 == 0, "bad tile");
   static_assert(KSB % 64 == 0 && kMfmaKS % KSB == 0, "bad stage size");
   static_assert(DEPTH == 1 || DEPTH == 2, "prefetch depth");
-  static_assert(mfma_lds_bytes<P, Q, KSB>() <= kMfmaLdsMax, "LDS");
+  static_assert(mfma_lds_bytes<P, Q, KSB>() <= kLdsPerCU, "LDS");
   int8_t *sR0 = lds, *sL0 = lds + P * LDR;
   constexpr int stage = (P + Q) * LDR;
 
@@ -89,12 +106,14 @@ __device__ __forceinline__ void mfma_tile(const MfmaOperands &a, uint32_t pt, ui
 #pragma unroll
     for (int i = 0; i < RLD; i++) {
       const int c = tid + 256 * i, row = c / CPR, ch = c % CPR;
-      if (RCH % 256 == 0 || c < RCH) *reinterpret_cast<v4i *>(&sR[row * LDR + ch * 16]) = r[i];
+      if (RCH % 256 == 0 || c < RCH)
+        *reinterpret_cast<v4i *>(&sR[row * LDR + lds_chunk<KSB>(row, ch) * 16]) = r[i];
     }
 #pragma unroll
     for (int i = 0; i < LLD; i++) {
       const int c = tid + 256 * i, row = c / CPR, ch = c % CPR;
-      if (LCH % 256 == 0 || c < LCH) *reinterpret_cast<v4i *>(&sL[row * LDR + ch * 16]) = l[i];
+      if (LCH % 256 == 0 || c < LCH)
+        *reinterpret_cast<v4i *>(&sL[row * LDR + lds_chunk<KSB>(row, ch) * 16]) = l[i];
     }
   };
 
@@ -104,18 +123,23 @@ __device__ __forceinline__ void mfma_tile(const MfmaOperands &a, uint32_t pt, ui
 #pragma unroll
     for (int j = 0; j < FQ; j++) acc[i][j] = v4i{0, 0, 0, 0};
 
-  const int fr = lane & 15, fk = (lane >> 4) * 16;
+  const int fr = lane & 15;
   auto compute = [&](int buf) {
     const int8_t *sR = sR0 + buf * stage, *sL = sL0 + buf * stage;
 #pragma unroll
     for (int kk = 0; kk < KSB / 64; kk++) {
       v4i af[FP], bf[FQ];
+      const int ck = kk * 4 + (lane >> 4);  // logical 16-byte chunk of this lane
 #pragma unroll
-      for (int i = 0; i < FP; i++)
-        af[i] = *reinterpret_cast<const v4i *>(&sR[(wp * PW + i * 16 + fr) * LDR + kk * 64 + fk]);
+      for (int i = 0; i < FP; i++) {
+        const int row = wp * PW + i * 16 + fr;
+        af[i] = *reinterpret_cast<const v4i *>(&sR[row * LDR + lds_chunk<KSB>(row, ck) * 16]);
+      }
 #pragma unroll
-      for (int j = 0; j < FQ; j++)
-        bf[j] = *reinterpret_cast<const v4i *>(&sL[(wq * QW + j * 16 + fr) * LDR + kk * 64 + fk]);
+      for (int j = 0; j < FQ; j++) {
+        const int row = wq * QW + j * 16 + fr;
+        bf[j] = *reinterpret_cast<const v4i *>(&sL[row * LDR + lds_chunk<KSB>(row, ck) * 16]);
+      }
 #pragma unroll
       for (int i = 0; i < FP; i++)
 #pragma unroll
@@ -177,11 +201,149 @@ __device__ __forceinline__ void mfma_tile(const MfmaOperands &a, uint32_t pt, ui
       }
 }
 
-// Shapes the persistent kernel carries (all but the 128 x 256 one, whose
-// registers would make the shared kernel spill); others map to shape 0.
-__host__ __device__ constexpr int batch_shape(int s) { return s == 7 ? 0 : s; }
+// Block-wide barrier that waits for this wave's LDS operations only (not
+// for global loads still in flight), and keeps the compiler from moving
+// memory operations across it.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// Software-pipelined tile: K in 64-byte stages.  In the iteration that
+// multiplies stage i (fragments already in registers), the block writes
+// stage i+1 from registers into the other LDS buffer, issues the global
+// loads of stage i+2, passes one LDS-only barrier, and reads stage i+1's
+// fragments -- those LDS reads and the global loads overlap the MFMAs of
+// stage i.
+template <int P, int Q, int WP>
+__device__ __forceinline__ void mfma_tile_pp(const MfmaOperands &a, uint32_t pt, uint32_t qt,
+                                             uint32_t kbeg, uint32_t kend, bool atomic,
+                                             int8_t *lds) {
+  constexpr int KSB = 64, LDR = KSB + 16;
+  constexpr int WQ = 4 / WP;
+  constexpr int PW = P / WP, QW = Q / WQ;
+  constexpr int FP = PW / 16, FQ = QW / 16;
+  constexpr int CPR = KSB / 16;
+  constexpr int RCH = P * CPR, LCH = Q * CPR;
+  constexpr int RLD = (RCH + 255) / 256, LLD = (LCH + 255) / 256;
+  static_assert(FP >= 1 && FQ >= 1 && PW % 16 == 0 && QW % 16 == 0, "bad tile");
+  static_assert((size_t)2 * (P + Q) * LDR <= kLdsPerCU, "LDS");
+  int8_t *sR0 = lds, *sL0 = lds + P * LDR;
+  constexpr int stage = (P + Q) * LDR;
+  const uint32_t p0 = pt * P, q0 = qt * Q;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wp = wave % WP, wq = wave / WP;
+  const int fr = lane & 15, fk = (lane >> 4) * 16;
+
+  v4i gr[RLD], gl[LLD];
+  auto gload = [&](uint32_t k0) {
+#pragma unroll
+    for (int i = 0; i < RLD; i++) {
+      const int c = tid + 256 * i, row = c / CPR, ch = c % CPR;
+      if (RCH % 256 == 0 || c < RCH)
+        gr[i] = *reinterpret_cast<const v4i *>(a.R + (uint64_t)(p0 + row) * a.r_ld + k0 + ch * 16);
+    }
+#pragma unroll
+    for (int i = 0; i < LLD; i++) {
+      const int c = tid + 256 * i, row = c / CPR, ch = c % CPR;
+      if (LCH % 256 == 0 || c < LCH)
+        gl[i] = *reinterpret_cast<const v4i *>(a.L + (uint64_t)(q0 + row) * a.l_ld + k0 + ch * 16);
+    }
+  };
+  auto swrite = [&](int buf) {
+    int8_t *sR = sR0 + buf * stage, *sL = sL0 + buf * stage;
+#pragma unroll
+    for (int i = 0; i < RLD; i++) {
+      const int c = tid + 256 * i, row = c / CPR, ch = c % CPR;
+      if (RCH % 256 == 0 || c < RCH) *reinterpret_cast<v4i *>(&sR[row * LDR + ch * 16]) = gr[i];
+    }
+#pragma unroll
+    for (int i = 0; i < LLD; i++) {
+      const int c = tid + 256 * i, row = c / CPR, ch = c % CPR;
+      if (LCH % 256 == 0 || c < LCH) *reinterpret_cast<v4i *>(&sL[row * LDR + ch * 16]) = gl[i];
+    }
+  };
+  v4i fa[2][FP], fb[2][FQ];
+  auto lread = [&](int buf, v4i(&xa)[FP], v4i(&xb)[FQ]) {
+    const int8_t *sR = sR0 + buf * stage, *sL = sL0 + buf * stage;
+#pragma unroll
+    for (int i = 0; i < FP; i++)
+      xa[i] = *reinterpret_cast<const v4i *>(&sR[(wp * PW + i * 16 + fr) * LDR + fk]);
+#pragma unroll
+    for (int j = 0; j < FQ; j++)
+      xb[j] = *reinterpret_cast<const v4i *>(&sL[(wq * QW + j * 16 + fr) * LDR + fk]);
+  };
+  v4i acc[FP][FQ];
+#pragma unroll
+  for (int i = 0; i < FP; i++)
+#pragma unroll
+    for (int j = 0; j < FQ; j++) acc[i][j] = v4i{0, 0, 0, 0};
+  auto mma = [&](const v4i(&xa)[FP], const v4i(&xb)[FQ]) {
+#pragma unroll
+    for (int i = 0; i < FP; i++)
+#pragma unroll
+      for (int j = 0; j < FQ; j++)
+        acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(xa[i], xb[j], acc[i][j], 0, 0, 0);
+  };
+
+  const uint32_t nst = kend > kbeg ? (kend - kbeg) / KSB : 0;
+  if (nst) {
+    gload(kbeg);
+    swrite(0);
+    if (nst > 1) gload(kbeg + KSB);
+  }
+  lds_barrier();
+  if (nst) lread(0, fa[0], fb[0]);
+  auto step = [&](uint32_t i, auto cur) {
+    constexpr int c = decltype(cur)::value, n = c ^ 1;
+    if (i + 1 < nst) swrite(n);
+    if (i + 2 < nst) gload(kbeg + (i + 2) * KSB);
+    lds_barrier();
+    if (i + 1 < nst) lread(n, fa[n], fb[n]);
+    mma(fa[c], fb[c]);
+  };
+  for (uint32_t i = 0; i < nst; i += 2) {
+    step(i, std::integral_constant<int, 0>());
+    if (i + 1 < nst) step(i + 1, std::integral_constant<int, 1>());
+  }
+  // the LDS is reused by the next tile of a persistent block
+  lds_barrier();
+
+#pragma unroll
+  for (int i = 0; i < FP; i++)
+#pragma unroll
+    for (int j = 0; j < FQ; j++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const uint32_t n = p0 + wp * PW + i * 16 + 4 * (lane >> 4) + r;
+        const uint32_t m = q0 + wq * QW + j * 16 + fr;
+        if (n < a.N && m < a.M) {
+          const uint64_t idx = (uint64_t)n * a.M + m;
+          if (atomic) atomicAdd(a.res + idx, acc[i][j][r]);
+          else a.res[idx] = acc[i][j][r];
+        }
+      }
+}
+
+// Shapes the persistent kernel carries: all but the 128 x 256 and 256 x 256
+// ones, whose registers or LDS would make the shared kernel spill or drop to
+// one block per CU; they map to the 128 x 128 shape of the same pipeline.
+__host__ __device__ constexpr int batch_shape(int s) {
+  return s == 7 || s == 11 || s == 12 ? 0 : s == 9 ? 8 : s;
+}
 
 // Dispatch one tile by shape id (the persistent kernel's per-product switch).
+constexpr bool batch_shapes_fit_lds() {
+  for (int s = 0; s < kMfmaShapes; s++) {
+    const MfmaTileShape t = kMfmaShape[batch_shape(s)];
+    const int row = t.KSB >= 128 ? t.KSB : t.KSB + 16;
+    if ((size_t)2 * (t.P + t.Q) * row > kMfmaLdsMax) return false;
+  }
+  return true;
+}
+static_assert(batch_shapes_fit_lds(), "a shape of the persistent kernel overflows its LDS");
+
 __device__ __forceinline__ void mfma_tile_by_shape(int shape, const MfmaOperands &a,
                                                    uint32_t pt, uint32_t qt, uint32_t kbeg,
                                                    uint32_t kend, bool atomic, int8_t *lds) {
@@ -192,7 +354,9 @@ __device__ __forceinline__ void mfma_tile_by_shape(int shape, const MfmaOperands
     case 3: mfma_tile<16, 256, 1, 128, 2>(a, pt, qt, kbeg, kend, atomic, lds); break;
     case 4: mfma_tile<64, 64, 2, 128, 2>(a, pt, qt, kbeg, kend, atomic, lds); break;
     case 5: mfma_tile<128, 64, 4, 128, 2>(a, pt, qt, kbeg, kend, atomic, lds); break;
-    default: mfma_tile<128, 128, 2, 64, 1>(a, pt, qt, kbeg, kend, atomic, lds); break;
+    case 6: mfma_tile<128, 128, 2, 64, 1>(a, pt, qt, kbeg, kend, atomic, lds); break;
+    case 8: mfma_tile_pp<128, 128, 2>(a, pt, qt, kbeg, kend, atomic, lds); break;
+    default: mfma_tile_pp<64, 128, 2>(a, pt, qt, kbeg, kend, atomic, lds); break;
   }
 }
 
@@ -212,14 +376,11 @@ __host__ __device__ inline void plan_decode(const MfmaPlan &pl, uint64_t t, uint
                                             uint32_t &qt, uint32_t &sp) {
   const uint64_t per = (uint64_t)pl.ptiles * pl.qtiles;
   sp = (uint32_t)(t / per);
-  const uint64_t r = t - (uint64_t)sp * per;
-  if (pl.order == 0) {
-    qt = (uint32_t)(r % pl.qtiles);
-    pt = (uint32_t)(r / pl.qtiles);
-  } else {
-    pt = (uint32_t)(r % pl.ptiles);
-    qt = (uint32_t)(r / pl.ptiles);
-  }
+  const uint32_t r = (uint32_t)(t - (uint64_t)sp * per);
+  const uint32_t fast = pl.order ? pl.ptiles : pl.qtiles;
+  const uint32_t lo = r % fast, hi = r / fast;
+  qt = pl.order ? hi : lo;
+  pt = pl.order ? lo : hi;
 }
 
 // Plan for a product (uses the tuned table; see mfma_config_for).

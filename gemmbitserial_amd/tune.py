@@ -28,10 +28,11 @@ DEFAULT_OUT = os.path.join(HERE, "tune_table.txt")
 # N <=16 / <=32 / <=64 / <=256 / <=2048 / >2048, K <1024 / <4096 / >=4096.
 # The products of the largest buckets are kept to a few hundred MB.
 REP_M = [160, 640, 2048, 4096, 32768]
-REP_N = [8, 24, 48, 160, 512, 16384]
+REP_N = [8, 24, 48, 256, 512, 16384]
 REP_K = [512, 2048, 8192]
 SHAPES = ["128x128", "64x128", "32x256/k64", "16x256", "64x64", "128x64", "128x128/k64",
-          "128x256/k64"]
+          "128x256/k64", "128x128/pp", "128x256/pp", "64x128/pp", "256x256/k64",
+          "256x256"]
 
 
 def time_config(l, r, d, shape, splits, reps, stream):

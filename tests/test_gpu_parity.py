@@ -263,3 +263,28 @@ def test_auto_matches_both_engines(gbs, orc):
         l, r = gbs.from_values(lhs, lb, ls), gbs.from_values(rhs, rb, rs)
         ra, rp, rm = gbs.gemm(l, r, "auto"), gbs.gemm(l, r, "popcount"), gbs.gemm(l, r, "mfma")
         assert np.array_equal(ra, rp) and np.array_equal(rp, rm)
+
+
+def test_mfma_every_tile_shape_and_split(gbs, orc):
+    """Each compiled tile shape, with and without a K split, on ragged
+    shapes (the autotuner may pick any of them)."""
+    n_classes, n_shapes = gbs.tune_classes()
+    rng = np.random.default_rng(99)
+    ws = gbs.Workspace()
+    for (M, K, N, lb, rb, ls, rs) in [(300, 1100, 70, 3, 2, True, False),
+                                      (129, 2048, 257, 1, 1, True, True),
+                                      (64, 320, 16, 4, 4, True, True)]:
+        lhs = rand_values(rng, (M, K), lb, ls)
+        rhs = rand_values(rng, (N, K), rb, rs)
+        l, r = gbs.from_values(lhs, lb, ls), gbs.from_values(rhs, rb, rs)
+        ws.reserve(l, r)
+        want = expected(lhs, rhs, lb, rb, ls, rs)
+        d = gbs.DeviceBuffer(4 * M * N)
+        for shape in range(n_shapes):
+            for splits in (1, 3):
+                say("shape", shape, "splits", splits, M, K, N)
+                d.fill_zero()
+                gbs.check(gbs.lib().gbs_gemm_async_config(ws.h, l.h, r.h, d.ptr, shape, splits,
+                                                          None))
+                gbs.sync()
+                assert np.array_equal(d.to_host(np.int32, (N, M)), want), (shape, splits)
