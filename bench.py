@@ -42,6 +42,7 @@ SEED0 = 0x67627300
 VALU_PEAK = 256 * 64 * 2.4e9      # int32 lane-ops/s: 256 CUs x 64 lanes x 2.4 GHz
 HBM_PEAK = 8.0e12                 # B/s
 MFMA_I8_PEAK = 5.0e15             # dense int8 ops/s
+MFMA_F4_PEAK = 10.1e15            # dense FP4 ops/s
 
 
 def log(*a):
@@ -103,11 +104,12 @@ def int8_bytes(M, K, N):
 
 
 def roofline_of(algo, M, K, N, lb, rb, sec):
-    if algo == "mfma":
+    if algo in ("mfma", "mfma_fp4"):
         ach = 2.0 * M * K * N / sec
+        peak, unit = ((MFMA_I8_PEAK, "TOP/s (int8)") if algo == "mfma"
+                      else (MFMA_F4_PEAK, "TOP/s (fp4 operands)"))
         return {"bound": "mfma", "achieved": round(ach / 1e12, 2),
-                "peak": MFMA_I8_PEAK / 1e12, "unit": "TOP/s (int8)",
-                "frac": round(ach / MFMA_I8_PEAK, 4)}
+                "peak": peak / 1e12, "unit": unit, "frac": round(ach / peak, 4)}
     ach = valu_lane_ops(M, K, N, lb, rb) / sec
     hbm = plane_bytes(M, K, N, lb, rb) / sec
     if hbm / HBM_PEAK > ach / VALU_PEAK:
@@ -145,12 +147,12 @@ def run_config(gbs, O, name, runs, engines, steps, warmup, stream, ws, cpu_secs)
         want = None
         row = {"encoding": enc, "M": M, "K": K, "N": N, "bits": [lb, rb],
                "signed": [ls, rs], "auto": auto, "engines": {}}
-        for eng in (["popcount", "mfma"] if engines == "all" else [auto]):
+        for eng in (["popcount", "mfma", "mfma_fp4"] if engines == "all" else [auto]):
             log(f"[bench] {name} {enc}: {eng} x {steps}")
             try:
                 ms = time_gemm(gbs, ws, l, r, d_res, eng, steps, warmup, stream)
-            except gbs.GbsError as e:  # e.g. MFMA cannot hold this product exactly
-                row["engines"][eng] = {"error": str(e)}
+            except gbs.GbsError as e:  # an engine that cannot hold this product exactly
+                row["engines"][eng] = {"not_exact": str(e).split(": ", 1)[-1]}
                 continue
             got = d_res.to_host(np.int32, (N, M))
             if want is None:

@@ -24,8 +24,8 @@ _NP_DT = {np.dtype(np.int8): I8, np.dtype(np.uint8): U8,
           np.dtype(np.int32): I32, np.dtype(np.uint32): U32,
           np.dtype(np.int64): I64, np.dtype(np.uint64): U64}
 # gbs_algo
-AUTO, POPCOUNT, MFMA = 0, 1, 2
-ALGOS = {"auto": AUTO, "popcount": POPCOUNT, "mfma": MFMA}
+AUTO, POPCOUNT, MFMA, MFMA_FP4 = 0, 1, 2, 3
+ALGOS = {"auto": AUTO, "popcount": POPCOUNT, "mfma": MFMA, "mfma_fp4": MFMA_FP4}
 
 EXPORTED = [
     "gbs_init", "gbs_last_error", "gbs_version", "gbs_host_alloc", "gbs_host_free",
@@ -84,7 +84,7 @@ def lib():
         "gbs_batch_run_per_call": [_p, _p], "gbs_batch_info": [_p, _p, _p, _p],
         "gbs_tune_classes": [_p, _p], "gbs_tune_class_of": [_u64, _u64, _u64, _p],
         "gbs_tune_set": [_p, _p, _i], "gbs_tune_load": [C.c_char_p],
-        "gbs_gemm_async_config": [_p, _p, _p, _p, _i, _i, _p],
+        "gbs_gemm_async_config": [_p, _p, _p, _p, _i, _i, _i, _p],
         "gbs_clock_ns": [_p], "gbs_server_create": [_p, _p, _i, _i, _p, _p],
         "gbs_server_destroy": [_p], "gbs_server_submit": [_p, _u64, _p, _i, C.c_int64],
         "gbs_server_step": [_p, _p],

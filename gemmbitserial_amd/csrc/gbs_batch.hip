@@ -1,8 +1,9 @@
 // gbs_batch.hip -- many independent bit-serial GEMMs (uneven M/K/N, mixed
 // bitwidths and signedness) in one persistent kernel launch.
 //
-// Plan (gbs_batch_create): every product that the int8 MFMA engine holds
-// exactly becomes a run of work items -- output tiles of the shape its
+// Plan (gbs_batch_create): every product that the matrix-core engine holds
+// exactly (FP4 copies where both operands fit them, else int8) becomes a
+// run of work items -- output tiles of the shape its
 // shape class maps to in the tuning table, times K splits (deep products
 // are cut into pieces of at most 2 KiB of K, whose partial sums are added
 // with integer atomics into results zeroed first).  Products are ordered by
@@ -88,7 +89,10 @@ __global__ __launch_bounds__(256, 2) void batch_kernel(const BatchProb *probs, u
     plan_decode(pl, item - probs[lo].tile_begin, pt, qt, sp);
     const uint32_t kbeg = sp * pl.kchunk;
     const uint32_t kend = min(ops.kbytes, kbeg + pl.kchunk);
-    mfma_tile_by_shape(pl.shape, ops, pt, qt, kbeg, kend, pl.splits > 1, lds);
+    if (pl.fmt == kFmtF4)
+      mfma_tile_by_shape<kFmtF4>(pl.shape, ops, pt, qt, kbeg, kend, pl.splits > 1, lds);
+    else
+      mfma_tile_by_shape<kFmtI8>(pl.shape, ops, pt, qt, kbeg, kend, pl.splits > 1, lds);
     __syncthreads();
   }
 }
@@ -99,12 +103,13 @@ __global__ __launch_bounds__(256, 2) void batch_kernel(const BatchProb *probs, u
 // widest tile (not carried by the persistent kernel) replaced by 128 x 128,
 // and deep products cut into pieces of at most 16 K units so that no item
 // is much longer than the rest.
-static MfmaPlan batch_plan(uint32_t M, uint32_t N, uint32_t kbytes) {
-  MfmaPlan pl = mfma_plan(M, N, kbytes);
-  if (pl.shape != batch_shape(pl.shape)) pl = mfma_plan(M, N, kbytes, 0, (int)pl.splits);
+static MfmaPlan batch_plan(uint32_t M, uint32_t N, uint32_t kbytes, int fmt) {
+  MfmaPlan pl = mfma_plan(M, N, kbytes, fmt);
+  if (pl.shape != batch_shape(pl.shape, fmt))
+    pl = mfma_plan(M, N, kbytes, fmt, batch_shape(pl.shape, fmt), (int)pl.splits);
   const uint32_t units = kbytes / kMfmaKS;
   const uint32_t want = (units + 15) / 16;
-  if (want > pl.splits) pl = mfma_plan(M, N, kbytes, pl.shape, (int)want);
+  if (want > pl.splits) pl = mfma_plan(M, N, kbytes, fmt, pl.shape, (int)want);
   return pl;
 }
 
@@ -122,7 +127,7 @@ int launch_views(const MfmaOperands *views, int n, void *host_desc, void *d_desc
     if (v.M == 0 || v.N == 0) continue;
     BatchProb d{};
     d.ops = v;
-    d.plan = batch_plan(v.M, v.N, v.kbytes);
+    d.plan = batch_plan(v.M, v.N, v.kbytes, kFmtI8);
     d.tile_begin = tiles;
     tiles += d.plan.tiles();
     if (d.plan.splits > 1) hz[nz++] = ZeroSpan{v.res, (uint64_t)v.M * v.N};
@@ -185,7 +190,7 @@ int gbs_batch_create(const gbs_problem *problems, uint64_t count, gbs_batch **ou
     delete b;
     return rc;
   }
-  struct Item { uint32_t idx; double item_cost; MfmaPlan plan; };
+  struct Item { uint32_t idx; double item_cost; MfmaPlan plan; MfmaOperands ops; };
   std::vector<Item> items;
   for (uint64_t i = 0; i < count; i++) {
     const gbs_problem &p = problems[i];
@@ -212,17 +217,19 @@ int gbs_batch_create(const gbs_problem *problems, uint64_t count, gbs_batch **ou
       return rc;
     }
     const Operand lo = view(p.lhs), ro = view(p.rhs);
-    if (mfma_supported(lo, ro, p.lhs->ncols) && p.lhs->ncols > 0) {
-      rc = ensure_vals(const_cast<gbs_matrix *>(p.lhs), s, true);
-      if (!rc) rc = ensure_vals(const_cast<gbs_matrix *>(p.rhs), s, true);
+    const bool f4 = f4_supported(lo, ro, p.lhs->ncols);
+    if ((f4 || mfma_supported(lo, ro, p.lhs->ncols)) && p.lhs->ncols > 0) {
+      const int fmt = f4 ? kFmtF4 : kFmtI8;
+      rc = ensure_copy(const_cast<gbs_matrix *>(p.lhs), fmt, s, true);
+      if (!rc) rc = ensure_copy(const_cast<gbs_matrix *>(p.rhs), fmt, s, true);
       if (rc) {
         gbs_batch_destroy(b);
         return rc;
       }
-      const uint32_t kb = (uint32_t)((p.lhs->ncols + kMfmaKS - 1) / kMfmaKS * kMfmaKS);
-      const MfmaPlan pl = batch_plan((uint32_t)p.lhs->nrows, (uint32_t)p.rhs->nrows, kb);
+      const MfmaOperands ops = mfma_operands(p.lhs, p.rhs, fmt, p.d_res);
+      const MfmaPlan pl = batch_plan(ops.M, ops.N, ops.kbytes, fmt);
       const MfmaTileShape sh = kMfmaShape[pl.shape];
-      items.push_back({(uint32_t)i, (double)sh.P * sh.Q * (double)pl.kchunk, pl});
+      items.push_back({(uint32_t)i, (double)sh.P * sh.Q * (double)pl.kchunk, pl, ops});
     } else {
       b->separate_idx.push_back((uint32_t)i);
     }
@@ -236,14 +243,7 @@ int gbs_batch_create(const gbs_problem *problems, uint64_t count, gbs_batch **ou
   for (const Item &it : items) {
     const gbs_problem &p = problems[it.idx];
     BatchProb d{};
-    d.ops.R = p.rhs->d_vals;
-    d.ops.L = p.lhs->d_vals;
-    d.ops.r_ld = p.rhs->vals_ld;
-    d.ops.l_ld = p.lhs->vals_ld;
-    d.ops.res = p.d_res;
-    d.ops.M = (uint32_t)p.lhs->nrows;
-    d.ops.N = (uint32_t)p.rhs->nrows;
-    d.ops.kbytes = (uint32_t)((p.lhs->ncols + kMfmaKS - 1) / kMfmaKS * kMfmaKS);
+    d.ops = it.ops;
     d.plan = it.plan;
     d.tile_begin = tiles;
     tiles += d.plan.tiles();

@@ -18,7 +18,11 @@ struct gbs_matrix {
   // first use after each change: [vals_rows][vals_ld], zero padded.
   int8_t *d_vals = nullptr;
   uint64_t vals_rows = 0, vals_ld = 0;
-  uint64_t planes_version = 1, vals_version = 0;
+  // FP4 copy (two values per byte) for operands of at most 2 bits
+  // (3 signed): [f4_rows][f4_ld], zero padded, same lifecycle.
+  int8_t *d_f4 = nullptr;
+  uint64_t f4_rows = 0, f4_ld = 0;
+  uint64_t planes_version = 1, vals_version = 0, f4_version = 0;
   uint64_t wpr() const { return ncols_a / 64; }
   uint64_t wpp() const { return nrows_a * wpr(); }
   uint64_t words() const { return nbits * wpp(); }
@@ -80,19 +84,23 @@ int launch_fill_i32(int32_t *d, uint64_t n, int32_t v, hipStream_t s);
 int launch_popc_gemm(const Operand &l, const Operand &r, uint64_t kwords,
                      uint64_t K, int32_t *d_res, const int32_t *d_rowsum,
                      hipStream_t s);
-// int8 MFMA engine (gbs_mfma.hip).  ensure_vals makes m->d_vals current
-// (allocating it on first use when may_allocate).
+// Matrix-core engine (gbs_mfma.hip).  ensure_vals / ensure_f4 make the
+// int8 / FP4 copy current (allocating it on first use when may_allocate);
+// ensure_copy picks by format (0 int8, 1 FP4).
 int ensure_vals(gbs_matrix *m, hipStream_t s, bool may_allocate);
+int ensure_f4(gbs_matrix *m, hipStream_t s, bool may_allocate);
+int ensure_copy(gbs_matrix *m, int fmt, hipStream_t s, bool may_allocate);
 int launch_mfma_gemm(const gbs_matrix *l, const gbs_matrix *r, int32_t *d_res,
-                     hipStream_t s, int forced_shape = -1, bool allow_split = true,
+                     hipStream_t s, int fmt, int forced_shape = -1, bool allow_split = true,
                      int forced_splits = 0);
 // tuning table of the MFMA engine: tile shape and K split per shape class
 // (shapes == nullptr restores the built-in heuristic; splits 0 = automatic)
 int set_mfma_table(const int *shapes, const int *splits, int n);
-bool mfma_supported(const Operand &l, const Operand &r, uint64_t K);
+bool mfma_supported(const Operand &l, const Operand &r, uint64_t K);  // int8 copies exact
+bool f4_supported(const Operand &l, const Operand &r, uint64_t K);    // FP4 copies exact
 gbs_algo select_algo(const Operand &l, const Operand &r, uint64_t K);
 constexpr uint64_t kValsRowAlign = 256;  // rows of the int8 copy
-constexpr uint64_t kValsColAlign = 256;  // bytes per row of the int8 copy
+constexpr uint64_t kValsColAlign = 256;  // values per row of a copy (int8: bytes)
 
 size_t dtype_size(gbs_dtype dt);
 

@@ -1,28 +1,35 @@
-// gbs_mfma.hip -- exact int8 MFMA engine for the bit-serial GEMM on gfx950.
+// gbs_mfma.hip -- exact matrix-core engine for the bit-serial GEMM on gfx950.
 //
-// Few-bit values fit int8 exactly: bipolar +-1, signed two's complement of
-// up to 8 bits, unsigned codes of up to 7 bits.  A matrix's planes are
-// expanded once per change into an int8 copy ("unpacked values",
-// [align(rows,256)][align(K,256)], zero padded), and the product is a plain
-// int8 GEMM on the matrix cores:
+// A matrix's planes are expanded once per change into a value copy
+// ([align(rows,256)][align(K,256) values], zero padded) and the product is
+// a plain GEMM on the matrix cores:
 //
 //   res[n*M + m] = sum_k val(rhs[n,k]) * val(lhs[m,k])      (int32)
 //
-// It is exact -- no wrap, no rounding -- whenever
-// K * max|val(lhs)| * max|val(rhs)| < 2^31 (checked by mfma_supported), in
-// which case it equals the reference's int32 result bit for bit.
+// Two copy formats:
+//  * int8 (one byte per value): bipolar +-1, signed two's complement of up
+//    to 8 bits, unsigned codes of up to 7 bits; int32 accumulation, exact
+//    whenever K * max|val(lhs)| * max|val(rhs)| < 2^31 (mfma_supported);
+//  * FP4 E2M1 (two values per byte, scale 2^1, so codes 0..4 are the
+//    integers 0..4): bipolar, unsigned of up to 2 bits, signed of up to 3
+//    bits; fp32 accumulation of integer products, exact whenever
+//    K * max|val(lhs)| * max|val(rhs)| < 2^24 (f4_supported), since every
+//    partial sum is then an integer below 2^24.  Half the bytes of int8 and
+//    twice the MACs per instruction.
+// Either way the result equals the reference's int32 result bit for bit.
 //
-// The tile itself (gbs_mfma_tile.h) comes in eight shapes; a product runs
-// the shape and K split its (M, N, K) class maps to in the tuning table
-// (written by gemmbitserial_amd/tune.py and loaded through gbs_tune_load,
-// else a size heuristic).  Split products zero the result first and add
-// their partial sums with integer atomics, which is exact in any order.
+// The tile itself (gbs_mfma_tile.h) comes in several shapes; a product runs
+// the shape and K split its (M, N, K) class maps to in the tuning table of
+// its format (written by gemmbitserial_amd/tune.py and loaded through
+// gbs_tune_load, else a size heuristic).  Split products zero the result
+// first and add their partial sums with integer atomics, which is exact in
+// any order.
 #include "gbs_mfma_tile.h"
 
 namespace gbs {
 namespace {
 
-template <int P, int Q, int WP, int KSB, int DEPTH>
+template <int P, int Q, int WP, int KSB, int DEPTH, int FMT>
 __global__ __launch_bounds__(256, (P * Q >= 65536 || (P * Q >= 32768 && DEPTH == 0)) ? 1 : 2)
 void mfma_kernel(
     MfmaOperands a, MfmaPlan pl) {
@@ -35,9 +42,9 @@ void mfma_kernel(
   const uint32_t kbeg = sp * pl.kchunk;
   const uint32_t kend = min(a.kbytes, kbeg + pl.kchunk);
   if constexpr (DEPTH == 0)
-    mfma_tile_pp<P, Q, WP>(a, pt, qt, kbeg, kend, pl.splits > 1, lds);
+    mfma_tile_pp<P, Q, WP, FMT>(a, pt, qt, kbeg, kend, pl.splits > 1, lds);
   else
-    mfma_tile<P, Q, WP, KSB, DEPTH>(a, pt, qt, kbeg, kend, pl.splits > 1, lds);
+    mfma_tile<P, Q, WP, KSB, DEPTH, FMT>(a, pt, qt, kbeg, kend, pl.splits > 1, lds);
 }
 
 // planes -> int8 values, 4 bytes per thread.  Zero outside the real matrix.
@@ -67,13 +74,47 @@ __global__ __launch_bounds__(256) void expand_kernel(
   reinterpret_cast<uint32_t *>(out)[i] = packed;
 }
 
+// planes -> FP4 codes: one thread per 64-value word of a copy row (32
+// bytes; value k of the word in nibble k, low nibble first).  Code of an
+// integer v (|v| <= 4) under scale 2^1: sign bit 8 | |v|.  Zero outside the
+// real matrix, also for bipolar operands.
+__global__ __launch_bounds__(256) void expand_f4_kernel(
+    const uint64_t *planes, uint64_t nrows, uint64_t ncols, uint64_t wpr, uint64_t W,
+    uint32_t nbits, bool issigned, uint64_t rows_a, uint64_t ld, uint8_t *out) {
+  const uint64_t wpl = ld / 32;
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows_a * wpl) return;
+  const uint64_t r = i / wpl, w = i - r * wpl;
+  uint32_t q[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (r < nrows && w * 64 < ncols) {
+    uint64_t p0 = planes[r * wpr + w], p1 = 0, p2 = 0;
+    if (nbits > 1) p1 = planes[W + r * wpr + w];
+    if (nbits > 2) p2 = planes[2 * W + r * wpr + w];
+    const uint64_t valid = ncols - w * 64;
+    const bool bip = nbits == 1 && issigned;
+#pragma unroll
+    for (int k = 0; k < 64; k++) {
+      const uint32_t code = (uint32_t)((p0 >> k) & 1) | (uint32_t)((p1 >> k) & 1) << 1 |
+                            (uint32_t)((p2 >> k) & 1) << 2;
+      uint32_t nib;
+      if (bip) nib = code ? 0x1u : 0x9u;
+      else if (issigned && (code >> (nbits - 1))) nib = 0x8u | ((1u << nbits) - code);
+      else nib = code;
+      if ((uint64_t)k < valid) q[k >> 3] |= nib << (4 * (k & 7));
+    }
+  }
+  uint4 *o = reinterpret_cast<uint4 *>(out + r * ld + w * 32);
+  o[0] = make_uint4(q[0], q[1], q[2], q[3]);
+  o[1] = make_uint4(q[4], q[5], q[6], q[7]);
+}
+
 uint64_t max_abs(const Operand &o) {
   if (o.bipolar) return 1;
   if (o.issigned) return 1ull << (o.nbits - 1);
   return (1ull << o.nbits) - 1;
 }
 
-template <int P, int Q, int WP, int KSB, int DEPTH>
+template <int P, int Q, int WP, int KSB, int DEPTH, int FMT>
 int run_plan(const MfmaOperands &a, const MfmaPlan &pl, hipStream_t s) {
   if (pl.splits > 1) {
     int rc = launch_fill_i32(a.res, (uint64_t)a.M * a.N, 0, s);
@@ -82,33 +123,39 @@ int run_plan(const MfmaOperands &a, const MfmaPlan &pl, hipStream_t s) {
   const uint64_t grid = pl.tiles();
   if (grid == 0) return GBS_OK;
   if (grid > 0x7fffffffull) return fail(GBS_ERANGE, "mfma grid too large");
-  hipLaunchKernelGGL((mfma_kernel<P, Q, WP, KSB, DEPTH>), dim3((uint32_t)grid), dim3(256), 0,
-                     s, a, pl);
+  hipLaunchKernelGGL((mfma_kernel<P, Q, WP, KSB, DEPTH, FMT>), dim3((uint32_t)grid), dim3(256),
+                     0, s, a, pl);
   GBS_HIP(hipGetLastError());
   return GBS_OK;
 }
 
-int run_plan_id(const MfmaOperands &a, const MfmaPlan &pl, hipStream_t s) {
+template <int FMT>
+int run_plan_fmt(const MfmaOperands &a, const MfmaPlan &pl, hipStream_t s) {
   switch (pl.shape) {
-    case 0: return run_plan<128, 128, 2, 128, 2>(a, pl, s);
-    case 1: return run_plan<64, 128, 2, 128, 2>(a, pl, s);
-    case 2: return run_plan<32, 256, 1, 64, 2>(a, pl, s);
-    case 3: return run_plan<16, 256, 1, 128, 2>(a, pl, s);
-    case 4: return run_plan<64, 64, 2, 128, 2>(a, pl, s);
-    case 5: return run_plan<128, 64, 4, 128, 2>(a, pl, s);
-    case 6: return run_plan<128, 128, 2, 64, 1>(a, pl, s);
-    case 7: return run_plan<128, 256, 2, 64, 2>(a, pl, s);
-    case 8: return run_plan<128, 128, 2, 64, 0>(a, pl, s);
-    case 9: return run_plan<128, 256, 2, 64, 0>(a, pl, s);
-    case 10: return run_plan<64, 128, 2, 64, 0>(a, pl, s);
-    case 11: return run_plan<256, 256, 2, 64, 1>(a, pl, s);
-    case 12: return run_plan<256, 256, 2, 128, 1>(a, pl, s);
+    case 0: return run_plan<128, 128, 2, 128, 2, FMT>(a, pl, s);
+    case 1: return run_plan<64, 128, 2, 128, 2, FMT>(a, pl, s);
+    case 2: return run_plan<32, 256, 1, 64, 2, FMT>(a, pl, s);
+    case 3: return run_plan<16, 256, 1, 128, 2, FMT>(a, pl, s);
+    case 4: return run_plan<64, 64, 2, 128, 2, FMT>(a, pl, s);
+    case 5: return run_plan<128, 64, 4, 128, 2, FMT>(a, pl, s);
+    case 6: return run_plan<128, 128, 2, 64, 1, FMT>(a, pl, s);
+    case 7: return run_plan<128, 256, 2, 64, 2, FMT>(a, pl, s);
+    case 8: return run_plan<128, 128, 2, 64, 0, FMT>(a, pl, s);
+    case 9: return run_plan<128, 256, 2, 64, 0, FMT>(a, pl, s);
+    case 10: return run_plan<64, 128, 2, 64, 0, FMT>(a, pl, s);
+    case 11: return run_plan<256, 256, 2, 64, 1, FMT>(a, pl, s);
+    case 12: return run_plan<256, 256, 2, 128, 1, FMT>(a, pl, s);
   }
   return fail(GBS_EINVAL, "mfma: unknown tile shape %d", pl.shape);
 }
 
-int g_shape[kClasses], g_splits[kClasses];
-bool g_table_set = false;
+int run_plan_id(const MfmaOperands &a, const MfmaPlan &pl, hipStream_t s) {
+  if (pl.fmt == kFmtF4) return run_plan_fmt<kFmtF4>(a, pl, s);
+  return run_plan_fmt<kFmtI8>(a, pl, s);
+}
+
+int g_shape[kMfmaFormats][kClasses], g_splits[kMfmaFormats][kClasses];
+bool g_table_set[kMfmaFormats] = {false, false};
 
 int default_shape(uint64_t M, uint64_t N) {
   (void)M;
@@ -126,8 +173,23 @@ bool mfma_supported(const Operand &l, const Operand &r, uint64_t K) {
   return bound < ((unsigned __int128)1 << 31);
 }
 
+// Values an FP4 code holds as an integer under scale 2^1: -4..4 (codes
+// 0..4 with a sign bit); 3 is the largest unsigned 2-bit code, -4 the
+// smallest signed 3-bit one.
+static bool f4_holds(const Operand &o) {
+  if (o.bipolar) return true;
+  return o.issigned ? o.nbits <= 3 : o.nbits <= 2;
+}
+
+bool f4_supported(const Operand &l, const Operand &r, uint64_t K) {
+  if (!f4_holds(l) || !f4_holds(r)) return false;
+  const unsigned __int128 bound = (unsigned __int128)K * max_abs(l) * max_abs(r);
+  return bound < ((unsigned __int128)1 << 24);
+}
+
 gbs_algo select_algo(const Operand &l, const Operand &r, uint64_t K) {
-  if (!mfma_supported(l, r, K)) return GBS_ALGO_POPCOUNT;
+  const bool f4 = f4_supported(l, r, K);
+  if (!f4 && !mfma_supported(l, r, K)) return GBS_ALGO_POPCOUNT;
   const double M = (double)l.nrows, N = (double)r.nrows, Kd = (double)K;
   const double pl = l.bipolar && r.bipolar ? 1 : (double)l.nbits;
   const double pr = l.bipolar && r.bipolar ? 1 : (double)r.nbits;
@@ -136,17 +198,22 @@ gbs_algo select_algo(const Operand &l, const Operand &r, uint64_t K) {
   const double valu = M * N * ((K + 31) / 32) * 2 * pl * pr / (0.55 * 39.3e12);
   const double pbytes = (M * Kd * l.nbits + N * Kd * r.nbits) / 8 + 4 * M * N;
   const double t_popc = (valu > pbytes / (0.6 * 8e12) ? valu : pbytes / (0.6 * 8e12));
-  const double mops = 2 * M * N * Kd / (0.35 * 5e15);
-  const double ibytes = (M + N) * Kd + 4 * M * N;
+  // matrix cores: the same instruction stream per byte of K for both
+  // formats, FP4 packing twice the values per byte
+  const int fmt = f4 ? kFmtF4 : kFmtI8;
+  const double vpb = f4 ? 2 : 1;  // values per byte of the copy
+  const double mops = 2 * M * N * Kd / vpb / (0.35 * 5e15);
+  const double ibytes = (M + N) * Kd / vpb + 4 * M * N;
   // a tile row/column only partly filled still costs the whole tile
   int shape_id, splits;
-  mfma_config_for(l.nrows, r.nrows, K, &shape_id, &splits);
+  mfma_config_for(l.nrows, r.nrows, K, fmt, &shape_id, &splits);
   const MfmaTileShape sh = kMfmaShape[shape_id];
   const double waste = (N < sh.P ? sh.P / (N > 1 ? N : 1) : 1) *
                        (M < sh.Q ? sh.Q / (M > 1 ? M : 1) : 1);
   const double t_mfma_c = mops * waste;
   const double t_mfma = (t_mfma_c > ibytes / (0.6 * 8e12) ? t_mfma_c : ibytes / (0.6 * 8e12));
-  return t_mfma < t_popc ? GBS_ALGO_MFMA : GBS_ALGO_POPCOUNT;
+  if (t_mfma >= t_popc) return GBS_ALGO_POPCOUNT;
+  return f4 ? GBS_ALGO_MFMA_FP4 : GBS_ALGO_MFMA;
 }
 
 int ensure_vals(gbs_matrix *m, hipStream_t s, bool may_allocate) {
@@ -170,45 +237,82 @@ int ensure_vals(gbs_matrix *m, hipStream_t s, bool may_allocate) {
   return GBS_OK;
 }
 
+int ensure_f4(gbs_matrix *m, hipStream_t s, bool may_allocate) {
+  if (m->f4_version == m->planes_version && m->d_f4) return GBS_OK;
+  if (!m->d_f4) {
+    const uint64_t rows = ((m->nrows + kValsRowAlign - 1) / kValsRowAlign) * kValsRowAlign;
+    const uint64_t ld = ((m->ncols + kValsColAlign - 1) / kValsColAlign) * kValsColAlign / 2;
+    const uint64_t rows_a = rows ? rows : kValsRowAlign, ld_a = ld ? ld : kValsColAlign / 2;
+    if (!may_allocate)
+      return fail(GBS_ERANGE, "mfma: operand has no FP4 copy yet; call gbs_workspace_reserve");
+    GBS_HIP(hipMalloc((void **)&m->d_f4, rows_a * ld_a));
+    m->f4_rows = rows_a;
+    m->f4_ld = ld_a;
+  }
+  const uint64_t n = m->f4_rows * (m->f4_ld / 32);
+  hipLaunchKernelGGL(expand_f4_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s,
+                     m->d_planes, m->nrows, m->ncols, m->wpr(), m->wpp(), (uint32_t)m->nbits,
+                     m->issigned, m->f4_rows, m->f4_ld, (uint8_t *)m->d_f4);
+  GBS_HIP(hipGetLastError());
+  m->f4_version = m->planes_version;
+  return GBS_OK;
+}
+
+int ensure_copy(gbs_matrix *m, int fmt, hipStream_t s, bool may_allocate) {
+  return fmt == kFmtF4 ? ensure_f4(m, s, may_allocate) : ensure_vals(m, s, may_allocate);
+}
+
+MfmaOperands mfma_operands(const gbs_matrix *l, const gbs_matrix *r, int fmt, int32_t *d_res) {
+  MfmaOperands a{};
+  const bool f4 = fmt == kFmtF4;
+  a.R = f4 ? r->d_f4 : r->d_vals;
+  a.L = f4 ? l->d_f4 : l->d_vals;
+  a.r_ld = f4 ? r->f4_ld : r->vals_ld;
+  a.l_ld = f4 ? l->f4_ld : l->vals_ld;
+  a.res = d_res;
+  a.M = (uint32_t)l->nrows;
+  a.N = (uint32_t)r->nrows;
+  const uint64_t bytes = f4 ? (l->ncols + 1) / 2 : l->ncols;
+  a.kbytes = (uint32_t)((bytes + kMfmaKS - 1) / kMfmaKS * kMfmaKS);
+  return a;
+}
+
 int launch_mfma_gemm(const gbs_matrix *l, const gbs_matrix *r, int32_t *d_res,
-                     hipStream_t s, int forced_shape, bool allow_split, int forced_splits) {
+                     hipStream_t s, int fmt, int forced_shape, bool allow_split,
+                     int forced_splits) {
   const uint64_t M = l->nrows, N = r->nrows;
   if (M == 0 || N == 0) return GBS_OK;
   if (M > 0x7fffffffull || N > 0x7fffffffull)
     return fail(GBS_ERANGE, "gemm dimensions exceed 2^31");
-  if (l->vals_ld != r->vals_ld || !l->d_vals || !r->d_vals)
-    return fail(GBS_EINVAL, "mfma: operand int8 copies are not current");
-  MfmaOperands a{};
-  a.R = r->d_vals;
-  a.L = l->d_vals;
-  a.r_ld = r->vals_ld;
-  a.l_ld = l->vals_ld;
-  a.res = d_res;
-  a.M = (uint32_t)M;
-  a.N = (uint32_t)N;
-  a.kbytes = (uint32_t)((l->ncols + kMfmaKS - 1) / kMfmaKS * kMfmaKS);
+  const MfmaOperands a = mfma_operands(l, r, fmt, d_res);
+  if (!a.R || !a.L || a.r_ld != a.l_ld ||
+      (fmt == kFmtF4 ? l->f4_version != l->planes_version || r->f4_version != r->planes_version
+                     : l->vals_version != l->planes_version ||
+                           r->vals_version != r->planes_version))
+    return fail(GBS_EINVAL, "mfma: operand value copies are not current");
   if (a.kbytes == 0) return launch_fill_i32(d_res, M * N, 0, s);
-  const MfmaPlan pl = mfma_plan(a.M, a.N, a.kbytes, forced_shape,
+  const MfmaPlan pl = mfma_plan(a.M, a.N, a.kbytes, fmt, forced_shape,
                                 forced_splits > 0 ? forced_splits : (allow_split ? 0 : 1));
   return run_plan_id(a, pl, s);
 }
 
-void mfma_config_for(uint64_t M, uint64_t N, uint64_t K, int *shape, int *splits) {
+void mfma_config_for(uint64_t M, uint64_t N, uint64_t K, int fmt, int *shape, int *splits) {
   *shape = default_shape(M, N);
   *splits = 0;
-  if (g_table_set) {
+  if (fmt < 0 || fmt >= kMfmaFormats) fmt = kFmtI8;
+  if (g_table_set[fmt]) {
     const int c = shape_class(M, N, K);
-    if (g_shape[c] >= 0 && g_shape[c] < kMfmaShapes) {
-      *shape = g_shape[c];
-      *splits = g_splits[c];
+    if (g_shape[fmt][c] >= 0 && g_shape[fmt][c] < kMfmaShapes) {
+      *shape = g_shape[fmt][c];
+      *splits = g_splits[fmt][c];
     }
   }
 }
 
-MfmaPlan mfma_plan(uint32_t M, uint32_t N, uint32_t kbytes, int forced_shape,
+MfmaPlan mfma_plan(uint32_t M, uint32_t N, uint32_t kbytes, int fmt, int forced_shape,
                    int forced_splits) {
   int shape, splits;
-  mfma_config_for(M, N, kbytes, &shape, &splits);
+  mfma_config_for(M, N, (uint64_t)kbytes * (fmt == kFmtF4 ? 2 : 1), fmt, &shape, &splits);
   if (forced_shape >= 0) {
     shape = forced_shape;
     splits = 0;
@@ -217,6 +321,7 @@ MfmaPlan mfma_plan(uint32_t M, uint32_t N, uint32_t kbytes, int forced_shape,
   const MfmaTileShape sh = kMfmaShape[shape];
   MfmaPlan pl{};
   pl.shape = shape;
+  pl.fmt = fmt;
   pl.ptiles = (N + sh.P - 1) / sh.P;
   pl.qtiles = (M + sh.Q - 1) / sh.Q;
   // XCD groups share the larger operand: lhs larger -> rhs tiles fastest
@@ -239,15 +344,25 @@ MfmaPlan mfma_plan(uint32_t M, uint32_t N, uint32_t kbytes, int forced_shape,
 
 int set_mfma_table(const int *shapes, const int *splits, int n) {
   if (!shapes) {
-    g_table_set = false;
+    for (int f = 0; f < kMfmaFormats; f++) g_table_set[f] = false;
     return GBS_OK;
   }
-  if (n != kClasses) return fail(GBS_EINVAL, "tile table needs %d entries, got %d", kClasses, n);
-  for (int i = 0; i < kClasses; i++) {
-    g_shape[i] = shapes[i];
-    g_splits[i] = splits ? splits[i] : 0;
+  if (n != kClasses && n != kMfmaFormats * kClasses)
+    return fail(GBS_EINVAL, "tile table needs %d (int8) or %d (int8, FP4) entries, got %d",
+                kClasses, kMfmaFormats * kClasses, n);
+  for (int f = 0; f < kMfmaFormats; f++) {
+    if (f * kClasses >= n) {
+      g_table_set[f] = false;
+      continue;
+    }
+    bool any = false;
+    for (int i = 0; i < kClasses; i++) {
+      g_shape[f][i] = shapes[f * kClasses + i];
+      g_splits[f][i] = splits ? splits[f * kClasses + i] : 0;
+      any |= g_shape[f][i] >= 0;
+    }
+    g_table_set[f] = any;
   }
-  g_table_set = true;
   return GBS_OK;
 }
 

@@ -1,18 +1,22 @@
-// gbs_mfma_tile.h -- one output tile of the exact int8 MFMA GEMM, shared by
-// the single-product kernel (gbs_mfma.hip) and the batched persistent kernel
-// (gbs_batch.hip).
+// gbs_mfma_tile.h -- one output tile of the exact matrix-core GEMM, shared
+// by the single-product kernel (gbs_mfma.hip) and the batched persistent
+// kernel (gbs_batch.hip).
 //
 // Tile = P rhs rows (n) x Q lhs rows (m) of res; 256 threads = 4 waves laid
 // out WP x (4/WP) over it; each wave owns (P/WP) x (Q/(4/WP)) outputs as
-// 16 x 16 fragments of v_mfma_i32_16x16x64_i8.  The rhs operand goes to the
-// MFMA A port so that a fragment's output lanes run along m, the contiguous
-// dimension of res.  Both operand fragments take 16 contiguous bytes of a
-// row at k-offset 16*(lane/16) -- the same k-to-(lane, byte) assignment on
-// both ports, so the sum over k does not depend on the order the
-// instruction adds in.  K is staged KSB = 64 or 128 bytes at a time in LDS
-// (row stride KSB + 16 B: the 16 rows one fragment load touches fall in
-// distinct banks), double-buffered in LDS, with 16-byte global loads held in
-// registers one or two stages ahead of the stage being multiplied.
+// 16 x 16 fragments.  The rhs operand goes to the MFMA A port so that a
+// fragment's output lanes run along m, the contiguous dimension of res.
+// Two operand formats share the tile: FMT 0 = int8 values and
+// v_mfma_i32_16x16x64_i8, FMT 1 = FP4 (E2M1) codes, two per byte, and
+// v_mfma_scale_f32_16x16x128_f8f6f4 with both block scales 2^1.  Either
+// way one instruction consumes 16 bytes of each operand row per lane and
+// 64 bytes of K per fragment, so staging is byte-for-byte the same.  Both
+// operand fragments take 16 contiguous bytes of a row at k-offset
+// 16*(lane/16) -- the same k-to-(lane, byte) assignment on both ports, so
+// the sum over k does not depend on the order the instruction adds in.  K
+// is staged KSB = 64, 128 or 256 bytes at a time in LDS, double-buffered,
+// with 16-byte global loads held in registers one or two stages ahead of
+// the stage being multiplied.
 #pragma once
 
 #include <type_traits>
@@ -22,16 +26,45 @@
 namespace gbs {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v8i __attribute__((ext_vector_type(8)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// Operand formats of the matrix-core engine.
+constexpr int kFmtI8 = 0, kFmtF4 = 1, kMfmaFormats = 2;
+// E8M0 block scale 2^1: FP4 code c (|c| <= 4: 0, 0.5, 1, 1.5, 2) then
+// stands for the integer c, so products and their fp32 sums are exact
+// integers while |sum| < 2^24.
+constexpr int kF4Scale = 128;
+
+template <int FMT>
+using MfmaAcc = typename std::conditional<FMT == kFmtI8, v4i, v4f>::type;
+
+template <int FMT>
+__device__ __forceinline__ MfmaAcc<FMT> mfma_op(const v4i &a, const v4i &b,
+                                                const MfmaAcc<FMT> &c) {
+  if constexpr (FMT == kFmtI8) {
+    return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
+  } else {
+    // FP4 operands occupy the low four registers of the 8-register slots
+    const v8i a8 = {a[0], a[1], a[2], a[3], 0, 0, 0, 0};
+    const v8i b8 = {b[0], b[1], b[2], b[3], 0, 0, 0, 0};
+    return __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a8, b8, c, 4, 4, 0, kF4Scale, 0,
+                                                            kF4Scale);
+  }
+}
+__device__ __forceinline__ int32_t acc_value(int32_t x) { return x; }
+__device__ __forceinline__ int32_t acc_value(float x) { return (int32_t)x; }  // exact integer
 
 constexpr int kMfmaKS = 128;  // padding unit of K (bytes): every stage size divides it
 
-// Operands of one product: int8 value copies [rows >= tile-aligned][ld].
+// Operands of one product: value copies (int8 or FP4, same format on both
+// sides) [rows >= tile-aligned][ld bytes].
 struct MfmaOperands {
   const int8_t *R, *L;   // rhs / lhs values
   uint64_t r_ld, l_ld;   // row strides (bytes)
   int32_t *res;          // N x M
   uint32_t M, N;         // lhs rows, rhs rows
-  uint32_t kbytes;       // depth rounded up to kMfmaKS (zero padded)
+  uint32_t kbytes;       // bytes of K in the copies, rounded up to kMfmaKS (zero padded)
 };
 
 // Tile shapes the engine is compiled for: P x Q outputs, WP waves along P,
@@ -64,7 +97,7 @@ constexpr size_t mfma_lds_bytes() { return (size_t)2 * (P + Q) * lds_row_bytes<K
 constexpr size_t kLdsPerCU = 160 * 1024;
 constexpr size_t kMfmaLdsMax = (size_t)2 * 272 * 128;  // persistent kernel: largest carried shape (16 x 256)
 
-template <int P, int Q, int WP, int KSB, int DEPTH>
+template <int P, int Q, int WP, int KSB, int DEPTH, int FMT = kFmtI8>
 __device__ __forceinline__ void mfma_tile(const MfmaOperands &a, uint32_t pt, uint32_t qt,
                                           uint32_t kbeg, uint32_t kend, bool atomic,
                                           int8_t *lds) {
@@ -117,11 +150,11 @@ __device__ __forceinline__ void mfma_tile(const MfmaOperands &a, uint32_t pt, ui
     }
   };
 
-  v4i acc[FP][FQ];
+  MfmaAcc<FMT> acc[FP][FQ];
 #pragma unroll
   for (int i = 0; i < FP; i++)
 #pragma unroll
-    for (int j = 0; j < FQ; j++) acc[i][j] = v4i{0, 0, 0, 0};
+    for (int j = 0; j < FQ; j++) acc[i][j] = MfmaAcc<FMT>{0, 0, 0, 0};
 
   const int fr = lane & 15;
   auto compute = [&](int buf) {
@@ -144,7 +177,7 @@ __device__ __forceinline__ void mfma_tile(const MfmaOperands &a, uint32_t pt, ui
       for (int i = 0; i < FP; i++)
 #pragma unroll
         for (int j = 0; j < FQ; j++)
-          acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(af[i], bf[j], acc[i][j], 0, 0, 0);
+          acc[i][j] = mfma_op<FMT>(af[i], bf[j], acc[i][j]);
     }
   };
 
@@ -195,8 +228,9 @@ __device__ __forceinline__ void mfma_tile(const MfmaOperands &a, uint32_t pt, ui
         const uint32_t m = q0 + wq * QW + j * 16 + fr;
         if (n < a.N && m < a.M) {
           const uint64_t idx = (uint64_t)n * a.M + m;
-          if (atomic) atomicAdd(a.res + idx, acc[i][j][r]);
-          else a.res[idx] = acc[i][j][r];
+          const int32_t v = acc_value(acc[i][j][r]);
+          if (atomic) atomicAdd(a.res + idx, v);
+          else a.res[idx] = v;
         }
       }
 }
@@ -216,7 +250,7 @@ __device__ __forceinline__ void lds_barrier() {
 // loads of stage i+2, passes one LDS-only barrier, and reads stage i+1's
 // fragments -- those LDS reads and the global loads overlap the MFMAs of
 // stage i.
-template <int P, int Q, int WP>
+template <int P, int Q, int WP, int FMT = kFmtI8>
 __device__ __forceinline__ void mfma_tile_pp(const MfmaOperands &a, uint32_t pt, uint32_t qt,
                                              uint32_t kbeg, uint32_t kend, bool atomic,
                                              int8_t *lds) {
@@ -274,17 +308,17 @@ __device__ __forceinline__ void mfma_tile_pp(const MfmaOperands &a, uint32_t pt,
     for (int j = 0; j < FQ; j++)
       xb[j] = *reinterpret_cast<const v4i *>(&sL[(wq * QW + j * 16 + fr) * LDR + fk]);
   };
-  v4i acc[FP][FQ];
+  MfmaAcc<FMT> acc[FP][FQ];
 #pragma unroll
   for (int i = 0; i < FP; i++)
 #pragma unroll
-    for (int j = 0; j < FQ; j++) acc[i][j] = v4i{0, 0, 0, 0};
+    for (int j = 0; j < FQ; j++) acc[i][j] = MfmaAcc<FMT>{0, 0, 0, 0};
   auto mma = [&](const v4i(&xa)[FP], const v4i(&xb)[FQ]) {
 #pragma unroll
     for (int i = 0; i < FP; i++)
 #pragma unroll
       for (int j = 0; j < FQ; j++)
-        acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(xa[i], xb[j], acc[i][j], 0, 0, 0);
+        acc[i][j] = mfma_op<FMT>(xa[i], xb[j], acc[i][j]);
   };
 
   const uint32_t nst = kend > kbeg ? (kend - kbeg) / KSB : 0;
@@ -320,8 +354,9 @@ __device__ __forceinline__ void mfma_tile_pp(const MfmaOperands &a, uint32_t pt,
         const uint32_t m = q0 + wq * QW + j * 16 + fr;
         if (n < a.N && m < a.M) {
           const uint64_t idx = (uint64_t)n * a.M + m;
-          if (atomic) atomicAdd(a.res + idx, acc[i][j][r]);
-          else a.res[idx] = acc[i][j][r];
+          const int32_t v = acc_value(acc[i][j][r]);
+          if (atomic) atomicAdd(a.res + idx, v);
+          else a.res[idx] = v;
         }
       }
 }
@@ -329,14 +364,18 @@ __device__ __forceinline__ void mfma_tile_pp(const MfmaOperands &a, uint32_t pt,
 // Shapes the persistent kernel carries: all but the 128 x 256 and 256 x 256
 // ones, whose registers or LDS would make the shared kernel spill or drop to
 // one block per CU; they map to the 128 x 128 shape of the same pipeline.
-__host__ __device__ constexpr int batch_shape(int s) {
-  return s == 7 || s == 11 || s == 12 ? 0 : s == 9 ? 8 : s;
+// With FP4 copies the 128 x 128 tile with 128-byte stages maps to the
+// 64-byte-stage one (the FP4 code path holds more registers).
+__host__ __device__ constexpr int batch_shape(int s, int fmt = 0) {
+  return fmt == 1 && (s == 0 || s == 7 || s == 11 || s == 12) ? 6
+         : s == 7 || s == 11 || s == 12                      ? 0
+         : s == 9                                              ? 8
+                                                               : s;
 }
 
-// Dispatch one tile by shape id (the persistent kernel's per-product switch).
 constexpr bool batch_shapes_fit_lds() {
   for (int s = 0; s < kMfmaShapes; s++) {
-    const MfmaTileShape t = kMfmaShape[batch_shape(s)];
+    const MfmaTileShape t = kMfmaShape[batch_shape(s)];  // FP4 maps a subset of these
     const int row = t.KSB >= 128 ? t.KSB : t.KSB + 16;
     if ((size_t)2 * (t.P + t.Q) * row > kMfmaLdsMax) return false;
   }
@@ -344,19 +383,23 @@ constexpr bool batch_shapes_fit_lds() {
 }
 static_assert(batch_shapes_fit_lds(), "a shape of the persistent kernel overflows its LDS");
 
+template <int FMT>
 __device__ __forceinline__ void mfma_tile_by_shape(int shape, const MfmaOperands &a,
                                                    uint32_t pt, uint32_t qt, uint32_t kbeg,
                                                    uint32_t kend, bool atomic, int8_t *lds) {
   switch (shape) {
-    case 0: mfma_tile<128, 128, 2, 128, 2>(a, pt, qt, kbeg, kend, atomic, lds); break;
-    case 1: mfma_tile<64, 128, 2, 128, 2>(a, pt, qt, kbeg, kend, atomic, lds); break;
-    case 2: mfma_tile<32, 256, 1, 64, 2>(a, pt, qt, kbeg, kend, atomic, lds); break;
-    case 3: mfma_tile<16, 256, 1, 128, 2>(a, pt, qt, kbeg, kend, atomic, lds); break;
-    case 4: mfma_tile<64, 64, 2, 128, 2>(a, pt, qt, kbeg, kend, atomic, lds); break;
-    case 5: mfma_tile<128, 64, 4, 128, 2>(a, pt, qt, kbeg, kend, atomic, lds); break;
-    case 6: mfma_tile<128, 128, 2, 64, 1>(a, pt, qt, kbeg, kend, atomic, lds); break;
-    case 8: mfma_tile_pp<128, 128, 2>(a, pt, qt, kbeg, kend, atomic, lds); break;
-    default: mfma_tile_pp<64, 128, 2>(a, pt, qt, kbeg, kend, atomic, lds); break;
+    case 0:
+      if constexpr (FMT == kFmtI8)
+        mfma_tile<128, 128, 2, 128, 2, FMT>(a, pt, qt, kbeg, kend, atomic, lds);
+      break;
+    case 1: mfma_tile<64, 128, 2, 128, 2, FMT>(a, pt, qt, kbeg, kend, atomic, lds); break;
+    case 2: mfma_tile<32, 256, 1, 64, 2, FMT>(a, pt, qt, kbeg, kend, atomic, lds); break;
+    case 3: mfma_tile<16, 256, 1, 128, 2, FMT>(a, pt, qt, kbeg, kend, atomic, lds); break;
+    case 4: mfma_tile<64, 64, 2, 128, 2, FMT>(a, pt, qt, kbeg, kend, atomic, lds); break;
+    case 5: mfma_tile<128, 64, 4, 128, 2, FMT>(a, pt, qt, kbeg, kend, atomic, lds); break;
+    case 6: mfma_tile<128, 128, 2, 64, 1, FMT>(a, pt, qt, kbeg, kend, atomic, lds); break;
+    case 8: mfma_tile_pp<128, 128, 2, FMT>(a, pt, qt, kbeg, kend, atomic, lds); break;
+    default: mfma_tile_pp<64, 128, 2, FMT>(a, pt, qt, kbeg, kend, atomic, lds); break;
   }
 }
 
@@ -367,7 +410,7 @@ __device__ __forceinline__ void mfma_tile_by_shape(int shape, const MfmaOperands
 // fastest), and consecutive tiles land on one XCD, whose L2 then holds that
 // operand's rows.
 struct MfmaPlan {
-  int shape;
+  int shape, fmt;
   uint32_t splits, kchunk, ptiles, qtiles, order;
   uint64_t tiles() const { return (uint64_t)ptiles * qtiles * splits; }
 };
@@ -383,9 +426,13 @@ __host__ __device__ inline void plan_decode(const MfmaPlan &pl, uint64_t t, uint
   pt = pl.order ? lo : hi;
 }
 
-// Plan for a product (uses the tuned table; see mfma_config_for).
-MfmaPlan mfma_plan(uint32_t M, uint32_t N, uint32_t kbytes, int forced_shape = -1,
+// Plan for a product whose copies hold kbytes of K in format fmt (uses the
+// tuned table; see mfma_config_for).
+MfmaPlan mfma_plan(uint32_t M, uint32_t N, uint32_t kbytes, int fmt, int forced_shape = -1,
                    int forced_splits = 0);
+
+// Operand views of a product's value copies in format fmt.
+MfmaOperands mfma_operands(const gbs_matrix *l, const gbs_matrix *r, int fmt, int32_t *d_res);
 
 // Shape class of a product, for the tile autotuner's table: buckets of
 // M, N and K.  kClassM x kClassN x kClassK classes.
@@ -398,9 +445,10 @@ inline int shape_class(uint64_t M, uint64_t N, uint64_t K) {
   return (cm * kClassN + cn) * kClassK + ck;
 }
 
-// Tile shape and K split chosen for a class (table set by gbs_tune_load;
-// else a size heuristic).  splits 0 = decide from the tile count.
-void mfma_config_for(uint64_t M, uint64_t N, uint64_t K, int *shape, int *splits);
+// Tile shape and K split chosen for a product of depth K (values) in
+// format fmt (table set by gbs_tune_load; else a size heuristic).
+// splits 0 = decide from the tile count.
+void mfma_config_for(uint64_t M, uint64_t N, uint64_t K, int fmt, int *shape, int *splits);
 
 // Persistent launch over products given as operand views (gbs_batch.hip).
 int launch_views(const MfmaOperands *views, int n, void *host_desc, void *d_desc,

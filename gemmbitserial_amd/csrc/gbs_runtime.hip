@@ -121,17 +121,19 @@ int gemm_enqueue(gbs_workspace *ws, const gbs_matrix *l, const gbs_matrix *r,
   const uint64_t K = l->ncols;
   const uint64_t kwords = (K + 63) / 64;
   gbs_algo eng = algo == GBS_ALGO_AUTO ? select_algo(lo, ro, K) : algo;
-  if (eng == GBS_ALGO_MFMA) {
-    if (!mfma_supported(lo, ro, K))
+  if (eng == GBS_ALGO_MFMA || eng == GBS_ALGO_MFMA_FP4) {
+    const bool f4 = eng == GBS_ALGO_MFMA_FP4;
+    if (f4 ? !f4_supported(lo, ro, K) : !mfma_supported(lo, ro, K))
       return fail(GBS_ENOTSUP,
-                  "gemm: the int8 MFMA engine cannot hold this product exactly "
-                  "(bitwidths %llu x %llu, depth %llu); use the popcount engine",
-                  (unsigned long long)l->nbits, (unsigned long long)r->nbits,
-                  (unsigned long long)K);
-    int rc = ensure_vals(const_cast<gbs_matrix *>(l), s, may_allocate);
-    if (!rc) rc = ensure_vals(const_cast<gbs_matrix *>(r), s, may_allocate);
+                  "gemm: the %s matrix-core engine cannot hold this product exactly "
+                  "(bitwidths %llu x %llu, depth %llu)",
+                  f4 ? "FP4" : "int8", (unsigned long long)l->nbits,
+                  (unsigned long long)r->nbits, (unsigned long long)K);
+    const int fmt = f4 ? kFmtF4 : kFmtI8;
+    int rc = ensure_copy(const_cast<gbs_matrix *>(l), fmt, s, may_allocate);
+    if (!rc) rc = ensure_copy(const_cast<gbs_matrix *>(r), fmt, s, may_allocate);
     if (rc) return rc;
-    return launch_mfma_gemm(l, r, d_res, s);
+    return launch_mfma_gemm(l, r, d_res, s, fmt);
   }
   if (eng != GBS_ALGO_POPCOUNT) return fail(GBS_EINVAL, "gemm: unknown algo %d", (int)algo);
   const int32_t *d_rowsum = nullptr;
@@ -242,6 +244,7 @@ int gbs_matrix_destroy(gbs_matrix *m) {
   GBS_HIP(hipStreamSynchronize(thread_stream()));
   GBS_HIP(hipFree(m->d_planes));
   if (m->d_vals) GBS_HIP(hipFree(m->d_vals));
+  if (m->d_f4) GBS_HIP(hipFree(m->d_f4));
   delete m;
   return GBS_OK;
 }
@@ -427,10 +430,16 @@ int gbs_workspace_reserve(gbs_workspace *ws, const gbs_matrix *lhs, const gbs_ma
   if ((rc = ensure_device())) return rc;
   const size_t rows = lhs->nrows > rhs->nrows ? lhs->nrows : rhs->nrows;
   if ((rc = grow((void **)&ws->d_rowsum, &ws->rowsum_cap, rows * 4))) return rc;
-  if (lhs->ncols == rhs->ncols && mfma_supported(view(lhs), view(rhs), lhs->ncols)) {
+  if (lhs->ncols == rhs->ncols) {
     hipStream_t s = thread_stream();
-    if ((rc = ensure_vals(const_cast<gbs_matrix *>(lhs), s, true))) return rc;
-    if ((rc = ensure_vals(const_cast<gbs_matrix *>(rhs), s, true))) return rc;
+    if (mfma_supported(view(lhs), view(rhs), lhs->ncols)) {
+      if ((rc = ensure_vals(const_cast<gbs_matrix *>(lhs), s, true))) return rc;
+      if ((rc = ensure_vals(const_cast<gbs_matrix *>(rhs), s, true))) return rc;
+    }
+    if (f4_supported(view(lhs), view(rhs), lhs->ncols)) {
+      if ((rc = ensure_f4(const_cast<gbs_matrix *>(lhs), s, true))) return rc;
+      if ((rc = ensure_f4(const_cast<gbs_matrix *>(rhs), s, true))) return rc;
+    }
     GBS_HIP(hipStreamSynchronize(s));
   }
   return GBS_OK;
@@ -524,22 +533,26 @@ int gbs_event_elapsed_ms(void *start, void *stop, float *ms) {
 }
 
 int gbs_gemm_async_config(gbs_workspace *ws, const gbs_matrix *lhs, const gbs_matrix *rhs,
-                          int32_t *d_res, int shape, int splits, void *stream) {
+                          int32_t *d_res, gbs_algo algo, int shape, int splits, void *stream) {
   int rc = check_matrix(lhs, "gbs_gemm_async_config");
   if (!rc) rc = check_matrix(rhs, "gbs_gemm_async_config");
   if (rc) return rc;
   (void)ws;
   if (lhs->ncols != rhs->ncols) return fail(GBS_EINVAL, "gbs_gemm_async_config: depths differ");
-  if (!mfma_supported(view(lhs), view(rhs), lhs->ncols))
-    return fail(GBS_ENOTSUP, "gbs_gemm_async_config: not exact on the MFMA engine");
+  if (algo != GBS_ALGO_MFMA && algo != GBS_ALGO_MFMA_FP4)
+    return fail(GBS_EINVAL, "gbs_gemm_async_config: algo must be GBS_ALGO_MFMA or _MFMA_FP4");
+  const int fmt = algo == GBS_ALGO_MFMA_FP4 ? kFmtF4 : kFmtI8;
+  if (fmt == kFmtF4 ? !f4_supported(view(lhs), view(rhs), lhs->ncols)
+                    : !mfma_supported(view(lhs), view(rhs), lhs->ncols))
+    return fail(GBS_ENOTSUP, "gbs_gemm_async_config: not exact in this operand format");
   if (shape < 0 || shape >= kMfmaShapes || splits < 1)
     return fail(GBS_EINVAL, "gbs_gemm_async_config: shape %d / splits %d out of range", shape,
                 splits);
   if ((rc = ensure_device())) return rc;
   hipStream_t s = stream ? (hipStream_t)stream : thread_stream();
-  if ((rc = ensure_vals(const_cast<gbs_matrix *>(lhs), s, false))) return rc;
-  if ((rc = ensure_vals(const_cast<gbs_matrix *>(rhs), s, false))) return rc;
-  return launch_mfma_gemm(lhs, rhs, d_res, s, shape, true, splits > 0 ? splits : 1);
+  if ((rc = ensure_copy(const_cast<gbs_matrix *>(lhs), fmt, s, false))) return rc;
+  if ((rc = ensure_copy(const_cast<gbs_matrix *>(rhs), fmt, s, false))) return rc;
+  return launch_mfma_gemm(lhs, rhs, d_res, s, fmt, shape, true, splits > 0 ? splits : 1);
 }
 
 int gbs_tune_classes(int *n_classes, int *n_shapes) {
@@ -562,26 +575,27 @@ int gbs_tune_load(const char *path) {
   if (!path) return fail(GBS_EINVAL, "gbs_tune_load: null path");
   FILE *f = fopen(path, "r");
   if (!f) return fail(GBS_EINVAL, "gbs_tune_load: cannot open %s", path);
-  int shapes[kClasses], splits[kClasses];
-  for (int i = 0; i < kClasses; i++) {
+  int shapes[kMfmaFormats * kClasses], splits[kMfmaFormats * kClasses];
+  for (int i = 0; i < kMfmaFormats * kClasses; i++) {
     shapes[i] = -1;
     splits[i] = 0;
   }
   char line[512];
   int n = 0;
   while (fgets(line, sizeof(line), f)) {
-    int c, sh, sp = 0;
+    int c, sh, sp = 0, fmt = 0;
     if (line[0] == '#') continue;
-    const int got = sscanf(line, "%d %d %d", &c, &sh, &sp);
-    if (got >= 2 && c >= 0 && c < kClasses && sh >= 0 && sh < kMfmaShapes) {
-      shapes[c] = sh;
-      splits[c] = (got == 3 && sp > 0) ? sp : 0;
+    const int got = sscanf(line, "%d %d %d %d", &c, &sh, &sp, &fmt);
+    if (got >= 2 && c >= 0 && c < kClasses && sh >= 0 && sh < kMfmaShapes && fmt >= 0 &&
+        fmt < kMfmaFormats) {
+      shapes[fmt * kClasses + c] = sh;
+      splits[fmt * kClasses + c] = (got >= 3 && sp > 0) ? sp : 0;
       n++;
     }
   }
   fclose(f);
   if (n == 0) return fail(GBS_EINVAL, "gbs_tune_load: no entries in %s", path);
-  return set_mfma_table(shapes, splits, kClasses);
+  return set_mfma_table(shapes, splits, kMfmaFormats * kClasses);
 }
 
 }  // extern "C"

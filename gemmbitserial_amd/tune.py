@@ -1,12 +1,14 @@
-"""Tile autotuner of the MFMA engine.
+"""Tile autotuner of the matrix-core engine.
 
 Products fall into shape classes (buckets of M, N and K; gbs_tune_class_of).
-For every class this times each compiled tile shape with each K split
-(1, 2, 4, ... while a split keeps at least 256 bytes of K) on a
-representative product (int8 copies current, result in HBM) and keeps the
-fastest pair; the table is written as text, one "class shape splits" line
-per class, and loaded by the library at gbs_init when it sits next to
-libgbs_hip.so (gemmbitserial_amd/tune_table.txt) or through gbs_tune_load.
+For every class and operand format (int8 copies, FP4 copies) this times
+each compiled tile shape with each K split (1, 2, 4, ... while a split
+keeps at least 256 bytes of K) on a representative product (2-bit signed x
+2-bit unsigned, copies current, result in HBM) and keeps the fastest pair;
+the table is written as text, one "class shape splits format" line per
+class and format, and loaded by the library at gbs_init when it sits next
+to libgbs_hip.so (gemmbitserial_amd/tune_table.txt) or through
+gbs_tune_load.
 
     python -m gemmbitserial_amd.tune [--out PATH] [--reps 20]
 """
@@ -35,15 +37,18 @@ SHAPES = ["128x128", "64x128", "32x256/k64", "16x256", "64x64", "128x64", "128x1
           "256x256"]
 
 
-def time_config(l, r, d, shape, splits, reps, stream):
+FORMATS = [("int8", gbs.MFMA), ("fp4", gbs.MFMA_FP4)]
+
+
+def time_config(l, r, d, algo, shape, splits, reps, stream):
     call = gbs.lib().gbs_gemm_async_config
     for _ in range(3):
-        gbs.check(call(None, l.h, r.h, d.ptr, shape, splits, stream))
+        gbs.check(call(None, l.h, r.h, d.ptr, algo, shape, splits, stream))
     gbs.sync(stream)
     e0, e1 = gbs.Event(), gbs.Event()
     e0.record(stream)
     for _ in range(reps):
-        gbs.check(call(None, l.h, r.h, d.ptr, shape, splits, stream))
+        gbs.check(call(None, l.h, r.h, d.ptr, algo, shape, splits, stream))
     e1.record(stream)
     gbs.sync(stream)
     return e0.elapsed_ms(e1) / reps
@@ -54,7 +59,8 @@ def tune(reps=10, log=print):
     stream = gbs.stream()
     n_classes, n_shapes = gbs.tune_classes()
     rng = np.random.default_rng(W.SEED0 + 99)
-    shapes, splits = [-1] * n_classes, [0] * n_classes
+    nf = len(FORMATS)
+    shapes, splits = [-1] * (nf * n_classes), [0] * (nf * n_classes)
     detail = {}
     for M, N, K in itertools.product(REP_M, REP_N, REP_K):
         if M * N > 2 ** 26:  # 256 MB of result: left to the built-in heuristic
@@ -65,25 +71,28 @@ def tune(reps=10, log=print):
         ws = gbs.Workspace()
         ws.reserve(l, r)
         d = gbs.DeviceBuffer(4 * M * N)
-        times = {}
-        for sh in range(n_shapes):
-            sp = 1
-            while sp == 1 or K // sp >= 256:
-                times[(sh, sp)] = time_config(l, r, d, sh, sp, reps, stream)
-                sp *= 2
-        (bsh, bsp), bt = min(times.items(), key=lambda kv: kv[1])
-        # the winner must agree with the unsplit 64 x 64 product
-        time_config(l, r, d, 4, 1, 1, stream)
+        # every winner must agree with the unsplit int8 64 x 64 product
+        time_config(l, r, d, gbs.MFMA, 4, 1, 1, stream)
         want = d.to_host(np.int32, (N, M))
-        time_config(l, r, d, bsh, bsp, 1, stream)
-        if not np.array_equal(d.to_host(np.int32, (N, M)), want):
-            raise SystemExit(f"config {SHAPES[bsh]}/{bsp} disagrees at {M}x{K}x{N}")
-        shapes[cls], splits[cls] = bsh, bsp
-        unsplit = [round(times[(x, 1)] * 1e3, 2) for x in range(n_shapes)]
-        detail[cls] = {"M": M, "N": N, "K": K, "best": f"{SHAPES[bsh]} split {bsp}",
-                       "us_best": round(bt * 1e3, 2), "us_unsplit_by_shape": unsplit}
-        log(f"class {cls:2d} M={M:5d} N={N:4d} K={K:5d}: {SHAPES[bsh]:12s} x{bsp:<3d} "
-            f"{bt * 1e3:7.1f} us  (unsplit: " + " ".join(f"{t:.1f}" for t in unsplit) + ")")
+        for f, (fname, algo) in enumerate(FORMATS):
+            kbytes = K if algo == gbs.MFMA else K // 2
+            times = {}
+            for sh in range(n_shapes):
+                sp = 1
+                while sp == 1 or kbytes // sp >= 256:
+                    times[(sh, sp)] = time_config(l, r, d, algo, sh, sp, reps, stream)
+                    sp *= 2
+            (bsh, bsp), bt = min(times.items(), key=lambda kv: kv[1])
+            time_config(l, r, d, algo, bsh, bsp, 1, stream)
+            if not np.array_equal(d.to_host(np.int32, (N, M)), want):
+                raise SystemExit(f"{fname} config {SHAPES[bsh]}/{bsp} disagrees at {M}x{K}x{N}")
+            shapes[f * n_classes + cls], splits[f * n_classes + cls] = bsh, bsp
+            unsplit = [round(times[(x, 1)] * 1e3, 2) for x in range(n_shapes)]
+            detail[(f, cls)] = {"M": M, "N": N, "K": K, "best": f"{SHAPES[bsh]} split {bsp}",
+                                "us_best": round(bt * 1e3, 2), "us_unsplit_by_shape": unsplit}
+            log(f"{fname:4s} class {cls:2d} M={M:5d} N={N:5d} K={K:5d}: {SHAPES[bsh]:12s} "
+                f"x{bsp:<3d} {bt * 1e3:7.1f} us  (unsplit: " +
+                " ".join(f"{t:.1f}" for t in unsplit) + ")")
         for x in (l, r, d):
             x.close()
         ws.close()
@@ -91,15 +100,18 @@ def tune(reps=10, log=print):
 
 
 def write_table(shapes, splits, detail, path):
+    n_classes = len(shapes) // len(FORMATS)
     with open(path, "w") as f:
-        f.write("# MFMA tile shape and K split per shape class, measured by "
-                "gemmbitserial_amd.tune\n")
+        f.write("# matrix-core tile shape and K split per shape class and operand format, "
+                "measured by gemmbitserial_amd.tune\n")
         f.write("# shapes: " + " ".join(f"{i}={s}" for i, s in enumerate(SHAPES)) + "\n")
-        f.write("# class shape splits  # representative M N K, best time, unsplit times\n")
-        for cls, sh in enumerate(shapes):
+        f.write("# formats: " + " ".join(f"{i}={n}" for i, (n, _) in enumerate(FORMATS)) + "\n")
+        f.write("# class shape splits format  # representative M N K, best time, unsplit times\n")
+        for i, sh in enumerate(shapes):
             if sh >= 0:
-                d = detail.get(cls, {})
-                f.write(f"{cls} {sh} {splits[cls]}  # M={d.get('M')} N={d.get('N')} "
+                fmt, cls = divmod(i, n_classes)
+                d = detail.get((fmt, cls), {})
+                f.write(f"{cls} {sh} {splits[i]} {fmt}  # M={d.get('M')} N={d.get('N')} "
                         f"K={d.get('K')} best={d.get('us_best')}us "
                         f"unsplit={d.get('us_unsplit_by_shape')}\n")
 

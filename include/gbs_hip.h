@@ -56,12 +56,14 @@ typedef enum gbs_dtype {
   GBS_I32 = 5, GBS_U32 = 6, GBS_I64 = 7, GBS_U64 = 8
 } gbs_dtype;
 
-/* GEMM engine.  AUTO picks per shape from the measured crossover; both
- * engines return identical int32 results. */
+/* GEMM engine.  AUTO picks per shape from the measured crossover; every
+ * engine returns identical int32 results. */
 typedef enum gbs_algo {
   GBS_ALGO_AUTO = 0,
   GBS_ALGO_POPCOUNT = 1, /* fused AND/XOR + popcount over bit-planes */
-  GBS_ALGO_MFMA = 2      /* planes expanded to int8, v_mfma_i32_*_i8 */
+  GBS_ALGO_MFMA = 2,     /* planes expanded to int8, v_mfma_i32_*_i8 */
+  GBS_ALGO_MFMA_FP4 = 3  /* planes expanded to FP4 codes (operands of <= 2
+                            bits, <= 3 signed), v_mfma_scale_f32_*_f8f6f4 */
 } gbs_algo;
 
 typedef struct gbs_matrix gbs_matrix;       /* device-resident bit-serial matrix */
@@ -225,17 +227,20 @@ int gbs_batch_info(const gbs_batch *b, uint64_t *tiles, uint64_t *persistent,
  * gemmbitserial.hpp:343-377). */
 int gbs_tune_classes(int *n_classes, int *n_shapes);
 int gbs_tune_class_of(uint64_t M, uint64_t N, uint64_t K, int *cls);
-/* shapes[cls], splits[cls] for every class (shapes NULL: built-in
- * heuristic; splits NULL or 0: K split chosen from the tile count). */
+/* shapes[cls], splits[cls] for every class, n = n_classes (int8 copies)
+ * or 2 * n_classes (int8 classes, then FP4 classes).  shapes NULL: built-in
+ * heuristic; a shape of -1: heuristic for that class; splits NULL or 0: K
+ * split chosen from the tile count. */
 int gbs_tune_set(const int *shapes, const int *splits, int n);
-/* Text table, one "class shape [splits]" line per class, '#' comments. */
+/* Text table, one "class shape [splits [format]]" line per class and
+ * format (0 int8, 1 FP4), '#' comments. */
 int gbs_tune_load(const char *path);
-/* One MFMA product with a forced tile shape and K split (the tuner's
- * probe).  Operands must already have current int8 copies
- * (gbs_workspace_reserve). */
+/* One matrix-core product (algo GBS_ALGO_MFMA or GBS_ALGO_MFMA_FP4) with a
+ * forced tile shape and K split (the tuner's probe).  Operands must
+ * already have current copies in that format (gbs_workspace_reserve). */
 int gbs_gemm_async_config(gbs_workspace *ws, const gbs_matrix *lhs,
-                          const gbs_matrix *rhs, int32_t *d_res, int shape,
-                          int splits, void *stream);
+                          const gbs_matrix *rhs, int32_t *d_res, gbs_algo algo,
+                          int shape, int splits, void *stream);
 
 /* Serving -------------------------------------------------------------- */
 /* A continuous-batching request loop over a stack of layers whose weights

@@ -248,7 +248,7 @@ def test_mfma_sees_reimported_and_host_written_planes(gbs, orc):
 def test_mfma_refuses_inexact(gbs):
     l = gbs.DeviceMatrix(8, 4, 64, False)   # unsigned 8-bit codes do not fit int8
     r = gbs.DeviceMatrix(2, 4, 64, False)
-    with pytest.raises(gbs.GbsError, match="MFMA"):
+    with pytest.raises(gbs.GbsError, match="int8 matrix-core engine cannot hold"):
         gbs.gemm(l, r, "mfma")
     assert gbs.select(l, r) == "popcount"
 
@@ -284,7 +284,104 @@ def test_mfma_every_tile_shape_and_split(gbs, orc):
             for splits in (1, 3):
                 say("shape", shape, "splits", splits, M, K, N)
                 d.fill_zero()
-                gbs.check(gbs.lib().gbs_gemm_async_config(ws.h, l.h, r.h, d.ptr, shape, splits,
+                gbs.check(gbs.lib().gbs_gemm_async_config(ws.h, l.h, r.h, d.ptr, gbs.MFMA, shape,
+                                                          splits,
                                                           None))
+                gbs.sync()
+                assert np.array_equal(d.to_host(np.int32, (N, M)), want), (shape, splits)
+
+
+# ------------------------------------------------------------ FP4 MFMA engine
+# operands an FP4 code holds exactly: bipolar, unsigned <= 2 bits, signed <= 3 bits
+F4_ENC = [(1, True), (1, False), (2, False), (2, True), (3, True)]
+
+
+@pytest.mark.parametrize("lenc", F4_ENC)
+@pytest.mark.parametrize("renc", F4_ENC)
+def test_fp4_all_encodings(gbs, orc, lenc, renc):
+    (lb, ls), (rb, rs) = lenc, renc
+    rng = np.random.default_rng(9000 + lb * 100 + rb * 10 + 2 * ls + rs)
+    for (M, K, N) in [(3, 6, 9), (17, 34, 1), (130, 700, 67), (64, 1030, 200),
+                      (257, 3000, 129)]:
+        say("fp4", lb, rb, ls, rs, M, K, N)
+        lhs = rand_values(rng, (M, K), lb, ls)
+        rhs = rand_values(rng, (N, K), rb, rs)
+        got = run_gemm(gbs, lhs, rhs, lb, rb, ls, rs, algo="mfma_fp4")
+        assert np.array_equal(got, expected(lhs, rhs, lb, rb, ls, rs))
+
+
+def test_fp4_refuses_what_it_cannot_hold(gbs):
+    for (lb, ls) in [(3, False), (4, True)]:
+        l = gbs.DeviceMatrix(lb, 4, 64, ls)
+        r = gbs.DeviceMatrix(1, 4, 64, True)
+        with pytest.raises(gbs.GbsError, match="FP4 matrix-core engine cannot hold"):
+            gbs.gemm(l, r, "mfma_fp4")
+    # K * 4 * 4 reaches 2^24: fp32 sums could round
+    K = 1 << 20
+    l = gbs.DeviceMatrix(3, 2, K, True)
+    r = gbs.DeviceMatrix(3, 2, K, True)
+    with pytest.raises(gbs.GbsError, match="FP4"):
+        gbs.gemm(l, r, "mfma_fp4")
+    assert gbs.select(l, r) != "mfma_fp4"
+
+
+def test_fp4_exact_just_below_the_bound(gbs, orc):
+    """Sums of magnitude close to 2^24 with odd low bits: any fp32 rounding
+    inside the instruction or across the K loop would show."""
+    rng = np.random.default_rng(2024)
+    K = (1 << 20) - 64                      # K * 4 * 4 < 2^24
+    M, N = 48, 20
+    lhs = rng.choice(np.array([-4, 3, -3, 1], np.int8), size=(M, K), p=[0.85, 0.05, 0.05, 0.05])
+    rhs = rng.choice(np.array([-4, 3, -1], np.int8), size=(N, K), p=[0.9, 0.05, 0.05])
+    l, r = gbs.from_values(lhs, 3, True), gbs.from_values(rhs, 3, True)
+    assert gbs.select(l, r) == "mfma_fp4" or gbs.select(l, r) == "mfma"
+    got = gbs.gemm(l, r, "mfma_fp4")
+    want = O.wrap_i32(O.blas_gemm_exact(lhs.astype(np.int64), rhs.astype(np.int64)))
+    assert np.abs(want).max() > (1 << 23)
+    assert np.array_equal(got, want)
+    # bipolar, deep: every sum is K - 2 * (disagreements)
+    K2 = 300001
+    a = rng.integers(0, 2, size=(33, K2)).astype(np.int8) * 2 - 1
+    b = np.where(rng.random((7, K2)) < 0.97, 1, -1).astype(np.int8)
+    la, lb_ = gbs.from_values(a, 1, True), gbs.from_values(b, 1, True)
+    want2 = O.wrap_i32(O.blas_gemm_exact(a.astype(np.int64), b.astype(np.int64)))
+    assert np.array_equal(gbs.gemm(la, lb_, "mfma_fp4"), want2)
+
+
+def test_fp4_sees_reimported_and_host_written_planes(gbs, orc):
+    rng = np.random.default_rng(15)
+    a = rand_values(rng, (70, 300), 2, True)
+    b = rand_values(rng, (50, 300), 2, False)
+    l, r = gbs.from_values(a, 2, True), gbs.from_values(b, 2, False)
+    assert np.array_equal(gbs.gemm(l, r, "mfma_fp4"), expected(a, b, 2, 2, True, False))
+    b2 = rand_values(rng, (50, 300), 2, False)
+    r.import_regular(b2)
+    assert np.array_equal(gbs.gemm(l, r, "mfma_fp4"), expected(a, b2, 2, 2, True, False))
+    a2 = rand_values(rng, (70, 300), 2, True)
+    l.set_planes(O.import_regular(a2, 2, True).planes)
+    assert np.array_equal(gbs.gemm(l, r, "mfma_fp4"), expected(a2, b2, 2, 2, True, False))
+    # the int8 copy of the same matrices stays coherent too
+    assert np.array_equal(gbs.gemm(l, r, "mfma"), expected(a2, b2, 2, 2, True, False))
+
+
+def test_fp4_every_tile_shape_and_split(gbs, orc):
+    n_classes, n_shapes = gbs.tune_classes()
+    rng = np.random.default_rng(199)
+    ws = gbs.Workspace()
+    for (M, K, N, lb, rb, ls, rs) in [(300, 1100, 70, 3, 2, True, False),
+                                      (129, 2048, 257, 1, 1, True, True),
+                                      (64, 320, 16, 1, 2, False, True)]:
+        lhs = rand_values(rng, (M, K), lb, ls)
+        rhs = rand_values(rng, (N, K), rb, rs)
+        l, r = gbs.from_values(lhs, lb, ls), gbs.from_values(rhs, rb, rs)
+        ws.reserve(l, r)
+        want = expected(lhs, rhs, lb, rb, ls, rs)
+        d = gbs.DeviceBuffer(4 * M * N)
+        for shape in range(n_shapes):
+            for splits in (1, 3):
+                say("fp4 shape", shape, "splits", splits, M, K, N)
+                d.fill_zero()
+                gbs.check(gbs.lib().gbs_gemm_async_config(ws.h, l.h, r.h, d.ptr, gbs.MFMA_FP4,
+                                                          shape, splits, None))
                 gbs.sync()
                 assert np.array_equal(d.to_host(np.int32, (N, M)), want), (shape, splits)
