@@ -42,7 +42,7 @@ SEED0 = 0x67627300
 VALU_PEAK = 256 * 64 * 2.4e9      # int32 lane-ops/s: 256 CUs x 64 lanes x 2.4 GHz
 HBM_PEAK = 8.0e12                 # B/s
 MFMA_I8_PEAK = 5.0e15             # dense int8 ops/s
-MFMA_F4_PEAK = 10.1e15            # dense FP4 ops/s
+MFMA_F4_PEAK = 10.1e15            # dense FP4 / FP6 ops/s
 
 
 def log(*a):
@@ -104,10 +104,10 @@ def int8_bytes(M, K, N):
 
 
 def roofline_of(algo, M, K, N, lb, rb, sec):
-    if algo in ("mfma", "mfma_fp4"):
+    if algo in ("mfma", "mfma_fp4", "mfma_fp6"):
         ach = 2.0 * M * K * N / sec
         peak, unit = ((MFMA_I8_PEAK, "TOP/s (int8)") if algo == "mfma"
-                      else (MFMA_F4_PEAK, "TOP/s (fp4 operands)"))
+                      else (MFMA_F4_PEAK, f"TOP/s ({algo[-3:]} operands)"))
         return {"bound": "mfma", "achieved": round(ach / 1e12, 2),
                 "peak": peak / 1e12, "unit": unit, "frac": round(ach / peak, 4)}
     ach = valu_lane_ops(M, K, N, lb, rb) / sec
@@ -147,7 +147,8 @@ def run_config(gbs, O, name, runs, engines, steps, warmup, stream, ws, cpu_secs)
         want = None
         row = {"encoding": enc, "M": M, "K": K, "N": N, "bits": [lb, rb],
                "signed": [ls, rs], "auto": auto, "engines": {}}
-        for eng in (["popcount", "mfma", "mfma_fp4"] if engines == "all" else [auto]):
+        for eng in (["popcount", "mfma", "mfma_fp4", "mfma_fp6"] if engines == "all"
+                    else [auto]):
             log(f"[bench] {name} {enc}: {eng} x {steps}")
             try:
                 ms = time_gemm(gbs, ws, l, r, d_res, eng, steps, warmup, stream)

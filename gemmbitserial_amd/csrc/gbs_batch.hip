@@ -2,23 +2,30 @@
 // bitwidths and signedness) in one persistent kernel launch.
 //
 // Plan (gbs_batch_create): every product that the matrix-core engine holds
-// exactly (FP4 copies where both operands fit them, else int8) becomes a
-// run of work items -- output tiles of the shape its
-// shape class maps to in the tuning table, times K splits (deep products
-// are cut into pieces of at most 2 KiB of K, whose partial sums are added
-// with integer atomics into results zeroed first).  Products are ordered by
+// exactly becomes a run of work items in the narrowest copy format both
+// operands fit (FP4, else FP6, else int8: inside one launch an item's time
+// follows the bytes of K it streams) -- output tiles of a shape picked by
+// the rhs row count (16 x 256 for N <= 16, 32 x 256 for N <= 32, else
+// 64 x 64: the least padding, and no grid to fill), times K splits (deep
+// products are cut into pieces of at most 16 K units, whose partial sums
+// are added with integer atomics into results zeroed first).  Products are ordered by
 // the cost of one item, largest first, so the long items start early and
 // the short ones fill the tail.  A device array holds one descriptor per product and
 // the prefix sum of tile counts.  Products the MFMA engine cannot hold
 // exactly (unsigned 8-bit codes, very deep K) go to the popcount engine as
 // separate launches after the persistent kernel.
 //
-// Run: a small kernel zeroes the queue counter, then a grid of
-// 2 blocks x 256 CUs loops: thread 0 takes the next tile index with one
-// atomicAdd, the block finds its product by binary search over the prefix
-// sums and computes that tile with the shape's code path.  Every block
-// leaves when the index passes the last tile.  The whole sequence can be
-// captured once into a hipGraph and replayed.
+// Products are grouped by copy format (int8, FP4, FP6); each group has its
+// own descriptor range, tile prefix sums and queue counter, and its own
+// persistent kernel (one kernel carrying every format's tile code would
+// spill registers).
+//
+// Run: a small kernel zeroes the queue counters, then per format group a
+// grid of 2 blocks x 256 CUs loops: thread 0 takes the next tile index with
+// one atomicAdd, the block finds its product by binary search over the
+// prefix sums and computes that tile with the shape's code path.  Every
+// block leaves when the index passes the group's last tile.  The whole
+// sequence can be captured once into a hipGraph and replayed.
 #include <algorithm>
 #include <vector>
 
@@ -29,12 +36,14 @@ struct gbs_batch {
   std::vector<gbs_problem> probs;          // as given
   std::vector<uint32_t> mfma_idx;          // persistent products, in run order
   std::vector<uint32_t> separate_idx;      // popcount products
-  void *d_desc = nullptr;                  // BatchProb[]
+  void *d_desc = nullptr;                  // BatchProb[], grouped by format
   void *d_zero = nullptr;                  // ZeroSpan[] of split products
   uint32_t n_zero = 0;
-  unsigned int *d_counter = nullptr;
+  unsigned int *d_counter = nullptr;       // one queue counter per format
   uint64_t total_tiles = 0;
-  uint32_t grid = 0;
+  uint32_t first[3] = {0, 0, 0}, count[3] = {0, 0, 0};  // descriptor range per format
+  uint64_t tiles[3] = {0, 0, 0};
+  uint32_t grid[3] = {0, 0, 0};
   gbs_workspace *ws = nullptr;             // for separate products
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
@@ -55,7 +64,9 @@ struct ZeroSpan {
   uint64_t n;
 };
 
-__global__ void reset_counter_kernel(unsigned int *c) { *c = 0; }
+__global__ void reset_counter_kernel(unsigned int *c) {
+  if (threadIdx.x < 3) c[threadIdx.x] = 0;
+}
 
 // Zero the results of split products (their partial sums are added
 // atomically); blockIdx.y picks the span.
@@ -66,6 +77,7 @@ __global__ __launch_bounds__(256) void zero_spans_kernel(const ZeroSpan *spans) 
     z.p[i] = 0;
 }
 
+template <int FMT>
 __global__ __launch_bounds__(256, 2) void batch_kernel(const BatchProb *probs, uint32_t nprobs,
                                                        uint64_t total, unsigned int *counter) {
   __shared__ __attribute__((aligned(16))) int8_t lds[kMfmaLdsMax];
@@ -89,28 +101,23 @@ __global__ __launch_bounds__(256, 2) void batch_kernel(const BatchProb *probs, u
     plan_decode(pl, item - probs[lo].tile_begin, pt, qt, sp);
     const uint32_t kbeg = sp * pl.kchunk;
     const uint32_t kend = min(ops.kbytes, kbeg + pl.kchunk);
-    if (pl.fmt == kFmtF4)
-      mfma_tile_by_shape<kFmtF4>(pl.shape, ops, pt, qt, kbeg, kend, pl.splits > 1, lds);
-    else
-      mfma_tile_by_shape<kFmtI8>(pl.shape, ops, pt, qt, kbeg, kend, pl.splits > 1, lds);
+    mfma_tile_by_shape<FMT>(pl.shape, ops, pt, qt, kbeg, kend, pl.splits > 1, lds);
     __syncthreads();
   }
 }
 
 }  // namespace
 
-// Plan of one product inside a persistent launch: the tuned plan, with the
-// widest tile (not carried by the persistent kernel) replaced by 128 x 128,
-// and deep products cut into pieces of at most 16 K units so that no item
-// is much longer than the rest.
+// Plan of one product inside a persistent launch: the tile shape with the
+// least padding for its rhs row count (the tuning table's shapes are
+// chosen to fill the chip with one product, which a shared queue does not
+// need), and deep products cut into pieces of at most 16 K units so that no
+// item is much longer than the rest.
 static MfmaPlan batch_plan(uint32_t M, uint32_t N, uint32_t kbytes, int fmt) {
-  MfmaPlan pl = mfma_plan(M, N, kbytes, fmt);
-  if (pl.shape != batch_shape(pl.shape, fmt))
-    pl = mfma_plan(M, N, kbytes, fmt, batch_shape(pl.shape, fmt), (int)pl.splits);
+  const int shape = batch_shape(N <= 16 ? 3 : N <= 32 ? 2 : 4, fmt);
   const uint32_t units = kbytes / kMfmaKS;
   const uint32_t want = (units + 15) / 16;
-  if (want > pl.splits) pl = mfma_plan(M, N, kbytes, fmt, pl.shape, (int)want);
-  return pl;
+  return mfma_plan(M, N, kbytes, fmt, shape, (int)(want ? want : 1));
 }
 
 // Plan and launch products given as int8 operand views (the serving loop's
@@ -139,10 +146,10 @@ int launch_views(const MfmaOperands *views, int n, void *host_desc, void *d_desc
     hipLaunchKernelGGL(zero_spans_kernel, dim3(64, nz), dim3(256), 0, s,
                        (const ZeroSpan *)((BatchProb *)d_desc + (n > 0 ? n : 1)));
   }
-  hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, d_counter);
+  hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, s, d_counter);
   const uint32_t grid = (uint32_t)(tiles < 512 ? tiles : 512);
-  hipLaunchKernelGGL(batch_kernel, dim3(grid), dim3(256), 0, s, (const BatchProb *)d_desc,
-                     (uint32_t)m, tiles, d_counter);
+  hipLaunchKernelGGL((batch_kernel<kFmtI8>), dim3(grid), dim3(256), 0, s,
+                     (const BatchProb *)d_desc, (uint32_t)m, tiles, d_counter);
   GBS_HIP(hipGetLastError());
   return GBS_OK;
 }
@@ -156,11 +163,21 @@ static int batch_launch(gbs_batch *b, hipStream_t s) {
     if (b->n_zero)
       hipLaunchKernelGGL(zero_spans_kernel, dim3(64, b->n_zero), dim3(256), 0, s,
                          (const ZeroSpan *)b->d_zero);
-    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(1), 0, s, b->d_counter);
+    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, s, b->d_counter);
     GBS_HIP(hipGetLastError());
-    hipLaunchKernelGGL(batch_kernel, dim3(b->grid), dim3(256), 0, s,
-                       (const BatchProb *)b->d_desc, (uint32_t)b->mfma_idx.size(),
-                       b->total_tiles, b->d_counter);
+    const BatchProb *desc = (const BatchProb *)b->d_desc;
+    if (b->tiles[kFmtI8])
+      hipLaunchKernelGGL((batch_kernel<kFmtI8>), dim3(b->grid[kFmtI8]), dim3(256), 0, s,
+                         desc + b->first[kFmtI8], b->count[kFmtI8], b->tiles[kFmtI8],
+                         b->d_counter + kFmtI8);
+    if (b->tiles[kFmtF4])
+      hipLaunchKernelGGL((batch_kernel<kFmtF4>), dim3(b->grid[kFmtF4]), dim3(256), 0, s,
+                         desc + b->first[kFmtF4], b->count[kFmtF4], b->tiles[kFmtF4],
+                         b->d_counter + kFmtF4);
+    if (b->tiles[kFmtF6])
+      hipLaunchKernelGGL((batch_kernel<kFmtF6>), dim3(b->grid[kFmtF6]), dim3(256), 0, s,
+                         desc + b->first[kFmtF6], b->count[kFmtF6], b->tiles[kFmtF6],
+                         b->d_counter + kFmtF6);
     GBS_HIP(hipGetLastError());
   }
   for (uint32_t i : b->separate_idx) {
@@ -217,9 +234,11 @@ int gbs_batch_create(const gbs_problem *problems, uint64_t count, gbs_batch **ou
       return rc;
     }
     const Operand lo = view(p.lhs), ro = view(p.rhs);
-    const bool f4 = f4_supported(lo, ro, p.lhs->ncols);
-    if ((f4 || mfma_supported(lo, ro, p.lhs->ncols)) && p.lhs->ncols > 0) {
-      const int fmt = f4 ? kFmtF4 : kFmtI8;
+    const int fmt = f4_supported(lo, ro, p.lhs->ncols)   ? kFmtF4
+                    : f6_supported(lo, ro, p.lhs->ncols) ? kFmtF6
+                    : mfma_supported(lo, ro, p.lhs->ncols) ? kFmtI8
+                                                           : -1;
+    if (fmt >= 0 && p.lhs->ncols > 0) {
       rc = ensure_copy(const_cast<gbs_matrix *>(p.lhs), fmt, s, true);
       if (!rc) rc = ensure_copy(const_cast<gbs_matrix *>(p.rhs), fmt, s, true);
       if (rc) {
@@ -234,26 +253,32 @@ int gbs_batch_create(const gbs_problem *problems, uint64_t count, gbs_batch **ou
       b->separate_idx.push_back((uint32_t)i);
     }
   }
-  // longest items first: they start early and the short ones fill the tail
-  std::stable_sort(items.begin(), items.end(),
-                   [](const Item &x, const Item &y) { return x.item_cost > y.item_cost; });
+  // grouped by format; inside a group the longest items first: they start
+  // early and the short ones fill the tail
+  std::stable_sort(items.begin(), items.end(), [](const Item &x, const Item &y) {
+    return x.plan.fmt != y.plan.fmt ? x.plan.fmt < y.plan.fmt : x.item_cost > y.item_cost;
+  });
   std::vector<BatchProb> desc;
   std::vector<ZeroSpan> zero;
-  uint64_t tiles = 0;
+  const uint64_t slots = 256ull * 2;
   for (const Item &it : items) {
     const gbs_problem &p = problems[it.idx];
+    const int f = it.plan.fmt;
+    if (b->count[f] == 0) b->first[f] = (uint32_t)desc.size();
     BatchProb d{};
     d.ops = it.ops;
     d.plan = it.plan;
-    d.tile_begin = tiles;
-    tiles += d.plan.tiles();
+    d.tile_begin = b->tiles[f];
+    b->tiles[f] += d.plan.tiles();
+    b->count[f]++;
     if (d.plan.splits > 1) zero.push_back(ZeroSpan{p.d_res, (uint64_t)d.ops.M * d.ops.N});
     desc.push_back(d);
     b->mfma_idx.push_back(it.idx);
   }
-  b->total_tiles = tiles;
-  const uint64_t slots = 256ull * 2;
-  b->grid = (uint32_t)(tiles < slots ? tiles : slots);
+  for (int f = 0; f < 3; f++) {
+    b->total_tiles += b->tiles[f];
+    b->grid[f] = (uint32_t)(b->tiles[f] < slots ? b->tiles[f] : slots);
+  }
   if (!desc.empty()) {
     hipError_t e = hipMalloc(&b->d_desc, desc.size() * sizeof(BatchProb));
     if (e == hipSuccess) e = hipMalloc((void **)&b->d_counter, 256);

@@ -22,7 +22,11 @@ struct gbs_matrix {
   // (3 signed): [f4_rows][f4_ld], zero padded, same lifecycle.
   int8_t *d_f4 = nullptr;
   uint64_t f4_rows = 0, f4_ld = 0;
-  uint64_t planes_version = 1, vals_version = 0, f4_version = 0;
+  // FP6 copy (6 bits per value) for operands of at most 4 bits (5 signed):
+  // [f6_rows][f6_ld], zero padded, same lifecycle.
+  int8_t *d_f6 = nullptr;
+  uint64_t f6_rows = 0, f6_ld = 0;
+  uint64_t planes_version = 1, vals_version = 0, f4_version = 0, f6_version = 0;
   uint64_t wpr() const { return ncols_a / 64; }
   uint64_t wpp() const { return nrows_a * wpr(); }
   uint64_t words() const { return nbits * wpp(); }
@@ -84,11 +88,12 @@ int launch_fill_i32(int32_t *d, uint64_t n, int32_t v, hipStream_t s);
 int launch_popc_gemm(const Operand &l, const Operand &r, uint64_t kwords,
                      uint64_t K, int32_t *d_res, const int32_t *d_rowsum,
                      hipStream_t s);
-// Matrix-core engine (gbs_mfma.hip).  ensure_vals / ensure_f4 make the
-// int8 / FP4 copy current (allocating it on first use when may_allocate);
-// ensure_copy picks by format (0 int8, 1 FP4).
+// Matrix-core engine (gbs_mfma.hip).  ensure_vals / ensure_f4 / ensure_f6
+// make the int8 / FP4 / FP6 copy current (allocating it on first use when
+// may_allocate); ensure_copy picks by format (0 int8, 1 FP4, 2 FP6).
 int ensure_vals(gbs_matrix *m, hipStream_t s, bool may_allocate);
 int ensure_f4(gbs_matrix *m, hipStream_t s, bool may_allocate);
+int ensure_f6(gbs_matrix *m, hipStream_t s, bool may_allocate);
 int ensure_copy(gbs_matrix *m, int fmt, hipStream_t s, bool may_allocate);
 int launch_mfma_gemm(const gbs_matrix *l, const gbs_matrix *r, int32_t *d_res,
                      hipStream_t s, int fmt, int forced_shape = -1, bool allow_split = true,
@@ -96,8 +101,15 @@ int launch_mfma_gemm(const gbs_matrix *l, const gbs_matrix *r, int32_t *d_res,
 // tuning table of the MFMA engine: tile shape and K split per shape class
 // (shapes == nullptr restores the built-in heuristic; splits 0 = automatic)
 int set_mfma_table(const int *shapes, const int *splits, int n);
+// measured best time (microseconds) per format and class, as the table
+// lists them (int8 classes, then FP4, then FP6; 0 = unknown); used to pick
+// between FP6 and int8 copies.  set_mfma_table clears them.
+int set_mfma_times(const float *times_us, int n);
 bool mfma_supported(const Operand &l, const Operand &r, uint64_t K);  // int8 copies exact
 bool f4_supported(const Operand &l, const Operand &r, uint64_t K);    // FP4 copies exact
+bool f6_supported(const Operand &l, const Operand &r, uint64_t K);    // FP6 copies exact
+// copy format AUTO uses (1 FP4, 2 FP6, 0 int8), -1 if none
+int best_format(const Operand &l, const Operand &r, uint64_t K);
 gbs_algo select_algo(const Operand &l, const Operand &r, uint64_t K);
 constexpr uint64_t kValsRowAlign = 256;  // rows of the int8 copy
 constexpr uint64_t kValsColAlign = 256;  // values per row of a copy (int8: bytes)

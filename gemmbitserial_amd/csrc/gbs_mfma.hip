@@ -6,16 +6,21 @@
 //
 //   res[n*M + m] = sum_k val(rhs[n,k]) * val(lhs[m,k])      (int32)
 //
-// Two copy formats:
+// Three copy formats:
 //  * int8 (one byte per value): bipolar +-1, signed two's complement of up
 //    to 8 bits, unsigned codes of up to 7 bits; int32 accumulation, exact
 //    whenever K * max|val(lhs)| * max|val(rhs)| < 2^31 (mfma_supported);
 //  * FP4 E2M1 (two values per byte, scale 2^1, so codes 0..4 are the
 //    integers 0..4): bipolar, unsigned of up to 2 bits, signed of up to 3
-//    bits; fp32 accumulation of integer products, exact whenever
-//    K * max|val(lhs)| * max|val(rhs)| < 2^24 (f4_supported), since every
-//    partial sum is then an integer below 2^24.  Half the bytes of int8 and
-//    twice the MACs per instruction.
+//    bits;
+//  * FP6 E2M3 (6 bits per value, scale 2^3, so codes 0..16 are the
+//    integers 0..16): bipolar, unsigned of up to 4 bits, signed of up to 5
+//    bits.
+// FP4 and FP6 products accumulate integer products in fp32, exact whenever
+// K * max|val(lhs)| * max|val(rhs)| < 2^24 (f4_supported, f6_supported),
+// since every partial sum is then an integer below 2^24.  They take half
+// and three quarters of the bytes of int8, and one instruction covers
+// twice the values.
 // Either way the result equals the reference's int32 result bit for bit.
 //
 // The tile itself (gbs_mfma_tile.h) comes in several shapes; a product runs
@@ -33,7 +38,7 @@ template <int P, int Q, int WP, int KSB, int DEPTH, int FMT>
 __global__ __launch_bounds__(256, (P * Q >= 65536 || (P * Q >= 32768 && DEPTH == 0)) ? 1 : 2)
 void mfma_kernel(
     MfmaOperands a, MfmaPlan pl) {
-  __shared__ __attribute__((aligned(16))) int8_t lds[mfma_lds_bytes<P, Q, KSB>()];
+  __shared__ __attribute__((aligned(16))) int8_t lds[mfma_lds_bytes<P, Q, KSB, FMT>()];
   const uint32_t T = gridDim.x, b = blockIdx.x;
   // consecutive logical tiles run on one XCD (blocks are dealt round-robin)
   const uint64_t t = (T % 8 == 0) ? (uint64_t)(b % 8) * (T / 8) + b / 8 : b;
@@ -108,6 +113,51 @@ __global__ __launch_bounds__(256) void expand_f4_kernel(
   o[1] = make_uint4(q[4], q[5], q[6], q[7]);
 }
 
+// planes -> FP6 codes: one thread per 64-value word of a copy row, i.e.
+// two lane groups (32 values, 24 bytes each) of one 96-byte k-step.  Value
+// j of a group sits in bits 6j..6j+5 of the group's 24 bytes; bytes 0..15
+// go to offset 16g of the k-step, bytes 16..23 to offset 64 + 8g.  Code of
+// an integer v (|v| <= 16) under scale 2^3: sign bit 32 | |v|.  Zero
+// outside the real matrix, also for bipolar operands.
+__global__ __launch_bounds__(256) void expand_f6_kernel(
+    const uint64_t *planes, uint64_t nrows, uint64_t ncols, uint64_t wpr, uint64_t W,
+    uint32_t nbits, bool issigned, uint64_t rows_a, uint64_t ld, uint8_t *out) {
+  const uint64_t wpl = ld / 48;
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows_a * wpl) return;
+  const uint64_t r = i / wpl, w = i - r * wpl;
+  uint32_t d[2][6] = {{0, 0, 0, 0, 0, 0}, {0, 0, 0, 0, 0, 0}};
+  if (r < nrows && w * 64 < ncols) {
+    uint64_t p[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+    for (uint32_t b = 0; b < 5; b++)
+      if (b < nbits) p[b] = planes[b * W + r * wpr + w];
+    const uint64_t valid = ncols - w * 64;
+    const bool bip = nbits == 1 && issigned;
+#pragma unroll
+    for (int k = 0; k < 64; k++) {
+      uint32_t code = 0;
+#pragma unroll
+      for (int b = 0; b < 5; b++) code |= (uint32_t)((p[b] >> k) & 1) << b;
+      uint32_t c6;
+      if (bip) c6 = code ? 0x01u : 0x21u;
+      else if (issigned && (code >> (nbits - 1))) c6 = 0x20u | ((1u << nbits) - code);
+      else c6 = code;
+      if ((uint64_t)k >= valid) c6 = 0;
+      const int gi = k >> 5, bit = 6 * (k & 31);
+      d[gi][bit >> 5] |= c6 << (bit & 31);
+      if ((bit & 31) > 26) d[gi][(bit >> 5) + 1] |= c6 >> (32 - (bit & 31));
+    }
+  }
+  uint8_t *base = out + r * ld + (w >> 1) * 96;
+#pragma unroll
+  for (int gi = 0; gi < 2; gi++) {
+    const int g = 2 * (int)(w & 1) + gi;
+    *reinterpret_cast<uint4 *>(base + 16 * g) = make_uint4(d[gi][0], d[gi][1], d[gi][2], d[gi][3]);
+    *reinterpret_cast<uint2 *>(base + 64 + 8 * g) = make_uint2(d[gi][4], d[gi][5]);
+  }
+}
+
 uint64_t max_abs(const Operand &o) {
   if (o.bipolar) return 1;
   if (o.issigned) return 1ull << (o.nbits - 1);
@@ -140,22 +190,28 @@ int run_plan_fmt(const MfmaOperands &a, const MfmaPlan &pl, hipStream_t s) {
     case 5: return run_plan<128, 64, 4, 128, 2, FMT>(a, pl, s);
     case 6: return run_plan<128, 128, 2, 64, 1, FMT>(a, pl, s);
     case 7: return run_plan<128, 256, 2, 64, 2, FMT>(a, pl, s);
-    case 8: return run_plan<128, 128, 2, 64, 0, FMT>(a, pl, s);
-    case 9: return run_plan<128, 256, 2, 64, 0, FMT>(a, pl, s);
-    case 10: return run_plan<64, 128, 2, 64, 0, FMT>(a, pl, s);
     case 11: return run_plan<256, 256, 2, 64, 1, FMT>(a, pl, s);
-    case 12: return run_plan<256, 256, 2, 128, 1, FMT>(a, pl, s);
   }
-  return fail(GBS_EINVAL, "mfma: unknown tile shape %d", pl.shape);
+  if constexpr (FMT != kFmtF6) {  // FP6 plans never name these (fmt_shape)
+    switch (pl.shape) {
+      case 8: return run_plan<128, 128, 2, 64, 0, FMT>(a, pl, s);
+      case 9: return run_plan<128, 256, 2, 64, 0, FMT>(a, pl, s);
+      case 10: return run_plan<64, 128, 2, 64, 0, FMT>(a, pl, s);
+      case 12: return run_plan<256, 256, 2, 128, 1, FMT>(a, pl, s);
+    }
+  }
+  return fail(GBS_EINVAL, "mfma: tile shape %d not compiled for format %d", pl.shape, FMT);
 }
 
 int run_plan_id(const MfmaOperands &a, const MfmaPlan &pl, hipStream_t s) {
   if (pl.fmt == kFmtF4) return run_plan_fmt<kFmtF4>(a, pl, s);
+  if (pl.fmt == kFmtF6) return run_plan_fmt<kFmtF6>(a, pl, s);
   return run_plan_fmt<kFmtI8>(a, pl, s);
 }
 
 int g_shape[kMfmaFormats][kClasses], g_splits[kMfmaFormats][kClasses];
-bool g_table_set[kMfmaFormats] = {false, false};
+bool g_table_set[kMfmaFormats] = {false, false, false};
+float g_time_us[kMfmaFormats][kClasses];  // measured best time per class, 0 = unknown
 
 int default_shape(uint64_t M, uint64_t N) {
   (void)M;
@@ -187,9 +243,38 @@ bool f4_supported(const Operand &l, const Operand &r, uint64_t K) {
   return bound < ((unsigned __int128)1 << 24);
 }
 
+// Values an FP6 E2M3 code holds as an integer under scale 2^3: -16..16;
+// 15 is the largest unsigned 4-bit code, -16 the smallest signed 5-bit one.
+static bool f6_holds(const Operand &o) {
+  if (o.bipolar) return true;
+  return o.issigned ? o.nbits <= 5 : o.nbits <= 4;
+}
+
+bool f6_supported(const Operand &l, const Operand &r, uint64_t K) {
+  if (!f6_holds(l) || !f6_holds(r)) return false;
+  const unsigned __int128 bound = (unsigned __int128)K * max_abs(l) * max_abs(r);
+  return bound < ((unsigned __int128)1 << 24);
+}
+
+// Format AUTO uses for one product: FP4 where it holds the product (always
+// the fastest measured); else FP6 where it holds the product and the
+// tuning table measured it at least 5% faster than int8 for the product's
+// shape class (it ranges from 0.7x to 1.2x of int8 over the classes);
+// else int8.
+int best_format(const Operand &l, const Operand &r, uint64_t K) {
+  if (f4_supported(l, r, K)) return kFmtF4;
+  if (f6_supported(l, r, K)) {
+    const int c = shape_class(l.nrows, r.nrows, K);
+    const float t6 = g_time_us[kFmtF6][c], t8 = g_time_us[kFmtI8][c];
+    if (t6 > 0 && t8 > 0 && t6 < 0.95f * t8) return kFmtF6;
+  }
+  if (mfma_supported(l, r, K)) return kFmtI8;
+  return -1;
+}
+
 gbs_algo select_algo(const Operand &l, const Operand &r, uint64_t K) {
-  const bool f4 = f4_supported(l, r, K);
-  if (!f4 && !mfma_supported(l, r, K)) return GBS_ALGO_POPCOUNT;
+  const int fmt = best_format(l, r, K);
+  if (fmt < 0) return GBS_ALGO_POPCOUNT;
   const double M = (double)l.nrows, N = (double)r.nrows, Kd = (double)K;
   const double pl = l.bipolar && r.bipolar ? 1 : (double)l.nbits;
   const double pr = l.bipolar && r.bipolar ? 1 : (double)r.nbits;
@@ -198,12 +283,12 @@ gbs_algo select_algo(const Operand &l, const Operand &r, uint64_t K) {
   const double valu = M * N * ((K + 31) / 32) * 2 * pl * pr / (0.55 * 39.3e12);
   const double pbytes = (M * Kd * l.nbits + N * Kd * r.nbits) / 8 + 4 * M * N;
   const double t_popc = (valu > pbytes / (0.6 * 8e12) ? valu : pbytes / (0.6 * 8e12));
-  // matrix cores: the same instruction stream per byte of K for both
-  // formats, FP4 packing twice the values per byte
-  const int fmt = f4 ? kFmtF4 : kFmtI8;
-  const double vpb = f4 ? 2 : 1;  // values per byte of the copy
-  const double mops = 2 * M * N * Kd / vpb / (0.35 * 5e15);
-  const double ibytes = (M + N) * Kd / vpb + 4 * M * N;
+  // matrix cores: FP4 / FP6 instructions cover twice the values of int8
+  // ones at the same issue rate
+  const double vpi = fmt == kFmtI8 ? 1 : 2;                              // values per int8 step
+  const double bpv = fmt == kFmtI8 ? 1 : fmt == kFmtF4 ? 0.5 : 0.75;    // copy bytes per value
+  const double mops = 2 * M * N * Kd / vpi / (0.35 * 5e15);
+  const double ibytes = (M + N) * Kd * bpv + 4 * M * N;
   // a tile row/column only partly filled still costs the whole tile
   int shape_id, splits;
   mfma_config_for(l.nrows, r.nrows, K, fmt, &shape_id, &splits);
@@ -213,7 +298,7 @@ gbs_algo select_algo(const Operand &l, const Operand &r, uint64_t K) {
   const double t_mfma_c = mops * waste;
   const double t_mfma = (t_mfma_c > ibytes / (0.6 * 8e12) ? t_mfma_c : ibytes / (0.6 * 8e12));
   if (t_mfma >= t_popc) return GBS_ALGO_POPCOUNT;
-  return f4 ? GBS_ALGO_MFMA_FP4 : GBS_ALGO_MFMA;
+  return fmt == kFmtF4 ? GBS_ALGO_MFMA_FP4 : fmt == kFmtF6 ? GBS_ALGO_MFMA_FP6 : GBS_ALGO_MFMA;
 }
 
 int ensure_vals(gbs_matrix *m, hipStream_t s, bool may_allocate) {
@@ -258,23 +343,53 @@ int ensure_f4(gbs_matrix *m, hipStream_t s, bool may_allocate) {
   return GBS_OK;
 }
 
+int ensure_f6(gbs_matrix *m, hipStream_t s, bool may_allocate) {
+  if (m->f6_version == m->planes_version && m->d_f6) return GBS_OK;
+  if (!m->d_f6) {
+    const uint64_t rows = ((m->nrows + kValsRowAlign - 1) / kValsRowAlign) * kValsRowAlign;
+    const uint64_t ld = ((m->ncols + kValsColAlign - 1) / kValsColAlign) * kValsColAlign / 4 * 3;
+    const uint64_t rows_a = rows ? rows : kValsRowAlign, ld_a = ld ? ld : kValsColAlign / 4 * 3;
+    if (!may_allocate)
+      return fail(GBS_ERANGE, "mfma: operand has no FP6 copy yet; call gbs_workspace_reserve");
+    GBS_HIP(hipMalloc((void **)&m->d_f6, rows_a * ld_a));
+    m->f6_rows = rows_a;
+    m->f6_ld = ld_a;
+  }
+  const uint64_t n = m->f6_rows * (m->f6_ld / 48);
+  hipLaunchKernelGGL(expand_f6_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s,
+                     m->d_planes, m->nrows, m->ncols, m->wpr(), m->wpp(), (uint32_t)m->nbits,
+                     m->issigned, m->f6_rows, m->f6_ld, (uint8_t *)m->d_f6);
+  GBS_HIP(hipGetLastError());
+  m->f6_version = m->planes_version;
+  return GBS_OK;
+}
+
 int ensure_copy(gbs_matrix *m, int fmt, hipStream_t s, bool may_allocate) {
-  return fmt == kFmtF4 ? ensure_f4(m, s, may_allocate) : ensure_vals(m, s, may_allocate);
+  return fmt == kFmtF4   ? ensure_f4(m, s, may_allocate)
+         : fmt == kFmtF6 ? ensure_f6(m, s, may_allocate)
+                         : ensure_vals(m, s, may_allocate);
 }
 
 MfmaOperands mfma_operands(const gbs_matrix *l, const gbs_matrix *r, int fmt, int32_t *d_res) {
   MfmaOperands a{};
-  const bool f4 = fmt == kFmtF4;
-  a.R = f4 ? r->d_f4 : r->d_vals;
-  a.L = f4 ? l->d_f4 : l->d_vals;
-  a.r_ld = f4 ? r->f4_ld : r->vals_ld;
-  a.l_ld = f4 ? l->f4_ld : l->vals_ld;
+  const bool f4 = fmt == kFmtF4, f6 = fmt == kFmtF6;
+  a.R = f4 ? r->d_f4 : f6 ? r->d_f6 : r->d_vals;
+  a.L = f4 ? l->d_f4 : f6 ? l->d_f6 : l->d_vals;
+  a.r_ld = f4 ? r->f4_ld : f6 ? r->f6_ld : r->vals_ld;
+  a.l_ld = f4 ? l->f4_ld : f6 ? l->f6_ld : l->vals_ld;
   a.res = d_res;
   a.M = (uint32_t)l->nrows;
   a.N = (uint32_t)r->nrows;
-  const uint64_t bytes = f4 ? (l->ncols + 1) / 2 : l->ncols;
+  // nominal bytes: 64 per k-step of 64 (int8) or 128 (FP4, FP6) values
+  const uint64_t bytes = fmt == kFmtI8 ? l->ncols : (l->ncols + 127) / 128 * 64;
   a.kbytes = (uint32_t)((bytes + kMfmaKS - 1) / kMfmaKS * kMfmaKS);
   return a;
+}
+
+static bool copies_current(const gbs_matrix *m, int fmt) {
+  return fmt == kFmtF4   ? m->f4_version == m->planes_version
+         : fmt == kFmtF6 ? m->f6_version == m->planes_version
+                         : m->vals_version == m->planes_version;
 }
 
 int launch_mfma_gemm(const gbs_matrix *l, const gbs_matrix *r, int32_t *d_res,
@@ -285,10 +400,7 @@ int launch_mfma_gemm(const gbs_matrix *l, const gbs_matrix *r, int32_t *d_res,
   if (M > 0x7fffffffull || N > 0x7fffffffull)
     return fail(GBS_ERANGE, "gemm dimensions exceed 2^31");
   const MfmaOperands a = mfma_operands(l, r, fmt, d_res);
-  if (!a.R || !a.L || a.r_ld != a.l_ld ||
-      (fmt == kFmtF4 ? l->f4_version != l->planes_version || r->f4_version != r->planes_version
-                     : l->vals_version != l->planes_version ||
-                           r->vals_version != r->planes_version))
+  if (!a.R || !a.L || a.r_ld != a.l_ld || !copies_current(l, fmt) || !copies_current(r, fmt))
     return fail(GBS_EINVAL, "mfma: operand value copies are not current");
   if (a.kbytes == 0) return launch_fill_i32(d_res, M * N, 0, s);
   const MfmaPlan pl = mfma_plan(a.M, a.N, a.kbytes, fmt, forced_shape,
@@ -312,11 +424,12 @@ void mfma_config_for(uint64_t M, uint64_t N, uint64_t K, int fmt, int *shape, in
 MfmaPlan mfma_plan(uint32_t M, uint32_t N, uint32_t kbytes, int fmt, int forced_shape,
                    int forced_splits) {
   int shape, splits;
-  mfma_config_for(M, N, (uint64_t)kbytes * (fmt == kFmtF4 ? 2 : 1), fmt, &shape, &splits);
+  mfma_config_for(M, N, (uint64_t)kbytes * (fmt == kFmtI8 ? 1 : 2), fmt, &shape, &splits);
   if (forced_shape >= 0) {
     shape = forced_shape;
     splits = 0;
   }
+  shape = fmt_shape(shape, fmt);
   if (forced_splits > 0) splits = forced_splits;
   const MfmaTileShape sh = kMfmaShape[shape];
   MfmaPlan pl{};
@@ -342,14 +455,23 @@ MfmaPlan mfma_plan(uint32_t M, uint32_t N, uint32_t kbytes, int fmt, int forced_
   return pl;
 }
 
+int set_mfma_times(const float *times_us, int n) {
+  for (int f = 0; f < kMfmaFormats; f++)
+    for (int i = 0; i < kClasses; i++)
+      g_time_us[f][i] = times_us && f * kClasses + i < n ? times_us[f * kClasses + i] : 0.f;
+  return GBS_OK;
+}
+
 int set_mfma_table(const int *shapes, const int *splits, int n) {
+  set_mfma_times(nullptr, 0);
   if (!shapes) {
     for (int f = 0; f < kMfmaFormats; f++) g_table_set[f] = false;
     return GBS_OK;
   }
-  if (n != kClasses && n != kMfmaFormats * kClasses)
-    return fail(GBS_EINVAL, "tile table needs %d (int8) or %d (int8, FP4) entries, got %d",
-                kClasses, kMfmaFormats * kClasses, n);
+  if (n <= 0 || n % kClasses != 0 || n > kMfmaFormats * kClasses)
+    return fail(GBS_EINVAL,
+                "tile table needs %d entries per format (int8, then FP4, then FP6), got %d",
+                kClasses, n);
   for (int f = 0; f < kMfmaFormats; f++) {
     if (f * kClasses >= n) {
       g_table_set[f] = false;

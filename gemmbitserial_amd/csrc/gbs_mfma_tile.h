@@ -6,17 +6,24 @@
 // out WP x (4/WP) over it; each wave owns (P/WP) x (Q/(4/WP)) outputs as
 // 16 x 16 fragments.  The rhs operand goes to the MFMA A port so that a
 // fragment's output lanes run along m, the contiguous dimension of res.
-// Two operand formats share the tile: FMT 0 = int8 values and
-// v_mfma_i32_16x16x64_i8, FMT 1 = FP4 (E2M1) codes, two per byte, and
-// v_mfma_scale_f32_16x16x128_f8f6f4 with both block scales 2^1.  Either
-// way one instruction consumes 16 bytes of each operand row per lane and
-// 64 bytes of K per fragment, so staging is byte-for-byte the same.  Both
-// operand fragments take 16 contiguous bytes of a row at k-offset
-// 16*(lane/16) -- the same k-to-(lane, byte) assignment on both ports, so
-// the sum over k does not depend on the order the instruction adds in.  K
-// is staged KSB = 64, 128 or 256 bytes at a time in LDS, double-buffered,
-// with 16-byte global loads held in registers one or two stages ahead of
-// the stage being multiplied.
+// Three operand formats share the tile (the same on both ports):
+//   FMT 0  int8 values, v_mfma_i32_16x16x64_i8: a k-step is 64 values,
+//          64 bytes of a row, 16 bytes per lane;
+//   FMT 1  FP4 (E2M1) codes, two per byte, v_mfma_scale_f32_16x16x128_f8f6f4
+//          with block scales 2^1: a k-step is 128 values, 64 bytes, 16 per lane;
+//   FMT 2  FP6 (E2M3) codes, 6 bits each, the same instruction with block
+//          scales 2^3: a k-step is 128 values, 96 bytes, 24 per lane.
+// K positions are counted in nominal bytes, 64 per k-step, for every
+// format (KSB, kbeg, kend, kbytes); an FP6 row holds 96 real bytes per 64
+// nominal ones.  Both operand fragments of an instruction take the same
+// k-to-(lane, byte) assignment on both ports, so the sum over k does not
+// depend on the order the instruction adds in.  K is staged KSB nominal
+// bytes at a time in LDS, double-buffered, with 16-byte global loads held
+// in registers one or two stages ahead of the stage being multiplied.
+//
+// FP6 copies store each 96-byte k-step as the four lane groups' first 16
+// bytes (64 bytes) followed by their last 8 bytes (32 bytes), so a lane's
+// 24 bytes are one aligned 16-byte and one aligned 8-byte LDS read.
 #pragma once
 
 #include <type_traits>
@@ -25,46 +32,61 @@
 
 namespace gbs {
 
+typedef int v2i __attribute__((ext_vector_type(2)));
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v8i __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 // Operand formats of the matrix-core engine.
-constexpr int kFmtI8 = 0, kFmtF4 = 1, kMfmaFormats = 2;
-// E8M0 block scale 2^1: FP4 code c (|c| <= 4: 0, 0.5, 1, 1.5, 2) then
-// stands for the integer c, so products and their fp32 sums are exact
-// integers while |sum| < 2^24.
-constexpr int kF4Scale = 128;
+constexpr int kFmtI8 = 0, kFmtF4 = 1, kFmtF6 = 2, kMfmaFormats = 3;
+// E8M0 block scales: 2^1 makes FP4 code c (|c| <= 4: 0, 0.5, 1, 1.5, 2)
+// the integer c; 2^3 makes FP6 E2M3 code c (|c| <= 16: subnormals c/8,
+// then (1 + m/8) * 2^(e-1)) the integer c.  Products and their fp32 sums
+// are then exact integers while |sum| < 2^24.
+constexpr int kF4Scale = 128, kF6Scale = 130;
+
+template <int FMT>
+constexpr int step_bytes() { return FMT == kFmtF6 ? 96 : 64; }  // row bytes per k-step
+
+// one lane's share of an operand fragment
+template <int FMT>
+struct MfmaFrag { v4i h; };
+template <>
+struct MfmaFrag<kFmtF6> { v8i x; };  // lanes 0..5 used, loaded in place (no register copies)
 
 template <int FMT>
 using MfmaAcc = typename std::conditional<FMT == kFmtI8, v4i, v4f>::type;
 
 template <int FMT>
-__device__ __forceinline__ MfmaAcc<FMT> mfma_op(const v4i &a, const v4i &b,
+__device__ __forceinline__ MfmaAcc<FMT> mfma_op(const MfmaFrag<FMT> &a, const MfmaFrag<FMT> &b,
                                                 const MfmaAcc<FMT> &c) {
   if constexpr (FMT == kFmtI8) {
-    return __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, c, 0, 0, 0);
-  } else {
+    return __builtin_amdgcn_mfma_i32_16x16x64_i8(a.h, b.h, c, 0, 0, 0);
+  } else if constexpr (FMT == kFmtF4) {
     // FP4 operands occupy the low four registers of the 8-register slots
-    const v8i a8 = {a[0], a[1], a[2], a[3], 0, 0, 0, 0};
-    const v8i b8 = {b[0], b[1], b[2], b[3], 0, 0, 0, 0};
+    const v8i a8 = {a.h[0], a.h[1], a.h[2], a.h[3], 0, 0, 0, 0};
+    const v8i b8 = {b.h[0], b.h[1], b.h[2], b.h[3], 0, 0, 0, 0};
     return __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a8, b8, c, 4, 4, 0, kF4Scale, 0,
                                                             kF4Scale);
+  } else {
+    // FP6 operands occupy the low six registers
+    return __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a.x, b.x, c, 2, 2, 0, kF6Scale, 0,
+                                                            kF6Scale);
   }
 }
 __device__ __forceinline__ int32_t acc_value(int32_t x) { return x; }
 __device__ __forceinline__ int32_t acc_value(float x) { return (int32_t)x; }  // exact integer
 
-constexpr int kMfmaKS = 128;  // padding unit of K (bytes): every stage size divides it
+constexpr int kMfmaKS = 128;  // padding unit of K (nominal bytes): every stage size divides it
 
-// Operands of one product: value copies (int8 or FP4, same format on both
-// sides) [rows >= tile-aligned][ld bytes].
+// Operands of one product: value copies (one format on both sides)
+// [rows >= tile-aligned][ld bytes].
 struct MfmaOperands {
   const int8_t *R, *L;   // rhs / lhs values
-  uint64_t r_ld, l_ld;   // row strides (bytes)
+  uint64_t r_ld, l_ld;   // row strides (real bytes)
   int32_t *res;          // N x M
   uint32_t M, N;         // lhs rows, rhs rows
-  uint32_t kbytes;       // bytes of K in the copies, rounded up to kMfmaKS (zero padded)
+  uint32_t kbytes;       // nominal bytes of K in the copies, a multiple of kMfmaKS (zero padded)
 };
 
 // Tile shapes the engine is compiled for: P x Q outputs, WP waves along P,
@@ -92,8 +114,15 @@ __device__ __forceinline__ int lds_chunk(int row, int ch) {
   else return ch;
 }
 
-template <int P, int Q, int KSB>
-constexpr size_t mfma_lds_bytes() { return (size_t)2 * (P + Q) * lds_row_bytes<KSB>(); }
+// LDS row stride of a stage in format FMT: FP6 rows (96 bytes per k-step)
+// are padded by 16 bytes, which keeps 16 rows at one offset in distinct
+// banks for every stage length used here.
+template <int KSB, int FMT>
+constexpr int lds_row_stride() {
+  return FMT == kFmtF6 ? KSB / 64 * 96 + 16 : lds_row_bytes<KSB>();
+}
+template <int P, int Q, int KSB, int FMT = kFmtI8>
+constexpr size_t mfma_lds_bytes() { return (size_t)2 * (P + Q) * lds_row_stride<KSB, FMT>(); }
 constexpr size_t kLdsPerCU = 160 * 1024;
 constexpr size_t kMfmaLdsMax = (size_t)2 * 272 * 128;  // persistent kernel: largest carried shape (16 x 256)
 
@@ -104,16 +133,20 @@ __device__ __forceinline__ void mfma_tile(const MfmaOperands &a, uint32_t pt, ui
   constexpr int WQ = 4 / WP;
   constexpr int PW = P / WP, QW = Q / WQ;       // wave tile
   constexpr int FP = PW / 16, FQ = QW / 16;     // fragments per wave
-  constexpr int LDR = lds_row_bytes<KSB>();     // LDS row stride (bytes)
-  constexpr int CPR = KSB / 16;                 // 16-byte chunks per row and stage
+  constexpr int S = KSB / 64;                   // k-steps per stage
+  constexpr int SB = S * step_bytes<FMT>();     // real bytes per row and stage
+  constexpr int LDR = lds_row_stride<KSB, FMT>();  // LDS row stride (bytes)
+  constexpr int CPR = SB / 16;                  // 16-byte chunks per row and stage
   constexpr int RCH = P * CPR, LCH = Q * CPR;   // chunks per stage
   constexpr int RLD = (RCH + 255) / 256, LLD = (LCH + 255) / 256;
   static_assert(FP >= 1 && FQ >= 1 && PW % 16 == 0 && QW % 16 == 0, "bad tile");
   static_assert(KSB % 64 == 0 && kMfmaKS % KSB == 0, "bad stage size");
   static_assert(DEPTH == 1 || DEPTH == 2, "prefetch depth");
-  static_assert(mfma_lds_bytes<P, Q, KSB>() <= kLdsPerCU, "LDS");
+  static_assert(mfma_lds_bytes<P, Q, KSB, FMT>() <= kLdsPerCU, "LDS");
   int8_t *sR0 = lds, *sL0 = lds + P * LDR;
   constexpr int stage = (P + Q) * LDR;
+  // LDS slot of chunk ch of a row: swizzled for int8 / FP4, in place for FP6
+  auto slot = [](int row, int ch) { return FMT == kFmtF6 ? ch : lds_chunk<KSB>(row, ch); };
 
   const uint32_t p0 = pt * P, q0 = qt * Q;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -121,17 +154,18 @@ __device__ __forceinline__ void mfma_tile(const MfmaOperands &a, uint32_t pt, ui
 
   v4i gr[DEPTH][RLD], gl[DEPTH][LLD];
   auto gload = [&](v4i(&r)[RLD], v4i(&l)[LLD], uint32_t k0) {
+    const uint64_t kb = (uint64_t)k0 / 64 * step_bytes<FMT>();  // real byte offset
 #pragma unroll
     for (int i = 0; i < RLD; i++) {
       const int c = tid + 256 * i, row = c / CPR, ch = c % CPR;
       if (RCH % 256 == 0 || c < RCH)
-        r[i] = *reinterpret_cast<const v4i *>(a.R + (uint64_t)(p0 + row) * a.r_ld + k0 + ch * 16);
+        r[i] = *reinterpret_cast<const v4i *>(a.R + (uint64_t)(p0 + row) * a.r_ld + kb + ch * 16);
     }
 #pragma unroll
     for (int i = 0; i < LLD; i++) {
       const int c = tid + 256 * i, row = c / CPR, ch = c % CPR;
       if (LCH % 256 == 0 || c < LCH)
-        l[i] = *reinterpret_cast<const v4i *>(a.L + (uint64_t)(q0 + row) * a.l_ld + k0 + ch * 16);
+        l[i] = *reinterpret_cast<const v4i *>(a.L + (uint64_t)(q0 + row) * a.l_ld + kb + ch * 16);
     }
   };
   auto swrite = [&](const v4i(&r)[RLD], const v4i(&l)[LLD], int buf) {
@@ -140,14 +174,29 @@ __device__ __forceinline__ void mfma_tile(const MfmaOperands &a, uint32_t pt, ui
     for (int i = 0; i < RLD; i++) {
       const int c = tid + 256 * i, row = c / CPR, ch = c % CPR;
       if (RCH % 256 == 0 || c < RCH)
-        *reinterpret_cast<v4i *>(&sR[row * LDR + lds_chunk<KSB>(row, ch) * 16]) = r[i];
+        *reinterpret_cast<v4i *>(&sR[row * LDR + slot(row, ch) * 16]) = r[i];
     }
 #pragma unroll
     for (int i = 0; i < LLD; i++) {
       const int c = tid + 256 * i, row = c / CPR, ch = c % CPR;
       if (LCH % 256 == 0 || c < LCH)
-        *reinterpret_cast<v4i *>(&sL[row * LDR + lds_chunk<KSB>(row, ch) * 16]) = l[i];
+        *reinterpret_cast<v4i *>(&sL[row * LDR + slot(row, ch) * 16]) = l[i];
     }
+  };
+  // lane's fragment of LDS row `row`, k-step kk of the stage
+  auto frag = [&](const int8_t *base, int row, int kk) {
+    MfmaFrag<FMT> f;
+    const int g = lane >> 4;
+    if constexpr (FMT == kFmtF6) {
+      const int8_t *p = base + row * LDR + kk * 96;
+      f.x.s0123 = *reinterpret_cast<const v4i *>(p + 16 * g);
+      f.x.s45 = *reinterpret_cast<const v2i *>(p + 64 + 8 * g);
+      f.x.s67 = v2i{0, 0};
+    } else {
+      f.h = *reinterpret_cast<const v4i *>(base + row * LDR +
+                                           lds_chunk<KSB>(row, kk * 4 + g) * 16);
+    }
+    return f;
   };
 
   MfmaAcc<FMT> acc[FP][FQ];
@@ -160,19 +209,12 @@ __device__ __forceinline__ void mfma_tile(const MfmaOperands &a, uint32_t pt, ui
   auto compute = [&](int buf) {
     const int8_t *sR = sR0 + buf * stage, *sL = sL0 + buf * stage;
 #pragma unroll
-    for (int kk = 0; kk < KSB / 64; kk++) {
-      v4i af[FP], bf[FQ];
-      const int ck = kk * 4 + (lane >> 4);  // logical 16-byte chunk of this lane
+    for (int kk = 0; kk < S; kk++) {
+      MfmaFrag<FMT> af[FP], bf[FQ];
 #pragma unroll
-      for (int i = 0; i < FP; i++) {
-        const int row = wp * PW + i * 16 + fr;
-        af[i] = *reinterpret_cast<const v4i *>(&sR[row * LDR + lds_chunk<KSB>(row, ck) * 16]);
-      }
+      for (int i = 0; i < FP; i++) af[i] = frag(sR, wp * PW + i * 16 + fr, kk);
 #pragma unroll
-      for (int j = 0; j < FQ; j++) {
-        const int row = wq * QW + j * 16 + fr;
-        bf[j] = *reinterpret_cast<const v4i *>(&sL[row * LDR + lds_chunk<KSB>(row, ck) * 16]);
-      }
+      for (int j = 0; j < FQ; j++) bf[j] = frag(sL, wq * QW + j * 16 + fr, kk);
 #pragma unroll
       for (int i = 0; i < FP; i++)
 #pragma unroll
@@ -254,6 +296,7 @@ template <int P, int Q, int WP, int FMT = kFmtI8>
 __device__ __forceinline__ void mfma_tile_pp(const MfmaOperands &a, uint32_t pt, uint32_t qt,
                                              uint32_t kbeg, uint32_t kend, bool atomic,
                                              int8_t *lds) {
+  static_assert(FMT != kFmtF6, "the pipelined tile takes 16-byte fragments only");
   constexpr int KSB = 64, LDR = KSB + 16;
   constexpr int WQ = 4 / WP;
   constexpr int PW = P / WP, QW = Q / WQ;
@@ -298,22 +341,22 @@ __device__ __forceinline__ void mfma_tile_pp(const MfmaOperands &a, uint32_t pt,
       if (LCH % 256 == 0 || c < LCH) *reinterpret_cast<v4i *>(&sL[row * LDR + ch * 16]) = gl[i];
     }
   };
-  v4i fa[2][FP], fb[2][FQ];
-  auto lread = [&](int buf, v4i(&xa)[FP], v4i(&xb)[FQ]) {
+  MfmaFrag<FMT> fa[2][FP], fb[2][FQ];
+  auto lread = [&](int buf, MfmaFrag<FMT>(&xa)[FP], MfmaFrag<FMT>(&xb)[FQ]) {
     const int8_t *sR = sR0 + buf * stage, *sL = sL0 + buf * stage;
 #pragma unroll
     for (int i = 0; i < FP; i++)
-      xa[i] = *reinterpret_cast<const v4i *>(&sR[(wp * PW + i * 16 + fr) * LDR + fk]);
+      xa[i].h = *reinterpret_cast<const v4i *>(&sR[(wp * PW + i * 16 + fr) * LDR + fk]);
 #pragma unroll
     for (int j = 0; j < FQ; j++)
-      xb[j] = *reinterpret_cast<const v4i *>(&sL[(wq * QW + j * 16 + fr) * LDR + fk]);
+      xb[j].h = *reinterpret_cast<const v4i *>(&sL[(wq * QW + j * 16 + fr) * LDR + fk]);
   };
   MfmaAcc<FMT> acc[FP][FQ];
 #pragma unroll
   for (int i = 0; i < FP; i++)
 #pragma unroll
     for (int j = 0; j < FQ; j++) acc[i][j] = MfmaAcc<FMT>{0, 0, 0, 0};
-  auto mma = [&](const v4i(&xa)[FP], const v4i(&xb)[FQ]) {
+  auto mma = [&](const MfmaFrag<FMT>(&xa)[FP], const MfmaFrag<FMT>(&xb)[FQ]) {
 #pragma unroll
     for (int i = 0; i < FP; i++)
 #pragma unroll
@@ -361,45 +404,62 @@ __device__ __forceinline__ void mfma_tile_pp(const MfmaOperands &a, uint32_t pt,
       }
 }
 
+// Shape run for shape s in format fmt: FP6 has no pipelined tile (it takes
+// the staged tile of the same size) and no 256 x 256 tile with 128-byte
+// stages (its LDS would not fit).
+__host__ __device__ constexpr int fmt_shape(int s, int fmt) {
+  return fmt != kFmtF6 ? s : s == 8 ? 6 : s == 9 ? 7 : s == 10 ? 1 : s == 12 ? 11 : s;
+}
+
 // Shapes the persistent kernel carries: all but the 128 x 256 and 256 x 256
 // ones, whose registers or LDS would make the shared kernel spill or drop to
 // one block per CU; they map to the 128 x 128 shape of the same pipeline.
 // With FP4 copies the 128 x 128 tile with 128-byte stages maps to the
-// 64-byte-stage one (the FP4 code path holds more registers).
+// 64-byte-stage one (the FP4 code path holds more registers); with FP6
+// copies (wider LDS rows, more registers) only 32 x 256 and 64 x 64 are
+// carried.
 __host__ __device__ constexpr int batch_shape(int s, int fmt = 0) {
-  return fmt == 1 && (s == 0 || s == 7 || s == 11 || s == 12) ? 6
-         : s == 7 || s == 11 || s == 12                      ? 0
-         : s == 9                                              ? 8
-                                                               : s;
+  return fmt == 2   ? (s == 2 || s == 3 ? 2 : 4)
+         : fmt == 1 && (s == 0 || s == 7 || s == 11 || s == 12) ? 6
+         : s == 7 || s == 11 || s == 12                          ? 0
+         : s == 9                                                  ? 8
+                                                                   : s;
 }
 
 constexpr bool batch_shapes_fit_lds() {
-  for (int s = 0; s < kMfmaShapes; s++) {
-    const MfmaTileShape t = kMfmaShape[batch_shape(s)];  // FP4 maps a subset of these
-    const int row = t.KSB >= 128 ? t.KSB : t.KSB + 16;
-    if ((size_t)2 * (t.P + t.Q) * row > kMfmaLdsMax) return false;
-  }
+  for (int f = 0; f < kMfmaFormats; f++)
+    for (int s = 0; s < kMfmaShapes; s++) {
+      const MfmaTileShape t = kMfmaShape[batch_shape(s, f)];
+      const int row = f == kFmtF6 ? t.KSB / 64 * 96 + 16 : t.KSB >= 128 ? t.KSB : t.KSB + 16;
+      if ((size_t)2 * (t.P + t.Q) * row > kMfmaLdsMax) return false;
+    }
   return true;
 }
 static_assert(batch_shapes_fit_lds(), "a shape of the persistent kernel overflows its LDS");
 
+// Dispatch one tile by shape id (the persistent kernel's per-product switch).
 template <int FMT>
 __device__ __forceinline__ void mfma_tile_by_shape(int shape, const MfmaOperands &a,
                                                    uint32_t pt, uint32_t qt, uint32_t kbeg,
                                                    uint32_t kend, bool atomic, int8_t *lds) {
-  switch (shape) {
-    case 0:
-      if constexpr (FMT == kFmtI8)
-        mfma_tile<128, 128, 2, 128, 2, FMT>(a, pt, qt, kbeg, kend, atomic, lds);
-      break;
-    case 1: mfma_tile<64, 128, 2, 128, 2, FMT>(a, pt, qt, kbeg, kend, atomic, lds); break;
-    case 2: mfma_tile<32, 256, 1, 64, 2, FMT>(a, pt, qt, kbeg, kend, atomic, lds); break;
-    case 3: mfma_tile<16, 256, 1, 128, 2, FMT>(a, pt, qt, kbeg, kend, atomic, lds); break;
-    case 4: mfma_tile<64, 64, 2, 128, 2, FMT>(a, pt, qt, kbeg, kend, atomic, lds); break;
-    case 5: mfma_tile<128, 64, 4, 128, 2, FMT>(a, pt, qt, kbeg, kend, atomic, lds); break;
-    case 6: mfma_tile<128, 128, 2, 64, 1, FMT>(a, pt, qt, kbeg, kend, atomic, lds); break;
-    case 8: mfma_tile_pp<128, 128, 2, FMT>(a, pt, qt, kbeg, kend, atomic, lds); break;
-    default: mfma_tile_pp<64, 128, 2, FMT>(a, pt, qt, kbeg, kend, atomic, lds); break;
+  if constexpr (FMT == kFmtF6) {
+    if (shape == 2) mfma_tile<32, 256, 1, 64, 2, FMT>(a, pt, qt, kbeg, kend, atomic, lds);
+    else mfma_tile<64, 64, 2, 128, 2, FMT>(a, pt, qt, kbeg, kend, atomic, lds);
+  } else {
+    switch (shape) {
+      case 0:
+        if constexpr (FMT == kFmtI8)
+          mfma_tile<128, 128, 2, 128, 2, FMT>(a, pt, qt, kbeg, kend, atomic, lds);
+        break;
+      case 1: mfma_tile<64, 128, 2, 128, 2, FMT>(a, pt, qt, kbeg, kend, atomic, lds); break;
+      case 2: mfma_tile<32, 256, 1, 64, 2, FMT>(a, pt, qt, kbeg, kend, atomic, lds); break;
+      case 3: mfma_tile<16, 256, 1, 128, 2, FMT>(a, pt, qt, kbeg, kend, atomic, lds); break;
+      case 4: mfma_tile<64, 64, 2, 128, 2, FMT>(a, pt, qt, kbeg, kend, atomic, lds); break;
+      case 5: mfma_tile<128, 64, 4, 128, 2, FMT>(a, pt, qt, kbeg, kend, atomic, lds); break;
+      case 6: mfma_tile<128, 128, 2, 64, 1, FMT>(a, pt, qt, kbeg, kend, atomic, lds); break;
+      case 8: mfma_tile_pp<128, 128, 2, FMT>(a, pt, qt, kbeg, kend, atomic, lds); break;
+      default: mfma_tile_pp<64, 128, 2, FMT>(a, pt, qt, kbeg, kend, atomic, lds); break;
+    }
   }
 }
 

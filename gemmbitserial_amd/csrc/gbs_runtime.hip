@@ -121,15 +121,18 @@ int gemm_enqueue(gbs_workspace *ws, const gbs_matrix *l, const gbs_matrix *r,
   const uint64_t K = l->ncols;
   const uint64_t kwords = (K + 63) / 64;
   gbs_algo eng = algo == GBS_ALGO_AUTO ? select_algo(lo, ro, K) : algo;
-  if (eng == GBS_ALGO_MFMA || eng == GBS_ALGO_MFMA_FP4) {
-    const bool f4 = eng == GBS_ALGO_MFMA_FP4;
-    if (f4 ? !f4_supported(lo, ro, K) : !mfma_supported(lo, ro, K))
+  if (eng == GBS_ALGO_MFMA || eng == GBS_ALGO_MFMA_FP4 || eng == GBS_ALGO_MFMA_FP6) {
+    const int fmt = eng == GBS_ALGO_MFMA_FP4 ? kFmtF4 : eng == GBS_ALGO_MFMA_FP6 ? kFmtF6 : kFmtI8;
+    const bool ok = fmt == kFmtF4   ? f4_supported(lo, ro, K)
+                    : fmt == kFmtF6 ? f6_supported(lo, ro, K)
+                                    : mfma_supported(lo, ro, K);
+    if (!ok)
       return fail(GBS_ENOTSUP,
                   "gemm: the %s matrix-core engine cannot hold this product exactly "
                   "(bitwidths %llu x %llu, depth %llu)",
-                  f4 ? "FP4" : "int8", (unsigned long long)l->nbits,
-                  (unsigned long long)r->nbits, (unsigned long long)K);
-    const int fmt = f4 ? kFmtF4 : kFmtI8;
+                  fmt == kFmtF4 ? "FP4" : fmt == kFmtF6 ? "FP6" : "int8",
+                  (unsigned long long)l->nbits, (unsigned long long)r->nbits,
+                  (unsigned long long)K);
     int rc = ensure_copy(const_cast<gbs_matrix *>(l), fmt, s, may_allocate);
     if (!rc) rc = ensure_copy(const_cast<gbs_matrix *>(r), fmt, s, may_allocate);
     if (rc) return rc;
@@ -245,6 +248,7 @@ int gbs_matrix_destroy(gbs_matrix *m) {
   GBS_HIP(hipFree(m->d_planes));
   if (m->d_vals) GBS_HIP(hipFree(m->d_vals));
   if (m->d_f4) GBS_HIP(hipFree(m->d_f4));
+  if (m->d_f6) GBS_HIP(hipFree(m->d_f6));
   delete m;
   return GBS_OK;
 }
@@ -440,6 +444,10 @@ int gbs_workspace_reserve(gbs_workspace *ws, const gbs_matrix *lhs, const gbs_ma
       if ((rc = ensure_f4(const_cast<gbs_matrix *>(lhs), s, true))) return rc;
       if ((rc = ensure_f4(const_cast<gbs_matrix *>(rhs), s, true))) return rc;
     }
+    if (f6_supported(view(lhs), view(rhs), lhs->ncols)) {
+      if ((rc = ensure_f6(const_cast<gbs_matrix *>(lhs), s, true))) return rc;
+      if ((rc = ensure_f6(const_cast<gbs_matrix *>(rhs), s, true))) return rc;
+    }
     GBS_HIP(hipStreamSynchronize(s));
   }
   return GBS_OK;
@@ -539,12 +547,14 @@ int gbs_gemm_async_config(gbs_workspace *ws, const gbs_matrix *lhs, const gbs_ma
   if (rc) return rc;
   (void)ws;
   if (lhs->ncols != rhs->ncols) return fail(GBS_EINVAL, "gbs_gemm_async_config: depths differ");
-  if (algo != GBS_ALGO_MFMA && algo != GBS_ALGO_MFMA_FP4)
-    return fail(GBS_EINVAL, "gbs_gemm_async_config: algo must be GBS_ALGO_MFMA or _MFMA_FP4");
-  const int fmt = algo == GBS_ALGO_MFMA_FP4 ? kFmtF4 : kFmtI8;
-  if (fmt == kFmtF4 ? !f4_supported(view(lhs), view(rhs), lhs->ncols)
-                    : !mfma_supported(view(lhs), view(rhs), lhs->ncols))
-    return fail(GBS_ENOTSUP, "gbs_gemm_async_config: not exact in this operand format");
+  if (algo != GBS_ALGO_MFMA && algo != GBS_ALGO_MFMA_FP4 && algo != GBS_ALGO_MFMA_FP6)
+    return fail(GBS_EINVAL, "gbs_gemm_async_config: algo must be a matrix-core engine");
+  const int fmt = algo == GBS_ALGO_MFMA_FP4 ? kFmtF4 : algo == GBS_ALGO_MFMA_FP6 ? kFmtF6 : kFmtI8;
+  const Operand lo = view(lhs), ro = view(rhs);
+  const bool ok = fmt == kFmtF4   ? f4_supported(lo, ro, lhs->ncols)
+                  : fmt == kFmtF6 ? f6_supported(lo, ro, lhs->ncols)
+                                  : mfma_supported(lo, ro, lhs->ncols);
+  if (!ok) return fail(GBS_ENOTSUP, "gbs_gemm_async_config: not exact in this operand format");
   if (shape < 0 || shape >= kMfmaShapes || splits < 1)
     return fail(GBS_EINVAL, "gbs_gemm_async_config: shape %d / splits %d out of range", shape,
                 splits);
@@ -576,26 +586,32 @@ int gbs_tune_load(const char *path) {
   FILE *f = fopen(path, "r");
   if (!f) return fail(GBS_EINVAL, "gbs_tune_load: cannot open %s", path);
   int shapes[kMfmaFormats * kClasses], splits[kMfmaFormats * kClasses];
+  float times[kMfmaFormats * kClasses];
   for (int i = 0; i < kMfmaFormats * kClasses; i++) {
     shapes[i] = -1;
     splits[i] = 0;
+    times[i] = 0.f;
   }
   char line[512];
   int n = 0;
   while (fgets(line, sizeof(line), f)) {
     int c, sh, sp = 0, fmt = 0;
+    float us = 0.f;
     if (line[0] == '#') continue;
-    const int got = sscanf(line, "%d %d %d %d", &c, &sh, &sp, &fmt);
+    const int got = sscanf(line, "%d %d %d %d %f", &c, &sh, &sp, &fmt, &us);
     if (got >= 2 && c >= 0 && c < kClasses && sh >= 0 && sh < kMfmaShapes && fmt >= 0 &&
         fmt < kMfmaFormats) {
       shapes[fmt * kClasses + c] = sh;
       splits[fmt * kClasses + c] = (got >= 3 && sp > 0) ? sp : 0;
+      times[fmt * kClasses + c] = (got >= 5 && us > 0.f) ? us : 0.f;
       n++;
     }
   }
   fclose(f);
   if (n == 0) return fail(GBS_EINVAL, "gbs_tune_load: no entries in %s", path);
-  return set_mfma_table(shapes, splits, kMfmaFormats * kClasses);
+  int rc = set_mfma_table(shapes, splits, kMfmaFormats * kClasses);
+  if (!rc) rc = set_mfma_times(times, kMfmaFormats * kClasses);
+  return rc;
 }
 
 }  // extern "C"

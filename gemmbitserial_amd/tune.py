@@ -1,14 +1,15 @@
 """Tile autotuner of the matrix-core engine.
 
 Products fall into shape classes (buckets of M, N and K; gbs_tune_class_of).
-For every class and operand format (int8 copies, FP4 copies) this times
+For every class and operand format (int8, FP4, FP6 copies) this times
 each compiled tile shape with each K split (1, 2, 4, ... while a split
 keeps at least 256 bytes of K) on a representative product (2-bit signed x
 2-bit unsigned, copies current, result in HBM) and keeps the fastest pair;
-the table is written as text, one "class shape splits format" line per
-class and format, and loaded by the library at gbs_init when it sits next
-to libgbs_hip.so (gemmbitserial_amd/tune_table.txt) or through
-gbs_tune_load.
+the table is written as text, one "class shape splits format microseconds"
+line per class and format, and loaded by the library at gbs_init when it
+sits next to libgbs_hip.so (gemmbitserial_amd/tune_table.txt) or through
+gbs_tune_load.  The times let AUTO pick FP6 over int8 copies only for the
+classes where FP6 measured faster.
 
     python -m gemmbitserial_amd.tune [--out PATH] [--reps 20]
 """
@@ -37,7 +38,7 @@ SHAPES = ["128x128", "64x128", "32x256/k64", "16x256", "64x64", "128x64", "128x1
           "256x256"]
 
 
-FORMATS = [("int8", gbs.MFMA), ("fp4", gbs.MFMA_FP4)]
+FORMATS = [("int8", gbs.MFMA), ("fp4", gbs.MFMA_FP4), ("fp6", gbs.MFMA_FP6)]
 
 
 def time_config(l, r, d, algo, shape, splits, reps, stream):
@@ -75,7 +76,7 @@ def tune(reps=10, log=print):
         time_config(l, r, d, gbs.MFMA, 4, 1, 1, stream)
         want = d.to_host(np.int32, (N, M))
         for f, (fname, algo) in enumerate(FORMATS):
-            kbytes = K if algo == gbs.MFMA else K // 2
+            kbytes = K if algo == gbs.MFMA else K // 2  # nominal bytes (64 per k-step)
             times = {}
             for sh in range(n_shapes):
                 sp = 1
@@ -106,13 +107,13 @@ def write_table(shapes, splits, detail, path):
                 "measured by gemmbitserial_amd.tune\n")
         f.write("# shapes: " + " ".join(f"{i}={s}" for i, s in enumerate(SHAPES)) + "\n")
         f.write("# formats: " + " ".join(f"{i}={n}" for i, (n, _) in enumerate(FORMATS)) + "\n")
-        f.write("# class shape splits format  # representative M N K, best time, unsplit times\n")
+        f.write("# class shape splits format us  # representative M N K, unsplit times\n")
         for i, sh in enumerate(shapes):
             if sh >= 0:
                 fmt, cls = divmod(i, n_classes)
                 d = detail.get((fmt, cls), {})
-                f.write(f"{cls} {sh} {splits[i]} {fmt}  # M={d.get('M')} N={d.get('N')} "
-                        f"K={d.get('K')} best={d.get('us_best')}us "
+                f.write(f"{cls} {sh} {splits[i]} {fmt} {d.get('us_best', 0)}  # M={d.get('M')} "
+                        f"N={d.get('N')} K={d.get('K')} "
                         f"unsplit={d.get('us_unsplit_by_shape')}\n")
 
 

@@ -62,8 +62,10 @@ typedef enum gbs_algo {
   GBS_ALGO_AUTO = 0,
   GBS_ALGO_POPCOUNT = 1, /* fused AND/XOR + popcount over bit-planes */
   GBS_ALGO_MFMA = 2,     /* planes expanded to int8, v_mfma_i32_*_i8 */
-  GBS_ALGO_MFMA_FP4 = 3  /* planes expanded to FP4 codes (operands of <= 2
+  GBS_ALGO_MFMA_FP4 = 3, /* planes expanded to FP4 codes (operands of <= 2
                             bits, <= 3 signed), v_mfma_scale_f32_*_f8f6f4 */
+  GBS_ALGO_MFMA_FP6 = 4  /* planes expanded to FP6 codes (operands of <= 4
+                            bits, <= 5 signed), v_mfma_scale_f32_*_f8f6f4 */
 } gbs_algo;
 
 typedef struct gbs_matrix gbs_matrix;       /* device-resident bit-serial matrix */
@@ -227,16 +229,17 @@ int gbs_batch_info(const gbs_batch *b, uint64_t *tiles, uint64_t *persistent,
  * gemmbitserial.hpp:343-377). */
 int gbs_tune_classes(int *n_classes, int *n_shapes);
 int gbs_tune_class_of(uint64_t M, uint64_t N, uint64_t K, int *cls);
-/* shapes[cls], splits[cls] for every class, n = n_classes (int8 copies)
- * or 2 * n_classes (int8 classes, then FP4 classes).  shapes NULL: built-in
+/* shapes[cls], splits[cls] for every class, n = n_classes, 2 * n_classes
+ * or 3 * n_classes (int8 classes, then FP4 classes, then FP6 classes).  shapes NULL: built-in
  * heuristic; a shape of -1: heuristic for that class; splits NULL or 0: K
  * split chosen from the tile count. */
 int gbs_tune_set(const int *shapes, const int *splits, int n);
-/* Text table, one "class shape [splits [format]]" line per class and
- * format (0 int8, 1 FP4), '#' comments. */
+/* Text table, one "class shape [splits [format [microseconds]]]" line per
+ * class and format (0 int8, 1 FP4, 2 FP6), '#' comments.  The measured
+ * times decide between FP6 and int8 copies for GBS_ALGO_AUTO. */
 int gbs_tune_load(const char *path);
-/* One matrix-core product (algo GBS_ALGO_MFMA or GBS_ALGO_MFMA_FP4) with a
- * forced tile shape and K split (the tuner's probe).  Operands must
+/* One matrix-core product (algo GBS_ALGO_MFMA, _MFMA_FP4 or _MFMA_FP6)
+ * with a forced tile shape and K split (the tuner's probe).  Operands must
  * already have current copies in that format (gbs_workspace_reserve). */
 int gbs_gemm_async_config(gbs_workspace *ws, const gbs_matrix *lhs,
                           const gbs_matrix *rhs, int32_t *d_res, gbs_algo algo,

@@ -385,3 +385,116 @@ def test_fp4_every_tile_shape_and_split(gbs, orc):
                                                           shape, splits, None))
                 gbs.sync()
                 assert np.array_equal(d.to_host(np.int32, (N, M)), want), (shape, splits)
+
+
+# ------------------------------------------------------------ FP6 MFMA engine
+# operands an FP6 code holds exactly: bipolar, unsigned <= 4 bits, signed <= 5 bits
+F6_ENC = [(1, True), (2, False), (3, False), (3, True), (4, False), (4, True), (5, True)]
+
+
+@pytest.mark.parametrize("lenc", F6_ENC)
+@pytest.mark.parametrize("renc", F6_ENC)
+def test_fp6_all_encodings(gbs, orc, lenc, renc):
+    (lb, ls), (rb, rs) = lenc, renc
+    rng = np.random.default_rng(11000 + lb * 100 + rb * 10 + 2 * ls + rs)
+    for (M, K, N) in [(3, 6, 9), (17, 34, 1), (130, 700, 67), (64, 1030, 200),
+                      (257, 3000, 129)]:
+        say("fp6", lb, rb, ls, rs, M, K, N)
+        lhs = rand_values(rng, (M, K), lb, ls)
+        rhs = rand_values(rng, (N, K), rb, rs)
+        got = run_gemm(gbs, lhs, rhs, lb, rb, ls, rs, algo="mfma_fp6")
+        assert np.array_equal(got, expected(lhs, rhs, lb, rb, ls, rs))
+
+
+def test_fp6_refuses_what_it_cannot_hold(gbs):
+    for (lb, ls) in [(5, False), (6, True)]:
+        l = gbs.DeviceMatrix(lb, 4, 64, ls)
+        r = gbs.DeviceMatrix(1, 4, 64, True)
+        with pytest.raises(gbs.GbsError, match="FP6 matrix-core engine cannot hold"):
+            gbs.gemm(l, r, "mfma_fp6")
+    # K * 8 * 15 reaches 2^24
+    K = 140000
+    l = gbs.DeviceMatrix(4, 2, K, True)
+    r = gbs.DeviceMatrix(4, 2, K, False)
+    with pytest.raises(gbs.GbsError, match="FP6"):
+        gbs.gemm(l, r, "mfma_fp6")
+    assert gbs.select(l, r) != "mfma_fp6"
+
+
+def test_fp6_exact_just_below_the_bound(gbs, orc):
+    rng = np.random.default_rng(2025)
+    K = 139000                              # K * 8 * 15 < 2^24
+    M, N = 40, 24
+    lhs = rng.choice(np.array([-8, 7, -5, 3], np.int8), size=(M, K), p=[0.9, 0.04, 0.03, 0.03])
+    rhs = rng.choice(np.array([15, 13, 1, 0], np.int8), size=(N, K), p=[0.9, 0.04, 0.03, 0.03])
+    l, r = gbs.from_values(lhs, 4, True), gbs.from_values(rhs, 4, False)
+    got = gbs.gemm(l, r, "mfma_fp6")
+    want = O.wrap_i32(O.blas_gemm_exact(lhs.astype(np.int64), rhs.astype(np.int64)))
+    assert np.abs(want).max() > (1 << 23)
+    assert np.array_equal(got, want)
+    # signed 5-bit reaches -16, the largest FP6 magnitude used
+    K2 = 60000                              # K * 16 * 16 < 2^24
+    a = rng.choice(np.array([-16, 15, -3], np.int8), size=(20, K2), p=[0.9, 0.05, 0.05])
+    b = rng.choice(np.array([-16, 11, 2], np.int8), size=(12, K2), p=[0.9, 0.05, 0.05])
+    la, lb_ = gbs.from_values(a, 5, True), gbs.from_values(b, 5, True)
+    want2 = O.wrap_i32(O.blas_gemm_exact(a.astype(np.int64), b.astype(np.int64)))
+    assert np.abs(want2).max() > (1 << 23)
+    assert np.array_equal(gbs.gemm(la, lb_, "mfma_fp6"), want2)
+
+
+def test_fp6_every_tile_shape_and_split(gbs, orc):
+    n_classes, n_shapes = gbs.tune_classes()
+    rng = np.random.default_rng(299)
+    ws = gbs.Workspace()
+    for (M, K, N, lb, rb, ls, rs) in [(300, 1100, 70, 3, 4, False, True),
+                                      (129, 2048, 257, 4, 4, True, False),
+                                      (64, 320, 16, 5, 1, True, True)]:
+        lhs = rand_values(rng, (M, K), lb, ls)
+        rhs = rand_values(rng, (N, K), rb, rs)
+        l, r = gbs.from_values(lhs, lb, ls), gbs.from_values(rhs, rb, rs)
+        ws.reserve(l, r)
+        want = expected(lhs, rhs, lb, rb, ls, rs)
+        d = gbs.DeviceBuffer(4 * M * N)
+        for shape in range(n_shapes):
+            for splits in (1, 3):
+                say("fp6 shape", shape, "splits", splits, M, K, N)
+                d.fill_zero()
+                gbs.check(gbs.lib().gbs_gemm_async_config(ws.h, l.h, r.h, d.ptr, gbs.MFMA_FP6,
+                                                          shape, splits, None))
+                gbs.sync()
+                assert np.array_equal(d.to_host(np.int32, (N, M)), want), (shape, splits)
+
+
+def test_auto_picks_format_from_support_and_measured_times(gbs, tmp_path):
+    import os
+    cases = [((1, True), (1, True), "mfma_fp4"), ((2, False), (3, True), "mfma_fp4"),
+             ((3, False), (2, True), "mfma"), ((4, True), (4, False), "mfma"),
+             ((6, True), (2, False), "mfma")]
+    M, N, K = 2048, 512, 2048
+    mats = {}
+
+    def mat(b, sg, rows):
+        key = (b, sg, rows)
+        if key not in mats:
+            mats[key] = gbs.DeviceMatrix(b, rows, K, sg)
+        return mats[key]
+
+    table = os.path.join(os.path.dirname(gbs.__file__), "tune_table.txt")
+    cls = gbs.tune_class_of(M, N, K)
+    try:
+        gbs.tune_set(None)              # no measured times: FP6 is never preferred
+        for (lb, ls), (rb, rs), want in cases:
+            assert gbs.select(mat(lb, ls, M), mat(rb, rs, N)) == want, ((lb, ls), (rb, rs))
+        # a table that measured FP6 faster than int8 for this class
+        t = tmp_path / "table.txt"
+        t.write_text(f"{cls} 4 1 0 10.0\n{cls} 4 1 2 9.0\n")
+        gbs.tune_load(str(t))
+        assert gbs.select(mat(3, False, M), mat(2, True, N)) == "mfma_fp6"
+        assert gbs.select(mat(4, True, M), mat(4, False, N)) == "mfma_fp6"
+        assert gbs.select(mat(6, True, M), mat(2, False, N)) == "mfma"      # FP6 cannot hold it
+        assert gbs.select(mat(1, True, M), mat(1, True, N)) == "mfma_fp4"
+        t.write_text(f"{cls} 4 1 0 10.0\n{cls} 4 1 2 9.8\n")             # within 5 %: int8
+        gbs.tune_load(str(t))
+        assert gbs.select(mat(3, False, M), mat(2, True, N)) == "mfma"
+    finally:
+        gbs.tune_load(table)
