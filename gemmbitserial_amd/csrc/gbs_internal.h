@@ -112,7 +112,8 @@ bool f6_supported(const Operand &l, const Operand &r, uint64_t K);    // FP6 cop
 int best_format(const Operand &l, const Operand &r, uint64_t K);
 gbs_algo select_algo(const Operand &l, const Operand &r, uint64_t K);
 constexpr uint64_t kValsRowAlign = 256;  // rows of the int8 copy
-constexpr uint64_t kValsColAlign = 256;  // values per row of a copy (int8: bytes)
+constexpr uint64_t kValsColAlign = 256;  // values per row of the int8 copy (bytes)
+constexpr uint64_t kPackedColAlign = 512;  // values per row of the FP4 / FP6 copies
 
 size_t dtype_size(gbs_dtype dt);
 

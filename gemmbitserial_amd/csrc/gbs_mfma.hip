@@ -38,7 +38,7 @@ template <int P, int Q, int WP, int KSB, int DEPTH, int FMT>
 __global__ __launch_bounds__(256, (P * Q >= 65536 || (P * Q >= 32768 && DEPTH == 0)) ? 1 : 2)
 void mfma_kernel(
     MfmaOperands a, MfmaPlan pl) {
-  __shared__ __attribute__((aligned(16))) int8_t lds[mfma_lds_bytes<P, Q, KSB, FMT>()];
+  __shared__ __attribute__((aligned(16))) int8_t lds[mfma_lds_bytes<P, Q, KSB, FMT, DEPTH>()];
   const uint32_t T = gridDim.x, b = blockIdx.x;
   // consecutive logical tiles run on one XCD (blocks are dealt round-robin)
   const uint64_t t = (T % 8 == 0) ? (uint64_t)(b % 8) * (T / 8) + b / 8 : b;
@@ -46,7 +46,9 @@ void mfma_kernel(
   plan_decode(pl, t, pt, qt, sp);
   const uint32_t kbeg = sp * pl.kchunk;
   const uint32_t kend = min(a.kbytes, kbeg + pl.kchunk);
-  if constexpr (DEPTH == 0)
+  if constexpr (DEPTH >= 3)
+    mfma_tile_dl<P, Q, WP, KSB, DEPTH, FMT>(a, pt, qt, kbeg, kend, pl.splits > 1, lds);
+  else if constexpr (DEPTH == 0)
     mfma_tile_pp<P, Q, WP, FMT>(a, pt, qt, kbeg, kend, pl.splits > 1, lds);
   else
     mfma_tile<P, Q, WP, KSB, DEPTH, FMT>(a, pt, qt, kbeg, kend, pl.splits > 1, lds);
@@ -198,6 +200,11 @@ int run_plan_fmt(const MfmaOperands &a, const MfmaPlan &pl, hipStream_t s) {
       case 9: return run_plan<128, 256, 2, 64, 0, FMT>(a, pl, s);
       case 10: return run_plan<64, 128, 2, 64, 0, FMT>(a, pl, s);
       case 12: return run_plan<256, 256, 2, 128, 1, FMT>(a, pl, s);
+      case 13: return run_plan<64, 64, 2, 128, 4, FMT>(a, pl, s);
+      case 14: return run_plan<64, 64, 2, 256, 3, FMT>(a, pl, s);
+      case 15: return run_plan<128, 128, 2, 128, 3, FMT>(a, pl, s);
+      case 16: return run_plan<128, 64, 4, 128, 3, FMT>(a, pl, s);
+      case 17: return run_plan<64, 128, 2, 128, 3, FMT>(a, pl, s);
     }
   }
   return fail(GBS_EINVAL, "mfma: tile shape %d not compiled for format %d", pl.shape, FMT);
@@ -326,8 +333,8 @@ int ensure_f4(gbs_matrix *m, hipStream_t s, bool may_allocate) {
   if (m->f4_version == m->planes_version && m->d_f4) return GBS_OK;
   if (!m->d_f4) {
     const uint64_t rows = ((m->nrows + kValsRowAlign - 1) / kValsRowAlign) * kValsRowAlign;
-    const uint64_t ld = ((m->ncols + kValsColAlign - 1) / kValsColAlign) * kValsColAlign / 2;
-    const uint64_t rows_a = rows ? rows : kValsRowAlign, ld_a = ld ? ld : kValsColAlign / 2;
+    const uint64_t ld = ((m->ncols + kPackedColAlign - 1) / kPackedColAlign) * kPackedColAlign / 2;
+    const uint64_t rows_a = rows ? rows : kValsRowAlign, ld_a = ld ? ld : kPackedColAlign / 2;
     if (!may_allocate)
       return fail(GBS_ERANGE, "mfma: operand has no FP4 copy yet; call gbs_workspace_reserve");
     GBS_HIP(hipMalloc((void **)&m->d_f4, rows_a * ld_a));
@@ -347,8 +354,9 @@ int ensure_f6(gbs_matrix *m, hipStream_t s, bool may_allocate) {
   if (m->f6_version == m->planes_version && m->d_f6) return GBS_OK;
   if (!m->d_f6) {
     const uint64_t rows = ((m->nrows + kValsRowAlign - 1) / kValsRowAlign) * kValsRowAlign;
-    const uint64_t ld = ((m->ncols + kValsColAlign - 1) / kValsColAlign) * kValsColAlign / 4 * 3;
-    const uint64_t rows_a = rows ? rows : kValsRowAlign, ld_a = ld ? ld : kValsColAlign / 4 * 3;
+    const uint64_t ld =
+        ((m->ncols + kPackedColAlign - 1) / kPackedColAlign) * kPackedColAlign / 4 * 3;
+    const uint64_t rows_a = rows ? rows : kValsRowAlign, ld_a = ld ? ld : kPackedColAlign / 4 * 3;
     if (!may_allocate)
       return fail(GBS_ERANGE, "mfma: operand has no FP6 copy yet; call gbs_workspace_reserve");
     GBS_HIP(hipMalloc((void **)&m->d_f6, rows_a * ld_a));
@@ -439,12 +447,12 @@ MfmaPlan mfma_plan(uint32_t M, uint32_t N, uint32_t kbytes, int fmt, int forced_
   pl.qtiles = (M + sh.Q - 1) / sh.Q;
   // XCD groups share the larger operand: lhs larger -> rhs tiles fastest
   pl.order = (uint64_t)M > (uint64_t)N ? 1 : 0;
-  const uint32_t ksteps = kbytes / kMfmaKS;  // 128-byte units
+  const uint32_t ksteps = kbytes / kMfmaKS;  // K units
   const uint64_t tiles = (uint64_t)pl.ptiles * pl.qtiles;
   if (splits <= 0) {
-    // fill two blocks per CU, keeping at least 8 units of K per split
+    // fill two blocks per CU, keeping at least 1 KiB (nominal) of K per split
     uint64_t want = tiles >= 512 ? 1 : (512 + tiles - 1) / tiles;
-    uint64_t maxs = ksteps / 8 ? ksteps / 8 : 1;
+    uint64_t maxs = ksteps / 4 ? ksteps / 4 : 1;
     splits = (int)(want < maxs ? want : maxs);
   }
   if (splits < 1) splits = 1;

@@ -77,7 +77,7 @@ __device__ __forceinline__ MfmaAcc<FMT> mfma_op(const MfmaFrag<FMT> &a, const Mf
 __device__ __forceinline__ int32_t acc_value(int32_t x) { return x; }
 __device__ __forceinline__ int32_t acc_value(float x) { return (int32_t)x; }  // exact integer
 
-constexpr int kMfmaKS = 128;  // padding unit of K (nominal bytes): every stage size divides it
+constexpr int kMfmaKS = 256;  // padding unit of K (nominal bytes): every stage size divides it
 
 // Operands of one product: value copies (one format on both sides)
 // [rows >= tile-aligned][ld bytes].
@@ -90,16 +90,19 @@ struct MfmaOperands {
 };
 
 // Tile shapes the engine is compiled for: P x Q outputs, WP waves along P,
-// KSB bytes of K per LDS stage, DEPTH stages of global loads in flight.
-// DEPTH 0 marks the software-pipelined tile (mfma_tile_pp).
+// KSB bytes of K per LDS stage, DEPTH: 1 or 2 stages of global loads held
+// in registers ahead (mfma_tile), 0 the software-pipelined tile
+// (mfma_tile_pp), 3 or more a ring of DEPTH LDS stages filled by direct
+// loads (mfma_tile_dl).
 struct MfmaTileShape { int P, Q, WP, KSB, DEPTH; };
-constexpr int kMfmaShapes = 13;
+constexpr int kMfmaShapes = 18;
 constexpr MfmaTileShape kMfmaShape[kMfmaShapes] = {
     {128, 128, 2, 128, 2}, {64, 128, 2, 128, 2}, {32, 256, 1, 64, 2},
     {16, 256, 1, 128, 2},  {64, 64, 2, 128, 2},  {128, 64, 4, 128, 2},
     {128, 128, 2, 64, 1},  {128, 256, 2, 64, 2}, {128, 128, 2, 64, 0},
     {128, 256, 2, 64, 0},  {64, 128, 2, 64, 0},  {256, 256, 2, 64, 1},
-    {256, 256, 2, 128, 1}};
+    {256, 256, 2, 128, 1}, {64, 64, 2, 128, 4},  {64, 64, 2, 256, 3},
+    {128, 128, 2, 128, 3}, {128, 64, 4, 128, 3}, {64, 128, 2, 128, 3}};
 
 // LDS row layout of a stage.  128- and 256-byte stages use rows of exactly
 // KSB bytes with the 16-byte chunks of row r XOR-permuted by (r / (256/KSB))
@@ -121,8 +124,11 @@ template <int KSB, int FMT>
 constexpr int lds_row_stride() {
   return FMT == kFmtF6 ? KSB / 64 * 96 + 16 : lds_row_bytes<KSB>();
 }
-template <int P, int Q, int KSB, int FMT = kFmtI8>
-constexpr size_t mfma_lds_bytes() { return (size_t)2 * (P + Q) * lds_row_stride<KSB, FMT>(); }
+template <int P, int Q, int KSB, int FMT = kFmtI8, int DEPTH = 2>
+constexpr size_t mfma_lds_bytes() {
+  return DEPTH >= 3 ? (size_t)DEPTH * (P + Q) * KSB
+                    : (size_t)2 * (P + Q) * lds_row_stride<KSB, FMT>();
+}
 constexpr size_t kLdsPerCU = 160 * 1024;
 constexpr size_t kMfmaLdsMax = (size_t)2 * 272 * 128;  // persistent kernel: largest carried shape (16 x 256)
 
@@ -286,6 +292,150 @@ __device__ __forceinline__ void lds_barrier() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Wait until at most n vector-memory operations of this wave are
+// outstanding (n a compile-time constant below 64); LDS and export
+// counters are left alone.
+template <int n>
+__device__ __forceinline__ void wait_vmcnt() {
+  static_assert(n >= 0 && n < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((n & 15) | ((n >> 4) << 14) | (7 << 4) | (15 << 8));
+}
+
+// Tile whose K stages go from HBM straight into LDS with
+// global_load_lds_dwordx4 (lane i of a wave writes the 16 bytes it loads
+// at the wave's LDS base + 16 i), through a ring of NBUF stage buffers:
+// the loads of stage i + NBUF - 1 are issued while stage i is multiplied,
+// and nothing passes through registers.  Each lane picks the global chunk
+// that its LDS slot holds under the XOR swizzle (slot s of row r holds
+// logical chunk s ^ f(r)), so the fragment reads are those of mfma_tile.
+// int8 and FP4 copies, stages of 128 or 256 bytes, and tile sides whose
+// chunk counts are multiples of 256 (every wave issues the same number of
+// loads, which the vmcnt waits count on).
+template <int P, int Q, int WP, int KSB, int NBUF, int FMT = kFmtI8>
+__device__ __forceinline__ void mfma_tile_dl(const MfmaOperands &a, uint32_t pt, uint32_t qt,
+                                             uint32_t kbeg, uint32_t kend, bool atomic,
+                                             int8_t *lds) {
+  constexpr int WQ = 4 / WP;
+  constexpr int PW = P / WP, QW = Q / WQ;
+  constexpr int FP = PW / 16, FQ = QW / 16;
+  constexpr int S = KSB / 64;
+  constexpr int CPR = KSB / 16;
+  constexpr int RCH = P * CPR, LCH = Q * CPR;
+  constexpr int RLD = RCH / 256, LLD = LCH / 256;  // load instructions per wave and stage
+  constexpr int PER = RLD + LLD;
+  constexpr int D = NBUF - 1;                      // stages in flight ahead
+  static_assert(FMT != kFmtF6, "direct loads need unpadded rows");
+  static_assert(KSB == 128 || KSB == 256, "stage size");
+  static_assert(RCH % 256 == 0 && LCH % 256 == 0, "uneven loads per wave");
+  static_assert(NBUF >= 2 && PER * (NBUF - 1) < 64, "ring depth");
+  static_assert(FP >= 1 && FQ >= 1 && PW % 16 == 0 && QW % 16 == 0, "bad tile");
+  constexpr int stage = (P + Q) * KSB;
+  static_assert((size_t)NBUF * stage <= kLdsPerCU, "LDS");
+
+  const uint32_t p0 = pt * P, q0 = qt * Q;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wp = wave % WP, wq = wave / WP;
+  const int fr = lane & 15;
+
+  // loads of stage `st` into ring slot `buf`
+  auto issue = [&](uint32_t st, int buf) {
+    const uint64_t kb = (uint64_t)(kbeg + st * KSB);
+    int8_t *base = lds + buf * stage;
+#pragma unroll
+    for (int i = 0; i < RLD; i++) {
+      const int c0 = (i * 4 + wave) * 64, c = c0 + lane;  // LDS chunk slot of this lane
+      const int row = c / CPR, lc = lds_chunk<KSB>(row, c % CPR);
+      __builtin_amdgcn_global_load_lds(
+          (const void *)(a.R + (uint64_t)(p0 + row) * a.r_ld + kb + lc * 16),
+          (__attribute__((address_space(3))) void *)(base + c0 * 16), 16, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < LLD; i++) {
+      const int c0 = (i * 4 + wave) * 64, c = c0 + lane;
+      const int row = c / CPR, lc = lds_chunk<KSB>(row, c % CPR);
+      __builtin_amdgcn_global_load_lds(
+          (const void *)(a.L + (uint64_t)(q0 + row) * a.l_ld + kb + lc * 16),
+          (__attribute__((address_space(3))) void *)(base + P * KSB + c0 * 16), 16, 0, 0);
+    }
+  };
+
+  MfmaAcc<FMT> acc[FP][FQ];
+#pragma unroll
+  for (int i = 0; i < FP; i++)
+#pragma unroll
+    for (int j = 0; j < FQ; j++) acc[i][j] = MfmaAcc<FMT>{0, 0, 0, 0};
+
+  auto compute = [&](int buf) {
+    const int8_t *sR = lds + buf * stage, *sL = sR + P * KSB;
+#pragma unroll
+    for (int kk = 0; kk < S; kk++) {
+      MfmaFrag<FMT> af[FP], bf[FQ];
+      const int ck = kk * 4 + (lane >> 4);
+#pragma unroll
+      for (int i = 0; i < FP; i++) {
+        const int row = wp * PW + i * 16 + fr;
+        af[i].h = *reinterpret_cast<const v4i *>(sR + row * KSB + lds_chunk<KSB>(row, ck) * 16);
+      }
+#pragma unroll
+      for (int j = 0; j < FQ; j++) {
+        const int row = wq * QW + j * 16 + fr;
+        bf[j].h = *reinterpret_cast<const v4i *>(sL + row * KSB + lds_chunk<KSB>(row, ck) * 16);
+      }
+#pragma unroll
+      for (int i = 0; i < FP; i++)
+#pragma unroll
+        for (int j = 0; j < FQ; j++) acc[i][j] = mfma_op<FMT>(af[i], bf[j], acc[i][j]);
+    }
+  };
+
+  const uint32_t nst = kend > kbeg ? (kend - kbeg) / KSB : 0;
+#pragma unroll
+  for (int d = 0; d < D; d++)
+    if ((uint32_t)d < nst) issue(d, d);
+  for (uint32_t i = 0; i < nst; i++) {
+    // stage i has landed once at most (stages issued after it) x PER loads
+    // of this wave are outstanding
+    const uint32_t after = nst - 1 - i < (uint32_t)(D - 1) ? nst - 1 - i : (uint32_t)(D - 1);
+    if constexpr (D >= 3) {
+      if (after >= 2) wait_vmcnt<2 * PER>();
+      else if (after == 1) wait_vmcnt<PER>();
+      else wait_vmcnt<0>();
+    } else if constexpr (D == 2) {
+      if (after >= 1) wait_vmcnt<PER>();
+      else wait_vmcnt<0>();
+    } else {
+      wait_vmcnt<0>();
+    }
+    // stage i visible to every wave; every wave is done with stage i - 1,
+    // whose ring slot the next loads reuse (its LDS reads fed MFMAs that
+    // have issued, so they are complete).  The vmcnt wait above is the
+    // only one needed: no fence, which would wait for every load in flight.
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    if (i + D < nst) issue(i + D, (int)((i + D) % NBUF));
+    compute((int)(i % NBUF));
+  }
+  // the LDS is reused by the next tile of a persistent block
+  lds_barrier();
+
+#pragma unroll
+  for (int i = 0; i < FP; i++)
+#pragma unroll
+    for (int j = 0; j < FQ; j++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const uint32_t n = p0 + wp * PW + i * 16 + 4 * (lane >> 4) + r;
+        const uint32_t m = q0 + wq * QW + j * 16 + fr;
+        if (n < a.N && m < a.M) {
+          const uint64_t idx = (uint64_t)n * a.M + m;
+          const int32_t v = acc_value(acc[i][j][r]);
+          if (atomic) atomicAdd(a.res + idx, v);
+          else a.res[idx] = v;
+        }
+      }
+}
+
 // Software-pipelined tile: K in 64-byte stages.  In the iteration that
 // multiplies stage i (fragments already in registers), the block writes
 // stage i+1 from registers into the other LDS buffer, issues the global
@@ -404,22 +554,38 @@ __device__ __forceinline__ void mfma_tile_pp(const MfmaOperands &a, uint32_t pt,
       }
 }
 
-// Shape run for shape s in format fmt: FP6 has no pipelined tile (it takes
-// the staged tile of the same size) and no 256 x 256 tile with 128-byte
-// stages (its LDS would not fit).
+// Shape run for shape s in format fmt: FP6 has no pipelined or direct-load
+// tile (it takes the register-staged tile of the same size) and no
+// 256 x 256 tile with 128-byte stages (its LDS would not fit).
 __host__ __device__ constexpr int fmt_shape(int s, int fmt) {
-  return fmt != kFmtF6 ? s : s == 8 ? 6 : s == 9 ? 7 : s == 10 ? 1 : s == 12 ? 11 : s;
+  return fmt != kFmtF6 ? s
+         : s == 8      ? 6
+         : s == 9      ? 7
+         : s == 10     ? 1
+         : s == 12     ? 11
+         : s == 13 || s == 14 ? 4
+         : s == 15     ? 0
+         : s == 16     ? 5
+         : s == 17     ? 1
+                       : s;
 }
 
 // Shapes the persistent kernel carries: all but the 128 x 256 and 256 x 256
 // ones, whose registers or LDS would make the shared kernel spill or drop to
 // one block per CU; they map to the 128 x 128 shape of the same pipeline.
+// Its 64 x 64 tile is the direct-load one (four 16 KiB stages); the other
+// direct-load shapes map to their register-staged counterparts.
 // With FP4 copies the 128 x 128 tile with 128-byte stages maps to the
 // 64-byte-stage one (the FP4 code path holds more registers); with FP6
 // copies (wider LDS rows, more registers) only 32 x 256 and 64 x 64 are
 // carried.
 __host__ __device__ constexpr int batch_shape(int s, int fmt = 0) {
-  return fmt == 2   ? (s == 2 || s == 3 ? 2 : 4)
+  return fmt != 2 && (s == 4 || s == 13 || s == 14) ? 13  // 64 x 64: the direct-load ring
+         : s == 13 || s == 14 ? batch_shape(4, fmt)
+         : s == 15          ? batch_shape(0, fmt)
+         : s == 16          ? batch_shape(5, fmt)
+         : s == 17          ? batch_shape(1, fmt)
+         : fmt == 2         ? (s == 2 || s == 3 ? 2 : 4)
          : fmt == 1 && (s == 0 || s == 7 || s == 11 || s == 12) ? 6
          : s == 7 || s == 11 || s == 12                          ? 0
          : s == 9                                                  ? 8
@@ -431,7 +597,9 @@ constexpr bool batch_shapes_fit_lds() {
     for (int s = 0; s < kMfmaShapes; s++) {
       const MfmaTileShape t = kMfmaShape[batch_shape(s, f)];
       const int row = f == kFmtF6 ? t.KSB / 64 * 96 + 16 : t.KSB >= 128 ? t.KSB : t.KSB + 16;
-      if ((size_t)2 * (t.P + t.Q) * row > kMfmaLdsMax) return false;
+      const size_t bytes = t.DEPTH >= 3 ? (size_t)t.DEPTH * (t.P + t.Q) * t.KSB
+                                        : (size_t)2 * (t.P + t.Q) * row;
+      if (bytes > kMfmaLdsMax) return false;
     }
   return true;
 }
@@ -454,7 +622,7 @@ __device__ __forceinline__ void mfma_tile_by_shape(int shape, const MfmaOperands
       case 1: mfma_tile<64, 128, 2, 128, 2, FMT>(a, pt, qt, kbeg, kend, atomic, lds); break;
       case 2: mfma_tile<32, 256, 1, 64, 2, FMT>(a, pt, qt, kbeg, kend, atomic, lds); break;
       case 3: mfma_tile<16, 256, 1, 128, 2, FMT>(a, pt, qt, kbeg, kend, atomic, lds); break;
-      case 4: mfma_tile<64, 64, 2, 128, 2, FMT>(a, pt, qt, kbeg, kend, atomic, lds); break;
+      case 13: mfma_tile_dl<64, 64, 2, 128, 4, FMT>(a, pt, qt, kbeg, kend, atomic, lds); break;
       case 5: mfma_tile<128, 64, 4, 128, 2, FMT>(a, pt, qt, kbeg, kend, atomic, lds); break;
       case 6: mfma_tile<128, 128, 2, 64, 1, FMT>(a, pt, qt, kbeg, kend, atomic, lds); break;
       case 8: mfma_tile_pp<128, 128, 2, FMT>(a, pt, qt, kbeg, kend, atomic, lds); break;
