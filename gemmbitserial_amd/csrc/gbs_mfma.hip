@@ -200,12 +200,15 @@ int run_plan_fmt(const MfmaOperands &a, const MfmaPlan &pl, hipStream_t s) {
       case 9: return run_plan<128, 256, 2, 64, 0, FMT>(a, pl, s);
       case 10: return run_plan<64, 128, 2, 64, 0, FMT>(a, pl, s);
       case 12: return run_plan<256, 256, 2, 128, 1, FMT>(a, pl, s);
-      case 13: return run_plan<64, 64, 2, 128, 4, FMT>(a, pl, s);
       case 14: return run_plan<64, 64, 2, 256, 3, FMT>(a, pl, s);
-      case 15: return run_plan<128, 128, 2, 128, 3, FMT>(a, pl, s);
-      case 16: return run_plan<128, 64, 4, 128, 3, FMT>(a, pl, s);
-      case 17: return run_plan<64, 128, 2, 128, 3, FMT>(a, pl, s);
     }
+  }
+  switch (pl.shape) {  // direct-load shapes every format has
+    case 13: return run_plan<64, 64, 2, 128, 4, FMT>(a, pl, s);
+    case 15: return run_plan<128, 128, 2, 128, 3, FMT>(a, pl, s);
+    case 16: return run_plan<128, 64, 4, 128, 3, FMT>(a, pl, s);
+    case 17: return run_plan<64, 128, 2, 128, 3, FMT>(a, pl, s);
+    case 18: return run_plan<64, 64, 2, 128, 3, FMT>(a, pl, s);
   }
   return fail(GBS_EINVAL, "mfma: tile shape %d not compiled for format %d", pl.shape, FMT);
 }

@@ -95,14 +95,15 @@ struct MfmaOperands {
 // (mfma_tile_pp), 3 or more a ring of DEPTH LDS stages filled by direct
 // loads (mfma_tile_dl).
 struct MfmaTileShape { int P, Q, WP, KSB, DEPTH; };
-constexpr int kMfmaShapes = 18;
+constexpr int kMfmaShapes = 19;
 constexpr MfmaTileShape kMfmaShape[kMfmaShapes] = {
     {128, 128, 2, 128, 2}, {64, 128, 2, 128, 2}, {32, 256, 1, 64, 2},
     {16, 256, 1, 128, 2},  {64, 64, 2, 128, 2},  {128, 64, 4, 128, 2},
     {128, 128, 2, 64, 1},  {128, 256, 2, 64, 2}, {128, 128, 2, 64, 0},
     {128, 256, 2, 64, 0},  {64, 128, 2, 64, 0},  {256, 256, 2, 64, 1},
     {256, 256, 2, 128, 1}, {64, 64, 2, 128, 4},  {64, 64, 2, 256, 3},
-    {128, 128, 2, 128, 3}, {128, 64, 4, 128, 3}, {64, 128, 2, 128, 3}};
+    {128, 128, 2, 128, 3}, {128, 64, 4, 128, 3}, {64, 128, 2, 128, 3},
+    {64, 64, 2, 128, 3}};
 
 // LDS row layout of a stage.  128- and 256-byte stages use rows of exactly
 // KSB bytes with the 16-byte chunks of row r XOR-permuted by (r / (256/KSB))
@@ -126,11 +127,12 @@ constexpr int lds_row_stride() {
 }
 template <int P, int Q, int KSB, int FMT = kFmtI8, int DEPTH = 2>
 constexpr size_t mfma_lds_bytes() {
-  return DEPTH >= 3 ? (size_t)DEPTH * (P + Q) * KSB
+  return DEPTH >= 3 ? (size_t)DEPTH * (P + Q) * (KSB / 64 * step_bytes<FMT>())
                     : (size_t)2 * (P + Q) * lds_row_stride<KSB, FMT>();
 }
 constexpr size_t kLdsPerCU = 160 * 1024;
-constexpr size_t kMfmaLdsMax = (size_t)2 * 272 * 128;  // persistent kernel: largest carried shape (16 x 256)
+// persistent kernel: the largest carried shape (64 x 64, three FP6 stages)
+constexpr size_t kMfmaLdsMax = (size_t)3 * 128 * 192;
 
 template <int P, int Q, int WP, int KSB, int DEPTH, int FMT = kFmtI8>
 __device__ __forceinline__ void mfma_tile(const MfmaOperands &a, uint32_t pt, uint32_t qt,
@@ -305,12 +307,30 @@ __device__ __forceinline__ void wait_vmcnt() {
 // global_load_lds_dwordx4 (lane i of a wave writes the 16 bytes it loads
 // at the wave's LDS base + 16 i), through a ring of NBUF stage buffers:
 // the loads of stage i + NBUF - 1 are issued while stage i is multiplied,
-// and nothing passes through registers.  Each lane picks the global chunk
-// that its LDS slot holds under the XOR swizzle (slot s of row r holds
-// logical chunk s ^ f(r)), so the fragment reads are those of mfma_tile.
-// int8 and FP4 copies, stages of 128 or 256 bytes, and tile sides whose
+// and nothing passes through registers.  Rows are unpadded; each lane
+// picks the global chunk that its LDS slot holds under a per-row chunk
+// permutation that keeps fragment reads free of bank conflicts:
+//   int8 / FP4 (rows of KSB bytes): slot s of row r holds logical chunk
+//     s ^ f(r), the swizzle of mfma_tile;
+//   FP6 (rows of 12 chunks per 128 nominal bytes): slot s of row r holds
+//     logical chunk (s - (r / 4) mod 4) mod CPR -- rows 4 apart share a
+//     bank offset (192-byte stride) and the rotation moves them onto
+//     distinct 16-byte bank groups.
+// Stages of 128 or 256 nominal bytes (FP6: 128), and tile sides whose
 // chunk counts are multiples of 256 (every wave issues the same number of
 // loads, which the vmcnt waits count on).
+template <int KSB, int FMT>
+__device__ __forceinline__ int dl_slot(int row, int chunk) {  // LDS slot of a logical chunk
+  constexpr int CPR = KSB / 64 * step_bytes<FMT>() / 16;
+  if constexpr (FMT == kFmtF6) return (chunk + ((row >> 2) & 3)) % CPR;
+  else return lds_chunk<KSB>(row, chunk);
+}
+template <int KSB, int FMT>
+__device__ __forceinline__ int dl_chunk(int row, int slot) {  // logical chunk of a slot
+  constexpr int CPR = KSB / 64 * step_bytes<FMT>() / 16;
+  if constexpr (FMT == kFmtF6) return (slot + CPR - ((row >> 2) & 3)) % CPR;
+  else return lds_chunk<KSB>(row, slot);
+}
 template <int P, int Q, int WP, int KSB, int NBUF, int FMT = kFmtI8>
 __device__ __forceinline__ void mfma_tile_dl(const MfmaOperands &a, uint32_t pt, uint32_t qt,
                                              uint32_t kbeg, uint32_t kend, bool atomic,
@@ -319,17 +339,17 @@ __device__ __forceinline__ void mfma_tile_dl(const MfmaOperands &a, uint32_t pt,
   constexpr int PW = P / WP, QW = Q / WQ;
   constexpr int FP = PW / 16, FQ = QW / 16;
   constexpr int S = KSB / 64;
-  constexpr int CPR = KSB / 16;
+  constexpr int SB = S * step_bytes<FMT>();        // real bytes per row and stage
+  constexpr int CPR = SB / 16;
   constexpr int RCH = P * CPR, LCH = Q * CPR;
   constexpr int RLD = RCH / 256, LLD = LCH / 256;  // load instructions per wave and stage
   constexpr int PER = RLD + LLD;
   constexpr int D = NBUF - 1;                      // stages in flight ahead
-  static_assert(FMT != kFmtF6, "direct loads need unpadded rows");
-  static_assert(KSB == 128 || KSB == 256, "stage size");
+  static_assert(FMT == kFmtF6 ? KSB == 128 : (KSB == 128 || KSB == 256), "stage size");
   static_assert(RCH % 256 == 0 && LCH % 256 == 0, "uneven loads per wave");
   static_assert(NBUF >= 2 && PER * (NBUF - 1) < 64, "ring depth");
   static_assert(FP >= 1 && FQ >= 1 && PW % 16 == 0 && QW % 16 == 0, "bad tile");
-  constexpr int stage = (P + Q) * KSB;
+  constexpr int stage = (P + Q) * SB;
   static_assert((size_t)NBUF * stage <= kLdsPerCU, "LDS");
 
   const uint32_t p0 = pt * P, q0 = qt * Q;
@@ -339,12 +359,12 @@ __device__ __forceinline__ void mfma_tile_dl(const MfmaOperands &a, uint32_t pt,
 
   // loads of stage `st` into ring slot `buf`
   auto issue = [&](uint32_t st, int buf) {
-    const uint64_t kb = (uint64_t)(kbeg + st * KSB);
+    const uint64_t kb = (uint64_t)(kbeg + st * KSB) / 64 * step_bytes<FMT>();  // real bytes
     int8_t *base = lds + buf * stage;
 #pragma unroll
     for (int i = 0; i < RLD; i++) {
       const int c0 = (i * 4 + wave) * 64, c = c0 + lane;  // LDS chunk slot of this lane
-      const int row = c / CPR, lc = lds_chunk<KSB>(row, c % CPR);
+      const int row = c / CPR, lc = dl_chunk<KSB, FMT>(row, c % CPR);
       __builtin_amdgcn_global_load_lds(
           (const void *)(a.R + (uint64_t)(p0 + row) * a.r_ld + kb + lc * 16),
           (__attribute__((address_space(3))) void *)(base + c0 * 16), 16, 0, 0);
@@ -352,11 +372,26 @@ __device__ __forceinline__ void mfma_tile_dl(const MfmaOperands &a, uint32_t pt,
 #pragma unroll
     for (int i = 0; i < LLD; i++) {
       const int c0 = (i * 4 + wave) * 64, c = c0 + lane;
-      const int row = c / CPR, lc = lds_chunk<KSB>(row, c % CPR);
+      const int row = c / CPR, lc = dl_chunk<KSB, FMT>(row, c % CPR);
       __builtin_amdgcn_global_load_lds(
           (const void *)(a.L + (uint64_t)(q0 + row) * a.l_ld + kb + lc * 16),
-          (__attribute__((address_space(3))) void *)(base + P * KSB + c0 * 16), 16, 0, 0);
+          (__attribute__((address_space(3))) void *)(base + P * SB + c0 * 16), 16, 0, 0);
     }
+  };
+  // lane's fragment of LDS row `row`, k-step kk of the stage
+  auto frag = [&](const int8_t *base, int row, int kk) {
+    MfmaFrag<FMT> f;
+    const int g = lane >> 4;
+    const int8_t *p = base + row * SB;
+    if constexpr (FMT == kFmtF6) {
+      f.x.s0123 = *reinterpret_cast<const v4i *>(p + dl_slot<KSB, FMT>(row, kk * 6 + g) * 16);
+      f.x.s45 = *reinterpret_cast<const v2i *>(
+          p + dl_slot<KSB, FMT>(row, kk * 6 + 4 + (g >> 1)) * 16 + (g & 1) * 8);
+      f.x.s67 = v2i{0, 0};
+    } else {
+      f.h = *reinterpret_cast<const v4i *>(p + dl_slot<KSB, FMT>(row, kk * 4 + g) * 16);
+    }
+    return f;
   };
 
   MfmaAcc<FMT> acc[FP][FQ];
@@ -366,21 +401,14 @@ __device__ __forceinline__ void mfma_tile_dl(const MfmaOperands &a, uint32_t pt,
     for (int j = 0; j < FQ; j++) acc[i][j] = MfmaAcc<FMT>{0, 0, 0, 0};
 
   auto compute = [&](int buf) {
-    const int8_t *sR = lds + buf * stage, *sL = sR + P * KSB;
+    const int8_t *sR = lds + buf * stage, *sL = sR + P * SB;
 #pragma unroll
     for (int kk = 0; kk < S; kk++) {
       MfmaFrag<FMT> af[FP], bf[FQ];
-      const int ck = kk * 4 + (lane >> 4);
 #pragma unroll
-      for (int i = 0; i < FP; i++) {
-        const int row = wp * PW + i * 16 + fr;
-        af[i].h = *reinterpret_cast<const v4i *>(sR + row * KSB + lds_chunk<KSB>(row, ck) * 16);
-      }
+      for (int i = 0; i < FP; i++) af[i] = frag(sR, wp * PW + i * 16 + fr, kk);
 #pragma unroll
-      for (int j = 0; j < FQ; j++) {
-        const int row = wq * QW + j * 16 + fr;
-        bf[j].h = *reinterpret_cast<const v4i *>(sL + row * KSB + lds_chunk<KSB>(row, ck) * 16);
-      }
+      for (int j = 0; j < FQ; j++) bf[j] = frag(sL, wq * QW + j * 16 + fr, kk);
 #pragma unroll
       for (int i = 0; i < FP; i++)
 #pragma unroll
@@ -554,38 +582,34 @@ __device__ __forceinline__ void mfma_tile_pp(const MfmaOperands &a, uint32_t pt,
       }
 }
 
-// Shape run for shape s in format fmt: FP6 has no pipelined or direct-load
-// tile (it takes the register-staged tile of the same size) and no
-// 256 x 256 tile with 128-byte stages (its LDS would not fit).
+// Shape run for shape s in format fmt: FP6 has no pipelined tile (it takes
+// the staged tile of the same size), no 256 x 256 tile with 128-byte
+// stages (its LDS would not fit) and no 256-byte direct-load stages.
 __host__ __device__ constexpr int fmt_shape(int s, int fmt) {
   return fmt != kFmtF6 ? s
          : s == 8      ? 6
          : s == 9      ? 7
          : s == 10     ? 1
          : s == 12     ? 11
-         : s == 13 || s == 14 ? 4
-         : s == 15     ? 0
-         : s == 16     ? 5
-         : s == 17     ? 1
+         : s == 14     ? 18
                        : s;
 }
 
 // Shapes the persistent kernel carries: all but the 128 x 256 and 256 x 256
 // ones, whose registers or LDS would make the shared kernel spill or drop to
 // one block per CU; they map to the 128 x 128 shape of the same pipeline.
-// Its 64 x 64 tile is the direct-load one (four 16 KiB stages); the other
-// direct-load shapes map to their register-staged counterparts.
+// Its 64 x 64 tile is the direct-load one (four stages; three for FP6);
+// the other direct-load shapes map to their register-staged counterparts.
 // With FP4 copies the 128 x 128 tile with 128-byte stages maps to the
 // 64-byte-stage one (the FP4 code path holds more registers); with FP6
 // copies (wider LDS rows, more registers) only 32 x 256 and 64 x 64 are
 // carried.
 __host__ __device__ constexpr int batch_shape(int s, int fmt = 0) {
-  return fmt != 2 && (s == 4 || s == 13 || s == 14) ? 13  // 64 x 64: the direct-load ring
-         : s == 13 || s == 14 ? batch_shape(4, fmt)
+  return s == 4 || s == 13 || s == 14 || s == 18 ? (fmt == 2 ? 18 : 13)  // 64 x 64: direct loads
          : s == 15          ? batch_shape(0, fmt)
          : s == 16          ? batch_shape(5, fmt)
          : s == 17          ? batch_shape(1, fmt)
-         : fmt == 2         ? (s == 2 || s == 3 ? 2 : 4)
+         : fmt == 2         ? (s == 2 || s == 3 ? 2 : 18)
          : fmt == 1 && (s == 0 || s == 7 || s == 11 || s == 12) ? 6
          : s == 7 || s == 11 || s == 12                          ? 0
          : s == 9                                                  ? 8
@@ -597,8 +621,9 @@ constexpr bool batch_shapes_fit_lds() {
     for (int s = 0; s < kMfmaShapes; s++) {
       const MfmaTileShape t = kMfmaShape[batch_shape(s, f)];
       const int row = f == kFmtF6 ? t.KSB / 64 * 96 + 16 : t.KSB >= 128 ? t.KSB : t.KSB + 16;
-      const size_t bytes = t.DEPTH >= 3 ? (size_t)t.DEPTH * (t.P + t.Q) * t.KSB
-                                        : (size_t)2 * (t.P + t.Q) * row;
+      const size_t bytes =
+          t.DEPTH >= 3 ? (size_t)t.DEPTH * (t.P + t.Q) * (t.KSB / 64 * (f == kFmtF6 ? 96 : 64))
+                       : (size_t)2 * (t.P + t.Q) * row;
       if (bytes > kMfmaLdsMax) return false;
     }
   return true;
@@ -612,7 +637,7 @@ __device__ __forceinline__ void mfma_tile_by_shape(int shape, const MfmaOperands
                                                    uint32_t kend, bool atomic, int8_t *lds) {
   if constexpr (FMT == kFmtF6) {
     if (shape == 2) mfma_tile<32, 256, 1, 64, 2, FMT>(a, pt, qt, kbeg, kend, atomic, lds);
-    else mfma_tile<64, 64, 2, 128, 2, FMT>(a, pt, qt, kbeg, kend, atomic, lds);
+    else mfma_tile_dl<64, 64, 2, 128, 3, FMT>(a, pt, qt, kbeg, kend, atomic, lds);
   } else {
     switch (shape) {
       case 0:
