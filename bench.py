@@ -43,6 +43,9 @@ VALU_PEAK = 256 * 64 * 2.4e9      # int32 lane-ops/s: 256 CUs x 64 lanes x 2.4 G
 HBM_PEAK = 8.0e12                 # B/s
 MFMA_I8_PEAK = 5.0e15             # dense int8 ops/s
 MFMA_F4_PEAK = 10.1e15            # dense FP4 / FP6 ops/s
+# HBM bytes per launch of the headline kernel, measured with rocprofv3 --pmc
+# (FETCH_SIZE, WRITE_SIZE, scaled by a copy of known size)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "mfma_fp4_binary_layers_traffic.txt")
 
 
 def log(*a):
@@ -101,6 +104,24 @@ def plane_bytes(M, K, N, lb, rb):
 
 def int8_bytes(M, K, N):
     return (M + N) * K + 4 * M * N
+
+
+# arithmetic each engine computes in: popcount on 32-bit plane words, int8
+# products, or FP4 / FP6 codes of the integer values with fp32 accumulation
+ENGINE_DTYPE = {"popcount": "u32", "mfma": "i8", "mfma_fp4": "fp4", "mfma_fp6": "fp6"}
+
+
+def measured_traffic(engine):
+    """Bytes per launch from the committed counter summary, for the engine
+    it was measured on (the FP4 matrix-core kernel), else None."""
+    if engine != "mfma_fp4" or not os.path.exists(TRAFFIC_FILE):
+        return None
+    kv = {}
+    for ln in open(TRAFFIC_FILE):
+        if "=" in ln and not ln.lstrip().startswith("#"):
+            k, v = ln.split("=", 1)
+            kv[k.strip()] = v.strip()
+    return int(kv["traffic_bytes_per_launch"]) if "traffic_bytes_per_launch" in kv else None
 
 
 def roofline_of(algo, M, K, N, lb, rb, sec):
@@ -357,7 +378,7 @@ def main():
     eng = c1[0]["auto"]
     r0 = c1[0]
     roof = roofline_of(eng, r0["M"], r0["K"], r0["N"], 1, 1, step_ms * 1e-3 / len(c1))
-    roof["traffic"] = None
+    roof["traffic"] = measured_traffic(eng)
     cpu = None
     if cpu_secs > 0 and all("cpu_reference" in r for r in c1):
         cpu_ms = sum(r["cpu_reference"]["ms_per_call"] for r in c1)
@@ -369,7 +390,7 @@ def main():
     line = {"metric": METRIC, "value": round(value, 2), "unit": "TOPS",
             "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": round(step_ms, 5), "higher_is_better": True,
-            "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+            "scaling": "weak", "vs_baseline": None, "dtype": ENGINE_DTYPE[eng],
             "data": "synthetic (seeded numpy), packed to bit-planes in HBM before timing",
             "config": {"workload": CONFIGS["c1"][0], "engine": eng,
                        "configs": results},
