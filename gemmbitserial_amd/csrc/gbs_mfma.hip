@@ -34,11 +34,11 @@
 namespace gbs {
 namespace {
 
-template <int P, int Q, int WP, int KSB, int DEPTH, int FMT>
-__global__ __launch_bounds__(256, (P * Q >= 65536 || (P * Q >= 32768 && DEPTH == 0)) ? 1 : 2)
-void mfma_kernel(
-    MfmaOperands a, MfmaPlan pl) {
-  __shared__ __attribute__((aligned(16))) int8_t lds[mfma_lds_bytes<P, Q, KSB, FMT, DEPTH>()];
+template <int P, int Q, int WP, int KSB, int DEPTH, int FMT, int KG>
+__global__ __launch_bounds__(256 * KG,
+                             (P * Q >= 65536 || (P * Q >= 32768 && DEPTH == 0)) ? 1 : 2)
+void mfma_kernel(MfmaOperands a, MfmaPlan pl) {
+  __shared__ __attribute__((aligned(16))) int8_t lds[mfma_lds_bytes<P, Q, KSB, FMT, DEPTH, KG>()];
   const uint32_t T = gridDim.x, b = blockIdx.x;
   // consecutive logical tiles run on one XCD (blocks are dealt round-robin)
   const uint64_t t = (T % 8 == 0) ? (uint64_t)(b % 8) * (T / 8) + b / 8 : b;
@@ -46,8 +46,8 @@ void mfma_kernel(
   plan_decode(pl, t, pt, qt, sp);
   const uint32_t kbeg = sp * pl.kchunk;
   const uint32_t kend = min(a.kbytes, kbeg + pl.kchunk);
-  if constexpr (DEPTH >= 3)
-    mfma_tile_dl<P, Q, WP, KSB, DEPTH, FMT>(a, pt, qt, kbeg, kend, pl.splits > 1, lds);
+  if constexpr (direct_loads(DEPTH, KG))
+    mfma_tile_dl<P, Q, WP, KSB, DEPTH, FMT, KG>(a, pt, qt, kbeg, kend, pl.splits > 1, lds);
   else if constexpr (DEPTH == 0)
     mfma_tile_pp<P, Q, WP, FMT>(a, pt, qt, kbeg, kend, pl.splits > 1, lds);
   else
@@ -166,7 +166,7 @@ uint64_t max_abs(const Operand &o) {
   return (1ull << o.nbits) - 1;
 }
 
-template <int P, int Q, int WP, int KSB, int DEPTH, int FMT>
+template <int P, int Q, int WP, int KSB, int DEPTH, int FMT, int KG = 1>
 int run_plan(const MfmaOperands &a, const MfmaPlan &pl, hipStream_t s) {
   if (pl.splits > 1) {
     int rc = launch_fill_i32(a.res, (uint64_t)a.M * a.N, 0, s);
@@ -175,8 +175,8 @@ int run_plan(const MfmaOperands &a, const MfmaPlan &pl, hipStream_t s) {
   const uint64_t grid = pl.tiles();
   if (grid == 0) return GBS_OK;
   if (grid > 0x7fffffffull) return fail(GBS_ERANGE, "mfma grid too large");
-  hipLaunchKernelGGL((mfma_kernel<P, Q, WP, KSB, DEPTH, FMT>), dim3((uint32_t)grid), dim3(256),
-                     0, s, a, pl);
+  hipLaunchKernelGGL((mfma_kernel<P, Q, WP, KSB, DEPTH, FMT, KG>), dim3((uint32_t)grid),
+                     dim3(256 * KG), 0, s, a, pl);
   GBS_HIP(hipGetLastError());
   return GBS_OK;
 }
@@ -209,6 +209,9 @@ int run_plan_fmt(const MfmaOperands &a, const MfmaPlan &pl, hipStream_t s) {
     case 16: return run_plan<128, 64, 4, 128, 3, FMT>(a, pl, s);
     case 17: return run_plan<64, 128, 2, 128, 3, FMT>(a, pl, s);
     case 18: return run_plan<64, 64, 2, 128, 3, FMT>(a, pl, s);
+    case 19: return run_plan<64, 64, 2, 128, 2, FMT, 2>(a, pl, s);
+    case 20: return run_plan<64, 128, 2, 128, 2, FMT, 2>(a, pl, s);
+    case 21: return run_plan<64, 64, 2, 128, 3, FMT, 2>(a, pl, s);
   }
   return fail(GBS_EINVAL, "mfma: tile shape %d not compiled for format %d", pl.shape, FMT);
 }

@@ -90,20 +90,24 @@ struct MfmaOperands {
 };
 
 // Tile shapes the engine is compiled for: P x Q outputs, WP waves along P,
-// KSB bytes of K per LDS stage, DEPTH: 1 or 2 stages of global loads held
-// in registers ahead (mfma_tile), 0 the software-pipelined tile
-// (mfma_tile_pp), 3 or more a ring of DEPTH LDS stages filled by direct
-// loads (mfma_tile_dl).
-struct MfmaTileShape { int P, Q, WP, KSB, DEPTH; };
-constexpr int kMfmaShapes = 19;
+// KSB bytes of K per LDS stage, KG groups of 4 waves per block (each group
+// multiplies every KG-th stage; partial sums are added in LDS at the end),
+// DEPTH: with KG = 1, 1 or 2 stages of global loads held in registers
+// ahead (mfma_tile), 0 the software-pipelined tile (mfma_tile_pp), 3 or
+// more a ring of DEPTH LDS slots filled by direct loads (mfma_tile_dl);
+// with KG > 1 always direct loads into a ring of DEPTH slots of KG stages.
+struct MfmaTileShape { int P, Q, WP, KSB, DEPTH, KG; };
+constexpr int kMfmaShapes = 22;
 constexpr MfmaTileShape kMfmaShape[kMfmaShapes] = {
-    {128, 128, 2, 128, 2}, {64, 128, 2, 128, 2}, {32, 256, 1, 64, 2},
-    {16, 256, 1, 128, 2},  {64, 64, 2, 128, 2},  {128, 64, 4, 128, 2},
-    {128, 128, 2, 64, 1},  {128, 256, 2, 64, 2}, {128, 128, 2, 64, 0},
-    {128, 256, 2, 64, 0},  {64, 128, 2, 64, 0},  {256, 256, 2, 64, 1},
-    {256, 256, 2, 128, 1}, {64, 64, 2, 128, 4},  {64, 64, 2, 256, 3},
-    {128, 128, 2, 128, 3}, {128, 64, 4, 128, 3}, {64, 128, 2, 128, 3},
-    {64, 64, 2, 128, 3}};
+    {128, 128, 2, 128, 2, 1}, {64, 128, 2, 128, 2, 1}, {32, 256, 1, 64, 2, 1},
+    {16, 256, 1, 128, 2, 1},  {64, 64, 2, 128, 2, 1},  {128, 64, 4, 128, 2, 1},
+    {128, 128, 2, 64, 1, 1},  {128, 256, 2, 64, 2, 1}, {128, 128, 2, 64, 0, 1},
+    {128, 256, 2, 64, 0, 1},  {64, 128, 2, 64, 0, 1},  {256, 256, 2, 64, 1, 1},
+    {256, 256, 2, 128, 1, 1}, {64, 64, 2, 128, 4, 1},  {64, 64, 2, 256, 3, 1},
+    {128, 128, 2, 128, 3, 1}, {128, 64, 4, 128, 3, 1}, {64, 128, 2, 128, 3, 1},
+    {64, 64, 2, 128, 3, 1},   {64, 64, 2, 128, 2, 2},  {64, 128, 2, 128, 2, 2},
+    {64, 64, 2, 128, 3, 2}};
+__host__ __device__ constexpr bool direct_loads(int DEPTH, int KG) { return DEPTH >= 3 || KG > 1; }
 
 // LDS row layout of a stage.  128- and 256-byte stages use rows of exactly
 // KSB bytes with the 16-byte chunks of row r XOR-permuted by (r / (256/KSB))
@@ -125,10 +129,10 @@ template <int KSB, int FMT>
 constexpr int lds_row_stride() {
   return FMT == kFmtF6 ? KSB / 64 * 96 + 16 : lds_row_bytes<KSB>();
 }
-template <int P, int Q, int KSB, int FMT = kFmtI8, int DEPTH = 2>
+template <int P, int Q, int KSB, int FMT = kFmtI8, int DEPTH = 2, int KG = 1>
 constexpr size_t mfma_lds_bytes() {
-  return DEPTH >= 3 ? (size_t)DEPTH * (P + Q) * (KSB / 64 * step_bytes<FMT>())
-                    : (size_t)2 * (P + Q) * lds_row_stride<KSB, FMT>();
+  return direct_loads(DEPTH, KG) ? (size_t)DEPTH * KG * (P + Q) * (KSB / 64 * step_bytes<FMT>())
+                                 : (size_t)2 * (P + Q) * lds_row_stride<KSB, FMT>();
 }
 constexpr size_t kLdsPerCU = 160 * 1024;
 // persistent kernel: the largest carried shape (64 x 64, three FP6 stages)
@@ -318,7 +322,9 @@ __device__ __forceinline__ void wait_vmcnt() {
 //     distinct 16-byte bank groups.
 // Stages of 128 or 256 nominal bytes (FP6: 128), and tile sides whose
 // chunk counts are multiples of 256 (every wave issues the same number of
-// loads, which the vmcnt waits count on).
+// loads, which the vmcnt waits count on).  With KG wave groups a ring slot
+// holds KG consecutive stages and group g multiplies the slot's stage g;
+// the groups' partial sums meet in LDS after the last stage.
 template <int KSB, int FMT>
 __device__ __forceinline__ int dl_slot(int row, int chunk) {  // LDS slot of a logical chunk
   constexpr int CPR = KSB / 64 * step_bytes<FMT>() / 16;
@@ -331,10 +337,11 @@ __device__ __forceinline__ int dl_chunk(int row, int slot) {  // logical chunk o
   if constexpr (FMT == kFmtF6) return (slot + CPR - ((row >> 2) & 3)) % CPR;
   else return lds_chunk<KSB>(row, slot);
 }
-template <int P, int Q, int WP, int KSB, int NBUF, int FMT = kFmtI8>
+template <int P, int Q, int WP, int KSB, int NBUF, int FMT = kFmtI8, int KG = 1>
 __device__ __forceinline__ void mfma_tile_dl(const MfmaOperands &a, uint32_t pt, uint32_t qt,
                                              uint32_t kbeg, uint32_t kend, bool atomic,
                                              int8_t *lds) {
+  constexpr int NW = 4 * KG;                       // waves per block
   constexpr int WQ = 4 / WP;
   constexpr int PW = P / WP, QW = Q / WQ;
   constexpr int FP = PW / 16, FQ = QW / 16;
@@ -344,38 +351,53 @@ __device__ __forceinline__ void mfma_tile_dl(const MfmaOperands &a, uint32_t pt,
   constexpr int RCH = P * CPR, LCH = Q * CPR;
   constexpr int RLD = RCH / 256, LLD = LCH / 256;  // load instructions per wave and stage
   constexpr int PER = RLD + LLD;
-  constexpr int D = NBUF - 1;                      // stages in flight ahead
+  constexpr int D = NBUF - 1;                      // ring slots in flight ahead
   static_assert(FMT == kFmtF6 ? KSB == 128 : (KSB == 128 || KSB == 256), "stage size");
   static_assert(RCH % 256 == 0 && LCH % 256 == 0, "uneven loads per wave");
   static_assert(NBUF >= 2 && PER * (NBUF - 1) < 64, "ring depth");
   static_assert(FP >= 1 && FQ >= 1 && PW % 16 == 0 && QW % 16 == 0, "bad tile");
+  static_assert(KG == 1 || KG == 2, "wave groups");
   constexpr int stage = (P + Q) * SB;
-  static_assert((size_t)NBUF * stage <= kLdsPerCU, "LDS");
+  constexpr int slot = KG * stage;                 // one ring slot: KG stages
+  static_assert((size_t)NBUF * slot <= kLdsPerCU, "LDS");
 
   const uint32_t p0 = pt * P, q0 = qt * Q;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wp = wave % WP, wq = wave / WP;
+  const int grp = wave >> 2, wl = wave & 3;        // wave group, wave within the group
+  const int wp = wl % WP, wq = wl / WP;
   const int fr = lane & 15;
+  const uint32_t nst = kend > kbeg ? (kend - kbeg) / KSB : 0;  // stages
+  const uint32_t nsl = (nst + KG - 1) / KG;                    // ring slots of work
 
-  // loads of stage `st` into ring slot `buf`
-  auto issue = [&](uint32_t st, int buf) {
-    const uint64_t kb = (uint64_t)(kbeg + st * KSB) / 64 * step_bytes<FMT>();  // real bytes
-    int8_t *base = lds + buf * stage;
+  // loads of ring slot `sl` (stages sl*KG ..) into ring buffer `buf`.  Every
+  // wave issues RLD + LLD loads; one for a stage past the end re-reads this
+  // product's first stage (in bounds) into a part of LDS nobody reads.
+  auto issue = [&](uint32_t sl, int buf) {
+    int8_t *base = lds + buf * slot;
 #pragma unroll
     for (int i = 0; i < RLD; i++) {
-      const int c0 = (i * 4 + wave) * 64, c = c0 + lane;  // LDS chunk slot of this lane
-      const int row = c / CPR, lc = dl_chunk<KSB, FMT>(row, c % CPR);
+      const int c0 = (i * NW + wave) * 64;           // first chunk of this wave's load
+      const int g = c0 / RCH, cs = c0 % RCH + lane;  // stage within the slot, chunk slot
+      const uint32_t st = sl * KG + g;
+      const uint64_t kb = (uint64_t)(kbeg + (st < nst ? st : 0) * KSB) / 64 * step_bytes<FMT>();
+      const int row = cs / CPR, lc = dl_chunk<KSB, FMT>(row, cs % CPR);
       __builtin_amdgcn_global_load_lds(
           (const void *)(a.R + (uint64_t)(p0 + row) * a.r_ld + kb + lc * 16),
-          (__attribute__((address_space(3))) void *)(base + c0 * 16), 16, 0, 0);
+          (__attribute__((address_space(3))) void *)(base + g * stage + (c0 % RCH) * 16), 16,
+          0, 0);
     }
 #pragma unroll
     for (int i = 0; i < LLD; i++) {
-      const int c0 = (i * 4 + wave) * 64, c = c0 + lane;
-      const int row = c / CPR, lc = dl_chunk<KSB, FMT>(row, c % CPR);
+      const int c0 = (i * NW + wave) * 64;
+      const int g = c0 / LCH, cs = c0 % LCH + lane;
+      const uint32_t st = sl * KG + g;
+      const uint64_t kb = (uint64_t)(kbeg + (st < nst ? st : 0) * KSB) / 64 * step_bytes<FMT>();
+      const int row = cs / CPR, lc = dl_chunk<KSB, FMT>(row, cs % CPR);
       __builtin_amdgcn_global_load_lds(
           (const void *)(a.L + (uint64_t)(q0 + row) * a.l_ld + kb + lc * 16),
-          (__attribute__((address_space(3))) void *)(base + P * SB + c0 * 16), 16, 0, 0);
+          (__attribute__((address_space(3))) void *)(base + g * stage + P * SB +
+                                                      (c0 % LCH) * 16),
+          16, 0, 0);
     }
   };
   // lane's fragment of LDS row `row`, k-step kk of the stage
@@ -401,7 +423,7 @@ __device__ __forceinline__ void mfma_tile_dl(const MfmaOperands &a, uint32_t pt,
     for (int j = 0; j < FQ; j++) acc[i][j] = MfmaAcc<FMT>{0, 0, 0, 0};
 
   auto compute = [&](int buf) {
-    const int8_t *sR = lds + buf * stage, *sL = sR + P * SB;
+    const int8_t *sR = lds + buf * slot + grp * stage, *sL = sR + P * SB;
 #pragma unroll
     for (int kk = 0; kk < S; kk++) {
       MfmaFrag<FMT> af[FP], bf[FQ];
@@ -416,14 +438,13 @@ __device__ __forceinline__ void mfma_tile_dl(const MfmaOperands &a, uint32_t pt,
     }
   };
 
-  const uint32_t nst = kend > kbeg ? (kend - kbeg) / KSB : 0;
 #pragma unroll
   for (int d = 0; d < D; d++)
-    if ((uint32_t)d < nst) issue(d, d);
-  for (uint32_t i = 0; i < nst; i++) {
-    // stage i has landed once at most (stages issued after it) x PER loads
-    // of this wave are outstanding
-    const uint32_t after = nst - 1 - i < (uint32_t)(D - 1) ? nst - 1 - i : (uint32_t)(D - 1);
+    if ((uint32_t)d < nsl) issue(d, d);
+  for (uint32_t i = 0; i < nsl; i++) {
+    // slot i has landed once at most (slots issued after it) x PER loads of
+    // this wave are outstanding
+    const uint32_t after = nsl - 1 - i < (uint32_t)(D - 1) ? nsl - 1 - i : (uint32_t)(D - 1);
     if constexpr (D >= 3) {
       if (after >= 2) wait_vmcnt<2 * PER>();
       else if (after == 1) wait_vmcnt<PER>();
@@ -441,11 +462,36 @@ __device__ __forceinline__ void mfma_tile_dl(const MfmaOperands &a, uint32_t pt,
     asm volatile("" ::: "memory");
     __builtin_amdgcn_s_barrier();
     asm volatile("" ::: "memory");
-    if (i + D < nst) issue(i + D, (int)((i + D) % NBUF));
-    compute((int)(i % NBUF));
+    if (i + D < nsl) issue(i + D, (int)((i + D) % NBUF));
+    if (KG == 1 || i * KG + grp < nst) compute((int)(i % NBUF));
   }
-  // the LDS is reused by the next tile of a persistent block
+  // the LDS is reused by the next tile of a persistent block, or by the
+  // groups' reduction
   lds_barrier();
+  if constexpr (KG > 1) {
+    // group 1 parks its sums in LDS (lane-contiguous: no bank conflicts),
+    // group 0 adds them and writes the tile
+    typedef typename std::conditional<FMT == kFmtI8, int32_t, float>::type E;
+    static_assert((size_t)FP * FQ * 4 * 256 * sizeof(E) <= (size_t)NBUF * slot, "reduction");
+    E *red = reinterpret_cast<E *>(lds);
+    const int at = wl * 64 + lane;
+    if (grp == 1) {
+#pragma unroll
+      for (int i = 0; i < FP; i++)
+#pragma unroll
+        for (int j = 0; j < FQ; j++)
+#pragma unroll
+          for (int r = 0; r < 4; r++) red[((i * FQ + j) * 4 + r) * 256 + at] = acc[i][j][r];
+    }
+    lds_barrier();
+    if (grp == 1) return;
+#pragma unroll
+    for (int i = 0; i < FP; i++)
+#pragma unroll
+      for (int j = 0; j < FQ; j++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) acc[i][j][r] += red[((i * FQ + j) * 4 + r) * 256 + at];
+  }
 
 #pragma unroll
   for (int i = 0; i < FP; i++)
@@ -605,7 +651,9 @@ __host__ __device__ constexpr int fmt_shape(int s, int fmt) {
 // copies (wider LDS rows, more registers) only 32 x 256 and 64 x 64 are
 // carried.
 __host__ __device__ constexpr int batch_shape(int s, int fmt = 0) {
-  return s == 4 || s == 13 || s == 14 || s == 18 ? (fmt == 2 ? 18 : 13)  // 64 x 64: direct loads
+  return s == 4 || s == 13 || s == 14 || s == 18 || s == 19 || s == 21
+             ? (fmt == 2 ? 18 : 13)  // 64 x 64: direct loads, one wave group
+         : s == 20          ? batch_shape(1, fmt)
          : s == 15          ? batch_shape(0, fmt)
          : s == 16          ? batch_shape(5, fmt)
          : s == 17          ? batch_shape(1, fmt)
@@ -621,9 +669,11 @@ constexpr bool batch_shapes_fit_lds() {
     for (int s = 0; s < kMfmaShapes; s++) {
       const MfmaTileShape t = kMfmaShape[batch_shape(s, f)];
       const int row = f == kFmtF6 ? t.KSB / 64 * 96 + 16 : t.KSB >= 128 ? t.KSB : t.KSB + 16;
-      const size_t bytes =
-          t.DEPTH >= 3 ? (size_t)t.DEPTH * (t.P + t.Q) * (t.KSB / 64 * (f == kFmtF6 ? 96 : 64))
-                       : (size_t)2 * (t.P + t.Q) * row;
+      const size_t bytes = direct_loads(t.DEPTH, t.KG)
+                               ? (size_t)t.DEPTH * t.KG * (t.P + t.Q) *
+                                     (t.KSB / 64 * (f == kFmtF6 ? 96 : 64))
+                               : (size_t)2 * (t.P + t.Q) * row;
+      if (t.KG != 1) return false;  // the persistent kernel runs 256-thread blocks
       if (bytes > kMfmaLdsMax) return false;
     }
   return true;

@@ -36,7 +36,7 @@ REP_K = [512, 2048, 8192]
 SHAPES = ["128x128", "64x128", "32x256/k64", "16x256", "64x64", "128x64", "128x128/k64",
           "128x256/k64", "128x128/pp", "128x256/pp", "64x128/pp", "256x256/k64",
           "256x256", "64x64/dl4", "64x64/k256/dl3", "128x128/dl3", "128x64/dl3",
-          "64x128/dl3", "64x64/dl3"]
+          "64x128/dl3", "64x64/dl3", "64x64/dl2x2g", "64x128/dl2x2g", "64x64/dl3x2g"]
 
 
 FORMATS = [("int8", gbs.MFMA), ("fp4", gbs.MFMA_FP4), ("fp6", gbs.MFMA_FP6)]
