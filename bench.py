@@ -20,6 +20,9 @@ on).  The other configurations are reported beside it under "configs":
   c4  serving: 8-layer MLP, 4-bit signed weights resident in HBM, Poisson
       arrivals of 1..64-row 4-bit requests, continuous batching; p50/p99
       latency and TOPS
+  c5  the CaffeNet GEMM list of the reference benchmark (8 products, 2-bit
+      unsigned x 2-bit unsigned) in one batched launch, against one launch
+      per product and the reference CPU loop
 Every GPU result is checked bit-exact against an exact product before it is
 reported.  The reference CPU path (gemmBitSerial compiled from the
 reference sources, oracle/_ref, single thread) is timed on a bounded sample
@@ -205,10 +208,61 @@ def run_config(gbs, O, name, runs, engines, steps, warmup, stream, ws, cpu_secs)
     return out
 
 
-def bench_batched(gbs, O, steps, warmup, stream, log):
+def bench_im2col_pack(gbs, steps, warmup, log):
+    """Conv activations of the c2 shape (16 x 256 x 56 x 56, 3 x 3, pad 1,
+    3-bit unsigned) packed into bit-planes two ways, sources resident in HBM:
+    straight from the NCHW tensor (gbs_import_im2col) and from the int8
+    im2col matrix (gbs_import).  Both must give the same words."""
+    import ctypes as C
+    rng = np.random.default_rng(SEED0 + 2)
+    x = rng.integers(1, 8, (16, 256, 56, 56)).astype(np.int8)
+    x[rng.random(x.shape) < 0.5] = 0
+    g = gbs.conv2d(16, 256, 56, 56, 3, 3, 1, 1)
+    rows, cols, oh, ow = gbs.conv2d_shape(g)
+    xp = np.zeros((16, 256, 58, 58), np.int8)
+    xp[:, :, 1:57, 1:57] = x
+    host = np.empty((16, oh, ow, 256, 3, 3), np.int8)
+    for ky in range(3):
+        for kx in range(3):
+            host[:, :, :, :, ky, kx] = xp[:, :, ky:ky + oh, kx:kx + ow].transpose(0, 2, 3, 1)
+    host = host.reshape(rows, cols)
+    d_x, d_m = gbs.DeviceBuffer(x.nbytes), gbs.DeviceBuffer(host.nbytes)
+    d_x.copy_from_host(x)
+    d_m.copy_from_host(host)
+    a, b = gbs.DeviceMatrix(3, rows, cols, False), gbs.DeviceMatrix(3, rows, cols, False)
+    st = gbs.stream()
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        e0, e1 = gbs.Event(), gbs.Event()
+        e0.record(st)
+        for _ in range(steps):
+            fn()
+        e1.record(st)
+        gbs.sync(st)
+        return e0.elapsed_ms(e1) / steps * 1e3
+    t_nchw = timed(lambda: gbs.check(gbs.lib().gbs_import_im2col(
+        a.h, C.c_void_p(d_x.ptr), gbs.I8, C.byref(g), 1, None)))
+    t_mat = timed(lambda: gbs.check(gbs.lib().gbs_import(
+        b.h, C.c_void_p(d_m.ptr), gbs.I8, 0, 1, None)))
+    if not np.array_equal(a.planes(), b.planes()):
+        raise SystemExit("c2 im2col pack: planes differ from the im2col-matrix import")
+    out = {"workload": "16x256x56x56 NCHW, 3x3 pad 1, 3-bit unsigned -> 50176 x 2304 planes",
+           "from_nchw_us": round(t_nchw, 1), "from_im2col_matrix_us": round(t_mat, 1),
+           "nchw_bytes": x.nbytes, "im2col_matrix_bytes": host.nbytes,
+           "plane_bytes": 3 * rows * cols // 8, "bit_exact": True}
+    log(f"[bench] c2 pack: from NCHW {t_nchw:.1f} us, from the int8 im2col matrix "
+        f"{t_mat:.1f} us (identical planes)")
+    for m in (a, b, d_x, d_m):
+        m.close()
+    return out
+
+
+def bench_batched(gbs, O, steps, warmup, stream, log, name="c3", cpu_secs=0.0):
     from gemmbitserial_amd import workloads as W
-    log("[bench] c3: generate and pack 512 products")
-    probs = W.batched_uneven()
+    probs = W.batched_uneven() if name == "c3" else W.caffenet()
+    log(f"[bench] {name}: generate and pack {len(probs)} products")
     items = []
     for p in probs:
         l = gbs.from_values(p["lhs"], p["lb"], p["ls"])
@@ -230,12 +284,14 @@ def bench_batched(gbs, O, steps, warmup, stream, log):
         gbs.sync(stream)
         return e0.elapsed_ms(e1) / n
 
-    out = {"workload": "512 uneven products in one launch", "problems": len(probs),
+    out = {"workload": ("512 uneven products in one launch" if name == "c3" else
+                        "CaffeNet GEMM list, 2u x 2u, in one launch"), "problems": len(probs),
            "tiles": info["tiles"], "persistent_products": info["persistent"],
            "separate_products": info["separate"], "ops": ops}
+    cname = name
     for name, fn, n in [("graph", b.run_graph, steps), ("persistent", b.run, steps),
                         ("per_call", b.run_per_call, max(3, steps // 5))]:
-        log(f"[bench] c3: {name} x {n}")
+        log(f"[bench] {cname}: {name} x {n}")
         ms = timed(fn, n)
         bad = 0
         for (l, r, d), p in zip(items, probs):
@@ -246,13 +302,28 @@ def bench_batched(gbs, O, steps, warmup, stream, log):
             if name != "graph":
                 break  # full check once; spot-check the other runs
         if bad:
-            raise SystemExit(f"c3 {name}: {bad} products differ from the exact result")
+            raise SystemExit(f"{cname} {name}: {bad} products differ from the exact result")
         out[name] = {"ms": round(ms, 4), "tops": round(ops / (ms * 1e-3) / 1e12, 2)}
-        log(f"[bench] c3: {name} {ms * 1e3:.1f} us, {ops / (ms * 1e-3) / 1e12:.1f} TOPS")
+        log(f"[bench] {cname}: {name} {ms * 1e3:.1f} us, {ops / (ms * 1e-3) / 1e12:.1f} TOPS")
         d0 = items[0][2]
         d0.fill_zero()
     out["speedup_graph_vs_per_call"] = round(out["per_call"]["ms"] / out["graph"]["ms"], 2)
     out["bit_exact"] = True
+    if cpu_secs > 0 and O.ref_available():
+        # the reference loop of benchmark.cpp:213-220: every product once per pass
+        ms = 0.0
+        for p in probs:
+            ns, reps, res = O.ref_bench_gemm(p["lhs"], p["rhs"], p["lb"], p["rb"], p["ls"],
+                                             p["rs"], min_secs=cpu_secs / len(probs),
+                                             max_reps=200)
+            want = O.wrap_i32(O.blas_gemm_exact(O.values_of(p["lhs"], p["lb"], p["ls"]),
+                                                O.values_of(p["rhs"], p["rb"], p["rs"])))
+            if not np.array_equal(res, want):
+                raise SystemExit(f"{cname}: reference CPU result differs")
+            ms += ns / 1e6
+        out["cpu_reference"] = {"ms_per_pass": round(ms, 3),
+                                "tops": round(ops / (ms * 1e-3) / 1e12, 4), "cores": 1}
+        log(f"[bench] {cname}: reference CPU {ms:.1f} ms per pass")
     b.close()
     for l, r, d in items:
         l.close(), r.close(), d.close()
@@ -324,7 +395,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--configs", default="c1,c0,c2,c3,c4")
+    ap.add_argument("--configs", default="c1,c0,c2,c3,c4,c5")
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--engines", default="all", choices=["auto", "all"])
@@ -358,8 +429,9 @@ def main():
         names = ["c1"] + names
     results = {}
     for i, name in enumerate(names):
-        if name == "c3":
-            results[name] = bench_batched(gbs, O, args.steps, args.warmup, stream, log)
+        if name in ("c3", "c5"):
+            results[name] = bench_batched(gbs, O, args.steps, args.warmup, stream, log, name,
+                                          cpu_secs if name == "c5" else 0.0)
             continue
         if name == "c4":
             results[name] = bench_serving(gbs, args.serve_rate, args.serve_requests, log)
@@ -369,6 +441,8 @@ def main():
         results[name] = {"workload": desc, "seed": SEED0 + int(name[1]),
                          "runs": run_config(gbs, O, name, runs, args.engines, args.steps,
                                             args.warmup, stream, ws, cpu_secs)}
+        if name == "c2":
+            results[name]["pack"] = bench_im2col_pack(gbs, min(args.steps, 20), 2, log)
     # headline: the binary layers through gemmBitSerial's own engine choice
     c1 = results["c1"]["runs"]
     step_ms = sum(r["engines"][r["auto"]]["us"] for r in c1) / 1e3

@@ -43,6 +43,7 @@ EXPORTED = [
     "gbs_clock_ns", "gbs_server_create", "gbs_server_destroy", "gbs_server_submit",
     "gbs_server_step", "gbs_server_take", "gbs_server_stats",
     "gbs_gemm_async_config",
+    "gbs_conv2d_shape", "gbs_import_im2col", "gbs_matrix_save", "gbs_matrix_load",
 ]
 
 
@@ -91,6 +92,9 @@ def lib():
         "gbs_server_step": [_p, _p],
         "gbs_server_take": [_p, _u64, _p, _u64, _p, _p, _p, _p],
         "gbs_server_stats": [_p, _p],
+        "gbs_conv2d_shape": [_p, _p, _p, _p, _p],
+        "gbs_import_im2col": [_p, _p, _i, _p, _i, _p],
+        "gbs_matrix_save": [_p, C.c_char_p], "gbs_matrix_load": [C.c_char_p, _p],
     }
     for name, args in sig.items():
         f = getattr(L, name)
@@ -116,20 +120,71 @@ def init(device=0):
     check(lib().gbs_init(device))
 
 
+class Conv2d(C.Structure):
+    """gbs_conv2d: NCHW input, kernel, strides, zero padding, dilations."""
+    _fields_ = [(n, C.c_uint32) for n in ("batch", "channels", "height", "width", "kh", "kw",
+                                           "stride_h", "stride_w", "pad_h", "pad_w",
+                                           "dil_h", "dil_w")]
+
+
+def conv2d(batch, channels, height, width, kh, kw, stride=1, pad=0, dil=1):
+    sh, sw = (stride, stride) if isinstance(stride, int) else stride
+    ph, pw = (pad, pad) if isinstance(pad, int) else pad
+    dh, dw = (dil, dil) if isinstance(dil, int) else dil
+    return Conv2d(batch, channels, height, width, kh, kw, sh, sw, ph, pw, dh, dw)
+
+
+def conv2d_shape(g):
+    """(rows, cols, out_h, out_w) of the im2col operand of geometry g."""
+    r, c, oh, ow = C.c_uint64(), C.c_uint64(), C.c_uint32(), C.c_uint32()
+    check(lib().gbs_conv2d_shape(C.byref(g), C.byref(r), C.byref(c), C.byref(oh), C.byref(ow)))
+    return r.value, c.value, oh.value, ow.value
+
+
+PLANE_FILE_MAGIC = b"GBSPLN01"
+
+
 class DeviceMatrix:
     """A gbs_matrix: bit-planes resident in HBM, reference layout
     [nbits][nrows_a][ncols_a/64] (BitSerialMatrix, gemmbitserial.hpp:19-50)."""
 
-    def __init__(self, nbits, nrows, ncols, issigned, nrows_a=None, ncols_a=None):
+    def __init__(self, nbits, nrows, ncols, issigned, nrows_a=None, ncols_a=None,
+                 _handle=None):
         self.nbits, self.nrows, self.ncols = int(nbits), int(nrows), int(ncols)
         self.issigned = bool(issigned)
         self.nrows_a = self.nrows if nrows_a is None else int(nrows_a)
         self.ncols_a = ((self.ncols + 63) // 64) * 64 if ncols_a is None else int(ncols_a)
+        if _handle is not None:
+            self.h = _handle
+            return
         h = C.c_void_p()
         check(lib().gbs_matrix_create(self.nbits, self.nrows, self.ncols,
                                       int(self.issigned), self.nrows_a, self.ncols_a,
                                       C.byref(h)))
         self.h = h
+
+    @classmethod
+    def load(cls, path):
+        """A matrix from a packed-matrix file (gbs_matrix_save's format)."""
+        h = C.c_void_p()
+        check(lib().gbs_matrix_load(os.fsencode(path), C.byref(h)))
+        with open(path, "rb") as f:
+            head = np.frombuffer(f.read(64)[8:], dtype="<u8")
+        nbits, nrows, ncols, nrows_a, ncols_a, issigned = (int(x) for x in head[:6])
+        return cls(nbits, nrows, ncols, issigned, nrows_a, ncols_a, _handle=h)
+
+    def save(self, path):
+        check(lib().gbs_matrix_save(self.h, os.fsencode(path)))
+
+    def import_im2col(self, x, g, want_planes=False):
+        """Pack the im2col matrix of NCHW tensor x (geometry g) into the planes."""
+        x = np.ascontiguousarray(x)
+        if x.size != g.batch * g.channels * g.height * g.width:
+            raise ValueError("tensor size does not match the geometry")
+        out = np.zeros(self.words, np.uint64) if want_planes else None
+        check(lib().gbs_import_im2col(self.h, _ptr(x), _NP_DT[x.dtype], C.byref(g), 0,
+                                      _ptr(out) if out is not None else None))
+        return out
 
     @property
     def words(self):
@@ -250,6 +305,14 @@ class DeviceBuffer:
         if rc != 0:
             raise GbsError(3, f"hipMemcpy D2H failed: {rc}")
         return out
+
+    def copy_from_host(self, a):
+        a = np.ascontiguousarray(a)
+        if a.nbytes > self.nbytes:
+            raise ValueError("array larger than the buffer")
+        rc = self.hip().hipMemcpy(C.c_void_p(self.ptr), _ptr(a), a.nbytes, 1)
+        if rc != 0:
+            raise GbsError(3, f"hipMemcpy H2D failed: {rc}")
 
     def copy_from(self, other):
         rc = self.hip().hipMemcpy(C.c_void_p(self.ptr), C.c_void_p(other.ptr),

@@ -75,6 +75,8 @@ inline Operand view(const gbs_matrix *m) {
 // kernels (gbs_pack.hip)
 int launch_pack(const gbs_matrix *m, const void *d_src, gbs_dtype dt,
                 bool col_major, hipStream_t s);
+int launch_pack_im2col(const gbs_matrix *m, const void *d_src, gbs_dtype dt,
+                       const gbs_conv2d &g, uint32_t oh, uint32_t ow, hipStream_t s);
 int launch_pack_quantize(const gbs_matrix *m, const int32_t *d_src,
                          const int32_t *d_thr, int nthres, bool col_major,
                          hipStream_t s);

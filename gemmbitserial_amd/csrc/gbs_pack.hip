@@ -1,14 +1,21 @@
 // gbs_pack.hip -- regular <-> bit-serial conversion kernels for gfx950.
 //
-//   pack (row-major)  one wave per 64 consecutive plane words; for each word
-//                     the 64 lanes load the 64 columns it covers (a coalesced
-//                     row segment), quantize, and one __ballot per plane
-//                     turns "bit b of my code" into that plane word.  Lane t
-//                     keeps word t, so each plane is written back as one
-//                     coalesced 512-byte store.
+//   pack (row-major)  one thread per 16 consecutive columns of a row (a
+//                     quarter of a plane word): one 16-byte vector load per
+//                     16 bytes of source when rows are 16-byte aligned,
+//                     quantize, and per plane one 16-bit store of those 16
+//                     bits -- the four threads of a word write its four
+//                     quarters, so a wave stores 128 contiguous bytes per
+//                     plane.  Threads past the real matrix write the zero
+//                     padding.  (The threshold import keeps a ballot kernel:
+//                     one wave per 64 plane words, one __ballot per plane.)
 //   pack (col-major)  one wave per 64 rows x 1 word; lane = row, so loads of
 //                     src[c*nrows + r] are coalesced; each lane shifts its
 //                     64 bits into its own words.
+//   pack (im2col)     the im2col matrix of an NCHW tensor packed without
+//                     building it: lane = output pixel, so for each
+//                     (c, ky, kx) the 64 lanes read 64 neighbouring input
+//                     elements of one image row (0 outside the input).
 //   export / sum_rows the inverse and the row-wise sums.
 //
 // Semantics follow gemmbitserial.hpp:169-186 (quantize), 212-275
@@ -108,6 +115,47 @@ __global__ __launch_bounds__(256) void pack_rows_kernel(
   }
 }
 
+// --- row-major pack, 16 columns per thread ----------------------------------
+// VEC: rows start 16-byte aligned and hold a multiple of 16 bytes, so the 16
+// elements of a full quarter are 16 * sizeof(T) / 16 aligned vector loads.
+template <typename T, bool RAW, bool VEC>
+__global__ __launch_bounds__(256) void pack_rows16_kernel(
+    const T *src, uint64_t nrows, uint64_t ncols, uint64_t nrows_a, uint64_t wpr,
+    uint64_t W, uint32_t nbits, bool bipolar, uint16_t *planes16) {
+  const uint64_t qpr = wpr * 4;  // quarters per row
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrows_a * qpr) return;
+  const uint64_t r = i / qpr, q = i - r * qpr, c0 = q * 16;
+  uint32_t code[16];
+#pragma unroll
+  for (int j = 0; j < 16; j++) code[j] = 0;
+  if (r < nrows && c0 < ncols) {
+    const T *row = src + r * ncols;
+    if (VEC && c0 + 16 <= ncols) {
+      constexpr int NV = 16 * (int)sizeof(T) / 16;
+      alignas(16) T e[16];
+#pragma unroll
+      for (int v = 0; v < NV; v++)
+        *reinterpret_cast<uint4 *>(&e[v * 16 / sizeof(T)]) =
+            reinterpret_cast<const uint4 *>(row + c0)[v];
+#pragma unroll
+      for (int j = 0; j < 16; j++) code[j] = CodeOf<T, RAW>::get(e[j], bipolar);
+    } else {
+#pragma unroll
+      for (int j = 0; j < 16; j++)
+        if (c0 + j < ncols) code[j] = CodeOf<T, RAW>::get(row[c0 + j], bipolar);
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < 8; b++) {
+    if ((uint32_t)b >= nbits) break;
+    uint32_t bits = 0;
+#pragma unroll
+    for (int j = 0; j < 16; j++) bits |= ((code[j] >> b) & 1u) << j;
+    planes16[((uint64_t)b * W + r * wpr) * 4 + q] = (uint16_t)bits;
+  }
+}
+
 // --- column-major pack: lane = row, each lane builds its own words ------
 template <typename Src>
 __global__ __launch_bounds__(256) void pack_cols_kernel(
@@ -126,7 +174,19 @@ __global__ __launch_bounds__(256) void pack_cols_kernel(
     const uint64_t c0 = w * kWave;
     const uint32_t jmax = c0 >= ncols ? 0u
                           : (uint32_t)((ncols - c0) < kWave ? (ncols - c0) : kWave);
-    for (uint32_t j = 0; j < jmax; j++) {
+    // whole words unrolled by 16 so that the loads of a group overlap
+    uint32_t j = 0;
+    for (; j + 16 <= jmax; j += 16) {
+      uint32_t code[16];
+#pragma unroll
+      for (int u = 0; u < 16; u++) code[u] = src.code(r, c0 + j + u);
+#pragma unroll
+      for (int u = 0; u < 16; u++)
+#pragma unroll
+        for (int b = 0; b < 8; b++)
+          if ((uint32_t)b < nbits) out[b] |= (uint64_t)((code[u] >> b) & 1u) << (j + u);
+    }
+    for (; j < jmax; j++) {
       uint32_t code = src.code(r, c0 + j);
 #pragma unroll
       for (int b = 0; b < 8; b++)
@@ -147,12 +207,17 @@ int pack_typed(const gbs_matrix *m, const void *d_src, bool col_major,
   const uint32_t nb = (uint32_t)m->nbits;
   if (W == 0) return GBS_OK;
   if (!col_major) {
-    RowMajorSrc<T, RAW> src{(const T *)d_src, m->nrows, m->ncols, m->bipolar()};
-    const uint64_t waves = (W + kWave - 1) / kWave;
-    const uint64_t blocks = (waves + 3) / 4;
-    hipLaunchKernelGGL((pack_rows_kernel<RowMajorSrc<T, RAW>>), dim3((uint32_t)blocks),
-                       dim3(256), 0, s, src, m->d_planes, m->nrows, m->ncols,
-                       m->wpr(), W, nb);
+    const uint64_t n = m->nrows_a * m->wpr() * 4;
+    const bool vec = ((uintptr_t)d_src % 16 == 0) && (m->ncols * sizeof(T)) % 16 == 0;
+    const dim3 grid((uint32_t)((n + 255) / 256));
+    if (vec)
+      hipLaunchKernelGGL((pack_rows16_kernel<T, RAW, true>), grid, dim3(256), 0, s,
+                         (const T *)d_src, m->nrows, m->ncols, m->nrows_a, m->wpr(), W, nb,
+                         m->bipolar(), (uint16_t *)m->d_planes);
+    else
+      hipLaunchKernelGGL((pack_rows16_kernel<T, RAW, false>), grid, dim3(256), 0, s,
+                         (const T *)d_src, m->nrows, m->ncols, m->nrows_a, m->wpr(), W, nb,
+                         m->bipolar(), (uint16_t *)m->d_planes);
   } else {
     ColMajorSrc<T, RAW> src{(const T *)d_src, m->nrows, m->ncols, m->bipolar()};
     const uint64_t waves = (m->nrows_a + kWave - 1) / kWave * m->wpr();
@@ -161,6 +226,81 @@ int pack_typed(const gbs_matrix *m, const void *d_src, bool col_major,
                        dim3(256), 0, s, src, m->d_planes, m->nrows, m->ncols,
                        m->nrows_a, m->wpr(), W, nb);
   }
+  GBS_HIP(hipGetLastError());
+  return GBS_OK;
+}
+
+// im2col of an NCHW tensor: row r = output pixel (n, oy, ox), column
+// c = (ch, ky, kx); element input[n][ch][oy*sh - ph + ky*dh][ox*sw - pw +
+// kx*dw], 0 outside the input.  One wave per 64 rows x one plane word;
+// lane = row, so for each column the wave reads 64 neighbouring pixels of
+// one input row.  Per lane the pixel's offsets are computed once; (ch, ky,
+// kx) advance as wave-uniform counters.  An image is below 2^31 elements
+// and the operand's rows and columns below 2^31 (checked on the host).
+struct Im2colGeom {
+  uint32_t C, H, W, KH, KW, OW, OHW, sh, sw, ph, pw, dh, dw;
+};
+
+template <typename T, bool RAW>
+__global__ __launch_bounds__(256) void pack_im2col_kernel(
+    const T *src, Im2colGeom g, uint64_t nrows, uint64_t ncols, uint64_t nrows_a,
+    uint64_t wpr, uint64_t W, uint32_t nbits, bool bipolar, uint64_t *planes) {
+  const uint32_t lane = threadIdx.x & (kWave - 1);
+  const uint64_t wave = (uint64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  const uint64_t row_groups = (nrows_a + kWave - 1) / kWave;
+  if (wave >= row_groups * wpr) return;
+  const uint64_t w = wave / row_groups;
+  const uint64_t r = (wave - w * row_groups) * kWave + lane;
+  uint64_t out[8];
+#pragma unroll
+  for (int b = 0; b < 8; b++) out[b] = 0;
+  const uint64_t c0 = w * kWave;
+  if (r < nrows && c0 < ncols) {
+    const uint32_t n = (uint32_t)(r / g.OHW), rem = (uint32_t)(r - (uint64_t)n * g.OHW);
+    const uint32_t oy = rem / g.OW, ox = rem - oy * g.OW;
+    const int iy0 = (int)(oy * g.sh) - (int)g.ph, ix0 = (int)(ox * g.sw) - (int)g.pw;
+    const T *img = src + (uint64_t)n * g.C * g.H * g.W;
+    const uint32_t jmax = (uint32_t)((ncols - c0) < kWave ? (ncols - c0) : kWave);
+    // wave-uniform position of column c0
+    uint32_t ch = (uint32_t)(c0 / (g.KH * g.KW));
+    const uint32_t kr = (uint32_t)(c0 - (uint64_t)ch * g.KH * g.KW);
+    uint32_t ky = kr / g.KW, kx = kr - ky * g.KW;
+    for (uint32_t j = 0; j < jmax; j++) {
+      const int iy = iy0 + (int)(ky * g.dh), ix = ix0 + (int)(kx * g.dw);
+      T e = (T)0;
+      if ((uint32_t)iy < g.H && (uint32_t)ix < g.W)
+        e = img[(ch * g.H + (uint32_t)iy) * g.W + (uint32_t)ix];
+      const uint32_t code = CodeOf<T, RAW>::get(e, bipolar);
+#pragma unroll
+      for (int b = 0; b < 8; b++)
+        if ((uint32_t)b < nbits) out[b] |= (uint64_t)((code >> b) & 1u) << j;
+      if (++kx == g.KW) {
+        kx = 0;
+        if (++ky == g.KH) {
+          ky = 0;
+          ++ch;
+        }
+      }
+    }
+  }
+  if (r < nrows_a) {
+#pragma unroll
+    for (int b = 0; b < 8; b++)
+      if ((uint32_t)b < nbits) planes[(uint64_t)b * W + r * wpr + w] = out[b];
+  }
+}
+
+template <typename T, bool RAW>
+int pack_im2col_typed(const gbs_matrix *m, const void *d_src, const gbs_conv2d &c,
+                      uint32_t oh, uint32_t ow, hipStream_t s) {
+  const uint64_t W = m->wpp();
+  if (W == 0) return GBS_OK;
+  const Im2colGeom g{c.channels, c.height, c.width, c.kh, c.kw, ow, oh * ow, c.stride_h,
+                     c.stride_w, c.pad_h, c.pad_w, c.dil_h, c.dil_w};
+  const uint64_t waves = (m->nrows_a + kWave - 1) / kWave * m->wpr();
+  hipLaunchKernelGGL((pack_im2col_kernel<T, RAW>), dim3((uint32_t)((waves + 3) / 4)),
+                     dim3(256), 0, s, (const T *)d_src, g, m->nrows, m->ncols, m->nrows_a,
+                     m->wpr(), W, (uint32_t)m->nbits, m->bipolar(), m->d_planes);
   GBS_HIP(hipGetLastError());
   return GBS_OK;
 }
@@ -255,6 +395,22 @@ int launch_pack(const gbs_matrix *m, const void *d_src, gbs_dtype dt,
     case GBS_U64: return pack_typed<uint64_t, false>(m, d_src, col_major, s);
   }
   return fail(GBS_EINVAL, "gbs_import: unknown dtype %d", (int)dt);
+}
+
+int launch_pack_im2col(const gbs_matrix *m, const void *d_src, gbs_dtype dt,
+                       const gbs_conv2d &g, uint32_t oh, uint32_t ow, hipStream_t s) {
+  switch (dt) {
+    case GBS_I8: return pack_im2col_typed<int8_t, false>(m, d_src, g, oh, ow, s);
+    case GBS_U8: return pack_im2col_typed<uint8_t, false>(m, d_src, g, oh, ow, s);
+    case GBS_U8_RAW: return pack_im2col_typed<uint8_t, true>(m, d_src, g, oh, ow, s);
+    case GBS_I16: return pack_im2col_typed<int16_t, false>(m, d_src, g, oh, ow, s);
+    case GBS_U16: return pack_im2col_typed<uint16_t, false>(m, d_src, g, oh, ow, s);
+    case GBS_I32: return pack_im2col_typed<int32_t, false>(m, d_src, g, oh, ow, s);
+    case GBS_U32: return pack_im2col_typed<uint32_t, false>(m, d_src, g, oh, ow, s);
+    case GBS_I64: return pack_im2col_typed<int64_t, false>(m, d_src, g, oh, ow, s);
+    case GBS_U64: return pack_im2col_typed<uint64_t, false>(m, d_src, g, oh, ow, s);
+  }
+  return fail(GBS_EINVAL, "gbs_import_im2col: unknown dtype %d", (int)dt);
 }
 
 int launch_pack_quantize(const gbs_matrix *m, const int32_t *d_src,

@@ -9,6 +9,7 @@
 
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "gbs_mfma_tile.h"
 
@@ -291,6 +292,124 @@ int gbs_import(gbs_matrix *m, const void *src, gbs_dtype dt, int read_col_major,
     GBS_HIP(hipMemcpyAsync(host_planes, m->d_planes, m->words() * 8,
                            hipMemcpyDeviceToHost, s));
   GBS_HIP(hipStreamSynchronize(s));
+  return GBS_OK;
+}
+
+int gbs_conv2d_shape(const gbs_conv2d *g, uint64_t *rows, uint64_t *cols, uint32_t *out_h,
+                     uint32_t *out_w) {
+  if (!g) return fail(GBS_EINVAL, "gbs_conv2d_shape: null geometry");
+  if (!g->batch || !g->channels || !g->height || !g->width || !g->kh || !g->kw ||
+      !g->stride_h || !g->stride_w || !g->dil_h || !g->dil_w)
+    return fail(GBS_EINVAL, "gbs_conv2d_shape: sizes, strides and dilations must be >= 1");
+  const uint64_t ekh = (uint64_t)g->dil_h * (g->kh - 1) + 1;
+  const uint64_t ekw = (uint64_t)g->dil_w * (g->kw - 1) + 1;
+  const uint64_t ph = (uint64_t)g->height + 2ull * g->pad_h, pw = (uint64_t)g->width + 2ull * g->pad_w;
+  if (ph < ekh || pw < ekw)
+    return fail(GBS_EINVAL, "gbs_conv2d_shape: the kernel does not fit the padded input");
+  const uint64_t oh = (ph - ekh) / g->stride_h + 1, ow = (pw - ekw) / g->stride_w + 1;
+  const uint64_t r = (uint64_t)g->batch * oh * ow, c = (uint64_t)g->channels * g->kh * g->kw;
+  if (r >= (1ull << 31) || c >= (1ull << 31) || ph >= (1ull << 31) || pw >= (1ull << 31) ||
+      (uint64_t)g->channels * g->height * g->width >= (1ull << 31))
+    return fail(GBS_ERANGE, "gbs_conv2d_shape: operand or image dimensions exceed 2^31");
+  if (rows) *rows = r;
+  if (cols) *cols = c;
+  if (out_h) *out_h = (uint32_t)oh;
+  if (out_w) *out_w = (uint32_t)ow;
+  return GBS_OK;
+}
+
+int gbs_import_im2col(gbs_matrix *m, const void *src, gbs_dtype dt, const gbs_conv2d *g,
+                      int src_on_device, uint64_t *host_planes) {
+  int rc = check_matrix(m, "gbs_import_im2col");
+  if (rc) return rc;
+  uint64_t rows = 0, cols = 0;
+  uint32_t oh = 0, ow = 0;
+  if ((rc = gbs_conv2d_shape(g, &rows, &cols, &oh, &ow))) return rc;
+  if (m->nrows != rows || m->ncols != cols)
+    return fail(GBS_EINVAL,
+                "gbs_import_im2col: matrix is %llu x %llu, the convolution needs %llu x %llu",
+                (unsigned long long)m->nrows, (unsigned long long)m->ncols,
+                (unsigned long long)rows, (unsigned long long)cols);
+  const size_t es = dtype_size(dt);
+  if (es == 0) return fail(GBS_EINVAL, "gbs_import_im2col: unknown dtype %d", (int)dt);
+  const size_t bytes = (size_t)g->batch * g->channels * g->height * g->width * es;
+  if (!src) return fail(GBS_EINVAL, "gbs_import_im2col: null source");
+  if ((rc = ensure_device())) return rc;
+  hipStream_t s = thread_stream();
+  const void *d_src = src;
+  if (!src_on_device && (rc = stage_host(src, bytes, &d_src, s))) return rc;
+  if ((rc = launch_pack_im2col(m, d_src, dt, *g, oh, ow, s))) return rc;
+  m->planes_version++;
+  if (host_planes && m->words())
+    GBS_HIP(hipMemcpyAsync(host_planes, m->d_planes, m->words() * 8,
+                           hipMemcpyDeviceToHost, s));
+  GBS_HIP(hipStreamSynchronize(s));
+  return GBS_OK;
+}
+
+static const char kPlaneMagic[8] = {'G', 'B', 'S', 'P', 'L', 'N', '0', '1'};
+
+int gbs_matrix_save(const gbs_matrix *m, const char *path) {
+  int rc = check_matrix(m, "gbs_matrix_save");
+  if (rc) return rc;
+  if (!path) return fail(GBS_EINVAL, "gbs_matrix_save: null path");
+  std::vector<uint64_t> words(m->words());
+  if (!words.empty()) {
+    if ((rc = ensure_device())) return rc;
+    hipStream_t s = thread_stream();
+    GBS_HIP(hipMemcpyAsync(words.data(), m->d_planes, words.size() * 8, hipMemcpyDeviceToHost,
+                           s));
+    GBS_HIP(hipStreamSynchronize(s));
+  }
+  const uint64_t head[7] = {m->nbits, m->nrows, m->ncols, m->nrows_a,
+                            m->ncols_a, m->issigned ? 1ull : 0ull, (uint64_t)words.size()};
+  FILE *f = fopen(path, "wb");
+  if (!f) return fail(GBS_EINVAL, "gbs_matrix_save: cannot create %s", path);
+  bool ok = fwrite(kPlaneMagic, 1, 8, f) == 8 && fwrite(head, 8, 7, f) == 7 &&
+            (words.empty() || fwrite(words.data(), 8, words.size(), f) == words.size());
+  ok = (fclose(f) == 0) && ok;
+  if (!ok) return fail(GBS_EINVAL, "gbs_matrix_save: short write to %s", path);
+  return GBS_OK;
+}
+
+int gbs_matrix_load(const char *path, gbs_matrix **out) {
+  if (!out) return fail(GBS_EINVAL, "gbs_matrix_load: null out");
+  *out = nullptr;
+  if (!path) return fail(GBS_EINVAL, "gbs_matrix_load: null path");
+  FILE *f = fopen(path, "rb");
+  if (!f) return fail(GBS_EINVAL, "gbs_matrix_load: cannot open %s", path);
+  char magic[8];
+  uint64_t head[7];
+  if (fread(magic, 1, 8, f) != 8 || memcmp(magic, kPlaneMagic, 8) != 0 ||
+      fread(head, 8, 7, f) != 7) {
+    fclose(f);
+    return fail(GBS_EINVAL, "gbs_matrix_load: %s is not a packed-matrix file", path);
+  }
+  const uint64_t nbits = head[0], nrows = head[1], ncols = head[2], nrows_a = head[3],
+                 ncols_a = head[4], issigned = head[5], nwords = head[6];
+  const unsigned __int128 want =
+      (unsigned __int128)nbits * nrows_a * (ncols_a / 64);
+  long end = -1;
+  if (fseek(f, 0, SEEK_END) == 0) end = ftell(f);
+  if (nbits < 1 || nbits > 8 || issigned > 1 || ncols_a % 64 || nrows > nrows_a ||
+      ncols > ncols_a || want != nwords || nwords > (1ull << 40) || end < 0 ||
+      (uint64_t)end != 64 + nwords * 8) {
+    fclose(f);
+    return fail(GBS_EINVAL, "gbs_matrix_load: %s has an inconsistent header or size", path);
+  }
+  std::vector<uint64_t> words(nwords);
+  bool ok = fseek(f, 64, SEEK_SET) == 0 &&
+            (nwords == 0 || fread(words.data(), 8, nwords, f) == nwords);
+  fclose(f);
+  if (!ok) return fail(GBS_EINVAL, "gbs_matrix_load: short read from %s", path);
+  gbs_matrix *m = nullptr;
+  int rc = gbs_matrix_create(nbits, nrows, ncols, (int)issigned, nrows_a, ncols_a, &m);
+  if (rc) return rc;
+  if (nwords && (rc = gbs_planes_from_host(m, words.data()))) {
+    gbs_matrix_destroy(m);
+    return rc;
+  }
+  *out = m;
   return GBS_OK;
 }
 

@@ -38,6 +38,25 @@ def batched_uneven(count=512, seed=SEED0 + 3, scale=1.0):
     return probs
 
 
+# CaffeNet's GEMMs as (rows M, depth K, columns N), the list the reference
+# benchmarks at 2-bit unsigned weights x 2-bit unsigned activations
+# (benchmark/benchmark.cpp:173-198)
+CAFFENET_GEMMS = [(96, 363, 3025), (256, 2400, 729), (384, 2304, 169), (384, 3456, 169),
+                  (256, 3456, 169), (4096, 9216, 1), (4096, 4096, 1), (1000, 4096, 1)]
+
+
+def caffenet(seed=SEED0 + 5, wbits=2, abits=2):
+    """The CaffeNet GEMM list with uniform unsigned values (weights lhs
+    M x K, activations rhs N x K).  Returns a list of problem dicts."""
+    rng = np.random.default_rng(seed)
+    probs = []
+    for M, K, N in CAFFENET_GEMMS:
+        probs.append(dict(M=M, K=K, N=N, lb=wbits, rb=abits, ls=False, rs=False,
+                          lhs=values(rng, (M, K), wbits, False),
+                          rhs=values(rng, (N, K), abits, False)))
+    return probs
+
+
 def mlp_shift(h_in, w_bits=4, a_bits=4):
     """Requant shift for a layer of depth h_in: acc >> s lands the typical
     accumulator (std of a sum of h_in products of uniform w_bits-bit signed

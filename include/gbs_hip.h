@@ -131,6 +131,45 @@ int gbs_import_quantize(gbs_matrix *m, const int32_t *src,
                         int read_col_major, int src_on_device,
                         uint64_t *host_planes);
 
+/* Geometry of a 2-D convolution whose activation operand is packed by
+ * gbs_import_im2col: NCHW input of batch x channels x height x width,
+ * kernel kh x kw, strides, zero padding and dilations.  The operand has one
+ * row per output pixel (n, oy, ox) in that order and one column per
+ * (c, ky, kx) in that order -- the depth order of an [O][C][kh][kw] weight
+ * tensor flattened row-major, which is then the lhs. */
+typedef struct gbs_conv2d {
+  uint32_t batch, channels, height, width;
+  uint32_t kh, kw, stride_h, stride_w, pad_h, pad_w, dil_h, dil_w;
+} gbs_conv2d;
+
+/* Rows (batch*out_h*out_w) and columns (channels*kh*kw) of the operand;
+ * GBS_EINVAL for an empty or inconsistent geometry.  Host only. */
+int gbs_conv2d_shape(const gbs_conv2d *g, uint64_t *rows, uint64_t *cols,
+                     uint32_t *out_h, uint32_t *out_w);
+
+/* Pack the im2col matrix of an NCHW tensor straight into m's bit-planes,
+ * without building it: element (row, col) is input[n][c][oy*stride_h -
+ * pad_h + ky*dil_h][ox*stride_w - pad_w + kx*dil_w], or 0 outside the
+ * input, quantized as gbs_import quantizes (so padding codes as 0, and as
+ * -1 for a bipolar matrix, exactly as importing a zero-padded im2col matrix
+ * would).  m must be rows x cols of gbs_conv2d_shape.  No reference
+ * counterpart: it generalises importRegular (gemmbitserial.hpp:212-275) to
+ * a convolution's activations, which the reference's callers would first
+ * expand to a K x N matrix on the host (the CaffeNet list,
+ * benchmark/benchmark.cpp:173-185, holds such GEMMs). */
+int gbs_import_im2col(gbs_matrix *m, const void *src, gbs_dtype dt,
+                      const gbs_conv2d *g, int src_on_device, uint64_t *host_planes);
+
+/* Packed-matrix files: a 64-byte header (magic "GBSPLN01", then uint64
+ * nbits, nrows, ncols, nrows_a, ncols_a, issigned, word count, little
+ * endian) followed by the plane words in the reference layout
+ * [nbits][nrows_a][ncols_a/64] -- byte for byte the BitSerialMatrix::data
+ * array of gemmbitserial.hpp:44-50 for a matrix of that geometry.  No
+ * reference counterpart (the reference keeps planes in memory only). */
+int gbs_matrix_save(const gbs_matrix *m, const char *path);
+/* Validates the header and size before touching the device. */
+int gbs_matrix_load(const char *path, gbs_matrix **out);
+
 /* Copy plane words between a host buffer (nbits*nrows_a*ncols_a/64 words)
  * and the device.  Used by the drop-in header to keep BitSerialMatrix::data
  * (gemmbitserial.hpp:50) and the device copy coherent after host-side

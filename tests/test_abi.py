@@ -54,3 +54,49 @@ def test_import_refuses_unknown_dtype():
     # a fake non-null matrix pointer is never dereferenced before the dtype check
     assert L.gbs_import(C.c_void_p(8), C.c_void_p(16), 42, 0, 0, None) == 1
     assert b"dtype" in L.gbs_last_error()
+
+
+def test_conv2d_shape_closed_form():
+    import gemmbitserial_amd as g
+    for (N, Cc, H, W, kh, kw, s, p, d) in [(16, 256, 56, 56, 3, 3, 1, 1, 1),
+                                         (1, 3, 227, 227, 11, 11, 4, 0, 1),
+                                         (2, 5, 9, 7, 3, 2, (2, 1), (1, 0), (1, 2)),
+                                         (1, 1, 1, 1, 1, 1, 1, 0, 1)]:
+        geo = g.conv2d(N, Cc, H, W, kh, kw, s, p, d)
+        sh, sw = (s, s) if isinstance(s, int) else s
+        ph, pw = (p, p) if isinstance(p, int) else p
+        dh, dw = (d, d) if isinstance(d, int) else d
+        oh = (H + 2 * ph - (dh * (kh - 1) + 1)) // sh + 1
+        ow = (W + 2 * pw - (dw * (kw - 1) + 1)) // sw + 1
+        assert g.conv2d_shape(geo) == (N * oh * ow, Cc * kh * kw, oh, ow)
+    # the CaffeNet conv1 operand: 55 x 55 output pixels, K = 3 * 11 * 11 = 363
+    assert g.conv2d_shape(g.conv2d(1, 3, 227, 227, 11, 11, 4))[:2] == (3025, 363)
+    for bad in [g.conv2d(1, 1, 2, 2, 3, 3), g.conv2d(0, 1, 4, 4, 1, 1),
+                g.conv2d(1, 1, 4, 4, 1, 1, stride=0)]:
+        with pytest.raises(g.GbsError):
+            g.conv2d_shape(bad)
+
+
+def test_matrix_load_rejects_bad_files_before_touching_the_device(tmp_path):
+    import numpy as np
+    import gemmbitserial_amd as g
+    L = g.lib()
+    out = C.c_void_p()
+    good_head = np.array([2, 5, 70, 8, 128, 1, 2 * 8 * 2], dtype="<u8")
+    cases = {
+        "magic": b"NOTPLANE" + good_head.tobytes() + bytes(8 * 32),
+        "short": g.PLANE_FILE_MAGIC + good_head.tobytes() + bytes(8 * 31),
+        "count": g.PLANE_FILE_MAGIC + np.array([2, 5, 70, 8, 128, 1, 31], "<u8").tobytes()
+        + bytes(8 * 31),
+        "ncols_a": g.PLANE_FILE_MAGIC + np.array([2, 5, 70, 8, 100, 1, 24], "<u8").tobytes()
+        + bytes(8 * 24),
+        "nbits": g.PLANE_FILE_MAGIC + np.array([9, 5, 70, 8, 128, 1, 144], "<u8").tobytes()
+        + bytes(8 * 144),
+    }
+    for name, blob in cases.items():
+        p = tmp_path / f"{name}.gbsp"
+        p.write_bytes(blob)
+        assert L.gbs_matrix_load(os.fsencode(str(p)), C.byref(out)) == 1, name
+        msg = L.gbs_last_error()
+        assert b"not a packed-matrix file" in msg or b"inconsistent" in msg, (name, msg)
+    assert L.gbs_matrix_load(b"/nonexistent/x.gbsp", C.byref(out)) == 1

@@ -103,3 +103,20 @@ def test_batch_full_size_c3(gbs, orc):
     gbs.sync()
     for g, p in zip(results(items, probs), probs):
         assert np.array_equal(g, exact(p))
+
+
+def test_batch_caffenet_list(gbs, orc):
+    """The reference benchmark's CaffeNet GEMM list (2u x 2u): batched, graph
+    replayed and per-call runs all equal the oracle's bit-serial product
+    (which follows the reference's gemmBitSerial)."""
+    probs = W.caffenet()
+    items = build(gbs, probs)
+    b = gbs.Batch([(l, r, d.ptr) for l, r, d in items])
+    want = [O.gemm(O.import_regular(p["lhs"], p["lb"], p["ls"]),
+                   O.import_regular(p["rhs"], p["rb"], p["rs"])) for p in probs]
+    for run in (b.run, b.run_graph, b.run_graph, b.run_per_call):
+        zero(items)
+        run()
+        gbs.sync()
+        for g, w in zip(results(items, probs), want):
+            assert np.array_equal(g, w)
