@@ -213,6 +213,7 @@ int run_plan_fmt(const MfmaOperands &a, const MfmaPlan &pl, hipStream_t s) {
     case 20: return run_plan<64, 128, 2, 128, 2, FMT, 2>(a, pl, s);
     case 21: return run_plan<64, 64, 2, 128, 3, FMT, 2>(a, pl, s);
   }
+
   return fail(GBS_EINVAL, "mfma: tile shape %d not compiled for format %d", pl.shape, FMT);
 }
 

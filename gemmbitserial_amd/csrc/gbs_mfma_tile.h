@@ -122,6 +122,7 @@ __device__ __forceinline__ int lds_chunk(int row, int ch) {
   else return ch;
 }
 
+
 // LDS row stride of a stage in format FMT: FP6 rows (96 bytes per k-step)
 // are padded by 16 bytes, which keeps 16 rows at one offset in distinct
 // banks for every stage length used here.
@@ -236,7 +237,7 @@ __device__ __forceinline__ void mfma_tile(const MfmaOperands &a, uint32_t pt, ui
   };
 
   const uint32_t nst = kend > kbeg ? (kend - kbeg) / KSB : 0;  // stages
-  if (DEPTH == 1) {
+  if constexpr (DEPTH == 1) {
     if (nst) {
       gload(gr[0], gl[0], kbeg);
       swrite(gr[0], gl[0], 0);
@@ -469,28 +470,33 @@ __device__ __forceinline__ void mfma_tile_dl(const MfmaOperands &a, uint32_t pt,
   // groups' reduction
   lds_barrier();
   if constexpr (KG > 1) {
-    // group 1 parks its sums in LDS (lane-contiguous: no bank conflicts),
-    // group 0 adds them and writes the tile
+    // groups 1.. park their sums in LDS (lane-contiguous: no bank
+    // conflicts), group 0 adds them and writes the tile
     typedef typename std::conditional<FMT == kFmtI8, int32_t, float>::type E;
-    static_assert((size_t)FP * FQ * 4 * 256 * sizeof(E) <= (size_t)NBUF * slot, "reduction");
+    constexpr int PART = FP * FQ * 4 * 256;  // elements per group
+    static_assert((size_t)(KG - 1) * PART * sizeof(E) <= (size_t)NBUF * slot, "reduction");
     E *red = reinterpret_cast<E *>(lds);
     const int at = wl * 64 + lane;
-    if (grp == 1) {
+    if (grp > 0) {
+      E *mine = red + (grp - 1) * PART;
 #pragma unroll
       for (int i = 0; i < FP; i++)
 #pragma unroll
         for (int j = 0; j < FQ; j++)
 #pragma unroll
-          for (int r = 0; r < 4; r++) red[((i * FQ + j) * 4 + r) * 256 + at] = acc[i][j][r];
+          for (int r = 0; r < 4; r++) mine[((i * FQ + j) * 4 + r) * 256 + at] = acc[i][j][r];
     }
     lds_barrier();
-    if (grp == 1) return;
+    if (grp > 0) return;
 #pragma unroll
-    for (int i = 0; i < FP; i++)
+    for (int g = 0; g < KG - 1; g++)
 #pragma unroll
-      for (int j = 0; j < FQ; j++)
+      for (int i = 0; i < FP; i++)
 #pragma unroll
-        for (int r = 0; r < 4; r++) acc[i][j][r] += red[((i * FQ + j) * 4 + r) * 256 + at];
+        for (int j = 0; j < FQ; j++)
+#pragma unroll
+          for (int r = 0; r < 4; r++)
+            acc[i][j][r] += red[g * PART + ((i * FQ + j) * 4 + r) * 256 + at];
   }
 
 #pragma unroll
