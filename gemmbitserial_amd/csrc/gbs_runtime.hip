@@ -184,7 +184,7 @@ int gbs_init(int device) {
 
 const char *gbs_last_error(void) { return t_err.c_str(); }
 
-int gbs_version(void) { return 10000; }
+int gbs_version(void) { return 10100; }
 
 int gbs_host_alloc(size_t bytes, void **out) {
   if (!out) return fail(GBS_EINVAL, "gbs_host_alloc: null out");
