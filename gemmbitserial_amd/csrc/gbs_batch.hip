@@ -22,8 +22,10 @@
 //
 // Run: a small kernel zeroes the queue counters, then per format group a
 // grid of 2 blocks x 256 CUs loops: thread 0 takes the next tile index with
-// one atomicAdd, the block finds its product by binary search over the
-// prefix sums and computes that tile with the shape's code path.  Every
+// one atomicAdd, the block finds its product in a per-tile product table
+// built at plan time (one load; launches planned per call, like the serving
+// loop's, binary-search the prefix sums instead) and computes that tile with
+// the shape's code path.  Every
 // block leaves when the index passes the group's last tile.  The whole
 // sequence can be captured once into a hipGraph and replayed.
 #include <algorithm>
@@ -40,9 +42,11 @@ struct gbs_batch {
   void *d_zero = nullptr;                  // ZeroSpan[] of split products
   uint32_t n_zero = 0;
   unsigned int *d_counter = nullptr;       // one queue counter per format
+  uint32_t *d_item_prod = nullptr;         // product of every tile, per group
   uint64_t total_tiles = 0;
   uint32_t first[3] = {0, 0, 0}, count[3] = {0, 0, 0};  // descriptor range per format
   uint64_t tiles[3] = {0, 0, 0};
+  uint64_t item0[3] = {0, 0, 0};            // group's first entry in d_item_prod
   uint32_t grid[3] = {0, 0, 0};
   gbs_workspace *ws = nullptr;             // for separate products
   hipGraphExec_t exec = nullptr;
@@ -79,7 +83,8 @@ __global__ __launch_bounds__(256) void zero_spans_kernel(const ZeroSpan *spans) 
 
 template <int FMT>
 __global__ __launch_bounds__(256, 2) void batch_kernel(const BatchProb *probs, uint32_t nprobs,
-                                                       uint64_t total, unsigned int *counter) {
+                                                       uint64_t total, unsigned int *counter,
+                                                       const uint32_t *item_prod) {
   __shared__ __attribute__((aligned(16))) int8_t lds[kMfmaLdsMax];
   __shared__ uint32_t s_item;
   for (;;) {
@@ -88,12 +93,17 @@ __global__ __launch_bounds__(256, 2) void batch_kernel(const BatchProb *probs, u
     const uint64_t item = s_item;
     __syncthreads();
     if (item >= total) break;
-    // largest p with tile_begin <= item
-    uint32_t lo = 0, hi = nprobs - 1;
-    while (lo < hi) {
-      const uint32_t mid = (lo + hi + 1) / 2;
-      if (probs[mid].tile_begin <= item) lo = mid;
-      else hi = mid - 1;
+    uint32_t lo = 0;
+    if (item_prod) {
+      lo = item_prod[item];
+    } else {
+      // largest p with tile_begin <= item
+      uint32_t hi = nprobs - 1;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) / 2;
+        if (probs[mid].tile_begin <= item) lo = mid;
+        else hi = mid - 1;
+      }
     }
     const MfmaOperands ops = probs[lo].ops;
     const MfmaPlan pl = probs[lo].plan;
@@ -149,7 +159,8 @@ int launch_views(const MfmaOperands *views, int n, void *host_desc, void *d_desc
   hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, s, d_counter);
   const uint32_t grid = (uint32_t)(tiles < 512 ? tiles : 512);
   hipLaunchKernelGGL((batch_kernel<kFmtI8>), dim3(grid), dim3(256), 0, s,
-                     (const BatchProb *)d_desc, (uint32_t)m, tiles, d_counter);
+                     (const BatchProb *)d_desc, (uint32_t)m, tiles, d_counter,
+                     (const uint32_t *)nullptr);
   GBS_HIP(hipGetLastError());
   return GBS_OK;
 }
@@ -169,15 +180,15 @@ static int batch_launch(gbs_batch *b, hipStream_t s) {
     if (b->tiles[kFmtI8])
       hipLaunchKernelGGL((batch_kernel<kFmtI8>), dim3(b->grid[kFmtI8]), dim3(256), 0, s,
                          desc + b->first[kFmtI8], b->count[kFmtI8], b->tiles[kFmtI8],
-                         b->d_counter + kFmtI8);
+                         b->d_counter + kFmtI8, b->d_item_prod + b->item0[kFmtI8]);
     if (b->tiles[kFmtF4])
       hipLaunchKernelGGL((batch_kernel<kFmtF4>), dim3(b->grid[kFmtF4]), dim3(256), 0, s,
                          desc + b->first[kFmtF4], b->count[kFmtF4], b->tiles[kFmtF4],
-                         b->d_counter + kFmtF4);
+                         b->d_counter + kFmtF4, b->d_item_prod + b->item0[kFmtF4]);
     if (b->tiles[kFmtF6])
       hipLaunchKernelGGL((batch_kernel<kFmtF6>), dim3(b->grid[kFmtF6]), dim3(256), 0, s,
                          desc + b->first[kFmtF6], b->count[kFmtF6], b->tiles[kFmtF6],
-                         b->d_counter + kFmtF6);
+                         b->d_counter + kFmtF6, b->d_item_prod + b->item0[kFmtF6]);
     GBS_HIP(hipGetLastError());
   }
   for (uint32_t i : b->separate_idx) {
@@ -276,12 +287,29 @@ int gbs_batch_create(const gbs_problem *problems, uint64_t count, gbs_batch **ou
     b->mfma_idx.push_back(it.idx);
   }
   for (int f = 0; f < 3; f++) {
+    b->item0[f] = b->total_tiles;
     b->total_tiles += b->tiles[f];
     b->grid[f] = (uint32_t)(b->tiles[f] < slots ? b->tiles[f] : slots);
   }
+  if (b->total_tiles >= (1ull << 32)) {
+    gbs_batch_destroy(b);
+    return fail(GBS_ERANGE, "gbs_batch_create: more than 2^32 tiles");
+  }
+  // product (index within its format group) of every tile
+  std::vector<uint32_t> item_prod(b->total_tiles);
+  for (int f = 0; f < 3; f++)
+    for (uint32_t k = 0; k < b->count[f]; k++) {
+      const BatchProb &d = desc[b->first[f] + k];
+      const uint64_t at = b->item0[f] + d.tile_begin;
+      std::fill(item_prod.begin() + at, item_prod.begin() + at + d.plan.tiles(), k);
+    }
   if (!desc.empty()) {
     hipError_t e = hipMalloc(&b->d_desc, desc.size() * sizeof(BatchProb));
     if (e == hipSuccess) e = hipMalloc((void **)&b->d_counter, 256);
+    if (e == hipSuccess) e = hipMalloc((void **)&b->d_item_prod, item_prod.size() * 4);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(b->d_item_prod, item_prod.data(), item_prod.size() * 4,
+                         hipMemcpyHostToDevice, s);
     if (e == hipSuccess && !zero.empty()) {
       e = hipMalloc(&b->d_zero, zero.size() * sizeof(ZeroSpan));
       if (e == hipSuccess)
@@ -318,6 +346,7 @@ int gbs_batch_destroy(gbs_batch *b) {
   if (b->graph) (void)hipGraphDestroy(b->graph);
   if (b->d_desc) (void)hipFree(b->d_desc);
   if (b->d_counter) (void)hipFree(b->d_counter);
+  if (b->d_item_prod) (void)hipFree(b->d_item_prod);
   if (b->d_zero) (void)hipFree(b->d_zero);
   if (b->ws) gbs_workspace_destroy(b->ws);
   delete b;
