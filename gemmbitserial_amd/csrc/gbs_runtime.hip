@@ -432,10 +432,15 @@ int gbs_import_quantize(gbs_matrix *m, const int32_t *src, const int32_t *thresh
   void *tmp = nullptr;
   if (!src_on_device && (bytes || tbytes)) {
     GBS_HIP(hipMalloc(&tmp, bytes + tbytes + 8));
-    if (bytes) GBS_HIP(hipMemcpyAsync(tmp, src, bytes, hipMemcpyHostToDevice, s));
-    if (tbytes)
-      GBS_HIP(hipMemcpyAsync((char *)tmp + bytes, thresholds, tbytes,
-                             hipMemcpyHostToDevice, s));
+    hipError_t e = bytes ? hipMemcpyAsync(tmp, src, bytes, hipMemcpyHostToDevice, s)
+                         : hipSuccess;
+    if (e == hipSuccess && tbytes)
+      e = hipMemcpyAsync((char *)tmp + bytes, thresholds, tbytes, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) {
+      (void)hipStreamSynchronize(s);
+      (void)hipFree(tmp);
+      return hip_fail(e, "gbs_import_quantize: staging");
+    }
     d_src = (const int32_t *)tmp;
     d_thr = (const int32_t *)((char *)tmp + bytes);
   }
@@ -677,11 +682,13 @@ int gbs_gemm_async_config(gbs_workspace *ws, const gbs_matrix *lhs, const gbs_ma
   if (shape < 0 || shape >= kMfmaShapes || splits < 1)
     return fail(GBS_EINVAL, "gbs_gemm_async_config: shape %d / splits %d out of range", shape,
                 splits);
+  if (lhs->nrows * rhs->nrows && !d_res)
+    return fail(GBS_EINVAL, "gbs_gemm_async_config: null result pointer");
   if ((rc = ensure_device())) return rc;
   hipStream_t s = stream ? (hipStream_t)stream : thread_stream();
   if ((rc = ensure_copy(const_cast<gbs_matrix *>(lhs), fmt, s, false))) return rc;
   if ((rc = ensure_copy(const_cast<gbs_matrix *>(rhs), fmt, s, false))) return rc;
-  return launch_mfma_gemm(lhs, rhs, d_res, s, fmt, shape, true, splits > 0 ? splits : 1);
+  return launch_mfma_gemm(lhs, rhs, d_res, s, fmt, shape, true, splits);
 }
 
 int gbs_tune_classes(int *n_classes, int *n_shapes) {

@@ -99,7 +99,9 @@ int gbs_matrix_create(uint64_t nbits, uint64_t nrows, uint64_t ncols,
                       int issigned, uint64_t nrows_a, uint64_t ncols_a,
                       gbs_matrix **out);
 
-/* Free a matrix.  Replaces BitSerialMatrix::dealloc (gemmbitserial.hpp:38-40). */
+/* Free a matrix.  Replaces BitSerialMatrix::dealloc (gemmbitserial.hpp:38-40).
+ * Waits for the calling thread's stream only: work the caller enqueued with
+ * this matrix on other streams (gbs_gemm_async, batches) must be complete. */
 int gbs_matrix_destroy(gbs_matrix *m);
 
 /* Device address of the plane words (layout above). */
@@ -275,7 +277,8 @@ int gbs_tune_class_of(uint64_t M, uint64_t N, uint64_t K, int *cls);
 int gbs_tune_set(const int *shapes, const int *splits, int n);
 /* Text table, one "class shape [splits [format [microseconds]]]" line per
  * class and format (0 int8, 1 FP4, 2 FP6), '#' comments.  The measured
- * times decide between FP6 and int8 copies for GBS_ALGO_AUTO. */
+ * times decide between FP6 and int8 copies for GBS_ALGO_AUTO.  The table is
+ * process-wide: set or load it while no other thread is planning a GEMM. */
 int gbs_tune_load(const char *path);
 /* One matrix-core product (algo GBS_ALGO_MFMA, _MFMA_FP4 or _MFMA_FP6)
  * with a forced tile shape and K split (the tuner's probe).  Operands must

@@ -253,6 +253,30 @@ def test_mfma_refuses_inexact(gbs):
     assert gbs.select(l, r) == "popcount"
 
 
+def test_config_entry_point_refuses_bad_arguments(gbs):
+    """The forced-shape entry point (the autotuner's) checks its arguments
+    before any launch: null result, shape and split ranges, format fit."""
+    n_classes, n_shapes = gbs.tune_classes()
+    l = gbs.from_values(np.ones((20, 128), np.int8), 1, False)
+    r = gbs.from_values(np.ones((10, 128), np.int8), 1, False)
+    ws = gbs.Workspace()
+    ws.reserve(l, r)
+    L = gbs.lib()
+    d = gbs.DeviceBuffer(4 * 20 * 10)
+    assert L.gbs_gemm_async_config(ws.h, l.h, r.h, None, gbs.MFMA, 0, 1, None) == 1
+    assert b"null result" in L.gbs_last_error()
+    for shape, splits in [(-1, 1), (n_shapes, 1), (0, 0)]:
+        assert L.gbs_gemm_async_config(ws.h, l.h, r.h, d.ptr, gbs.MFMA, shape, splits, None) == 1
+        assert b"out of range" in L.gbs_last_error()
+    assert L.gbs_gemm_async_config(ws.h, l.h, r.h, d.ptr, gbs.POPCOUNT, 0, 1, None) == 1
+    assert b"matrix-core engine" in L.gbs_last_error()
+    w = gbs.from_values(np.full((20, 128), 7, np.int8), 3, False)   # unsigned 3-bit: no FP4
+    assert L.gbs_gemm_async_config(ws.h, w.h, r.h, d.ptr, gbs.MFMA_FP4, 0, 1, None) == 5
+    gbs.check(L.gbs_gemm_async_config(ws.h, l.h, r.h, d.ptr, gbs.MFMA, 0, 1, None))
+    gbs.sync()
+    assert np.array_equal(d.to_host(np.int32, (10, 20)), np.full((10, 20), 128, np.int32))
+
+
 def test_auto_matches_both_engines(gbs, orc):
     rng = np.random.default_rng(8)
     for (M, K, N, lb, rb, ls, rs) in [(512, 2048, 512, 1, 1, True, True),
