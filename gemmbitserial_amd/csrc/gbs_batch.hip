@@ -39,6 +39,7 @@ struct gbs_batch {
   int device = 0;
   std::vector<gbs_problem> probs;          // as given
   std::vector<uint32_t> mfma_idx;          // persistent products, in run order
+  std::vector<uint8_t> mfma_fmt;           // their copy formats
   std::vector<uint32_t> separate_idx;      // popcount products
   void *d_desc = nullptr;                  // BatchProb[], grouped by format
   void *d_zero = nullptr;                  // ZeroSpan[] of split products
@@ -171,6 +172,19 @@ size_t batch_desc_bytes(int n) {
   return (size_t)(n > 0 ? n : 1) * (sizeof(BatchProb) + sizeof(ZeroSpan));
 }
 
+// Operands re-imported since the plan get their copies rebuilt before a
+// run, on the run's stream.  The plan stays valid: a copy buffer, once
+// allocated, keeps its address and layout for the matrix's lifetime.
+static int refresh_copies(gbs_batch *b, hipStream_t s) {
+  for (size_t k = 0; k < b->mfma_idx.size(); k++) {
+    const gbs_problem &p = b->probs[b->mfma_idx[k]];
+    int rc = ensure_copy(const_cast<gbs_matrix *>(p.lhs), b->mfma_fmt[k], s, false);
+    if (!rc) rc = ensure_copy(const_cast<gbs_matrix *>(p.rhs), b->mfma_fmt[k], s, false);
+    if (rc) return rc;
+  }
+  return GBS_OK;
+}
+
 static int batch_launch(gbs_batch *b, hipStream_t s) {
   if (b->total_tiles) {
     if (b->n_zero)
@@ -287,6 +301,7 @@ int gbs_batch_create(const gbs_problem *problems, uint64_t count, gbs_batch **ou
     if (d.plan.splits > 1) zero.push_back(ZeroSpan{p.d_res, (uint64_t)d.ops.M * d.ops.N});
     desc.push_back(d);
     b->mfma_idx.push_back(it.idx);
+    b->mfma_fmt.push_back((uint8_t)f);
   }
   for (int f = 0; f < 3; f++) {
     b->item0[f] = b->total_tiles;
@@ -359,7 +374,9 @@ int gbs_batch_run(gbs_batch *b, void *stream) {
   if (!b) return fail(GBS_EINVAL, "gbs_batch_run: null batch");
   int rc = ensure_device();
   if (rc) return rc;
-  return batch_launch(b, stream ? (hipStream_t)stream : thread_stream());
+  hipStream_t s = stream ? (hipStream_t)stream : thread_stream();
+  if ((rc = refresh_copies(b, s))) return rc;
+  return batch_launch(b, s);
 }
 
 int gbs_batch_run_graph(gbs_batch *b, void *stream) {
@@ -367,6 +384,7 @@ int gbs_batch_run_graph(gbs_batch *b, void *stream) {
   int rc = ensure_device();
   if (rc) return rc;
   hipStream_t s = stream ? (hipStream_t)stream : thread_stream();
+  if ((rc = refresh_copies(b, s))) return rc;  // outside the graph: only when stale
   if (!b->exec) {
     GBS_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     rc = batch_launch(b, s);

@@ -248,13 +248,17 @@ typedef struct gbs_problem {
 } gbs_problem;
 
 /* Plan a batch (validates shapes, builds the tile list and the device
- * descriptors, makes every operand's int8 copy current).  The matrices and
- * result buffers must outlive the batch; re-importing an operand requires
- * a new plan. */
+ * descriptors, makes every operand's value copy current in the narrowest
+ * format both operands of its product fit: FP4, FP6 or int8).  The
+ * matrices and result buffers must outlive the batch.  Operands may be
+ * re-imported between runs (same shapes): each run first rebuilds the
+ * copies of operands changed since, on the run's stream, outside the
+ * graph of gbs_batch_run_graph. */
 int gbs_batch_create(const gbs_problem *problems, uint64_t count, gbs_batch **out);
 int gbs_batch_destroy(gbs_batch *b);
-/* Enqueue the whole batch: one persistent launch (plus separate popcount
- * launches for products the int8 engine cannot hold exactly). */
+/* Enqueue the whole batch: one persistent launch per copy format present
+ * (plus separate popcount launches for products the matrix-core engine
+ * cannot hold exactly). */
 int gbs_batch_run(gbs_batch *b, void *stream);
 /* Same, captured into a hipGraph on the first call and replayed after. */
 int gbs_batch_run_graph(gbs_batch *b, void *stream);

@@ -120,3 +120,33 @@ def test_batch_caffenet_list(gbs, orc):
         gbs.sync()
         for g, w in zip(results(items, probs), want):
             assert np.array_equal(g, w)
+
+
+def test_batch_sees_reimported_operands(gbs, orc):
+    """An operand re-imported after the plan (same shape, new values): plain
+    and graph-replayed runs both use the new values, in every copy format."""
+    rng = np.random.default_rng(21)
+    shapes = [(96, 640, 40, 1, 1, True, True),     # FP4 copies
+              (70, 512, 24, 4, 3, True, False),    # FP6 copies
+              (50, 384, 90, 6, 2, True, True)]     # int8 copies
+    probs = []
+    for (M, K, N, lb, rb, ls, rs) in shapes:
+        probs.append(dict(M=M, K=K, N=N, lb=lb, rb=rb, ls=ls, rs=rs,
+                          lhs=W.values(rng, (M, K), lb, ls), rhs=W.values(rng, (N, K), rb, rs)))
+    items = build(gbs, probs)
+    b = gbs.Batch([(l, r, d.ptr) for l, r, d in items])
+    for run in (b.run, b.run_graph):
+        for step in range(2):
+            for (l, r, _), p in zip(items, probs):
+                p["lhs"] = W.values(rng, (p["M"], p["K"]), p["lb"], p["ls"])
+                l.import_regular(p["lhs"])
+                if step:
+                    p["rhs"] = W.values(rng, (p["N"], p["K"]), p["rb"], p["rs"])
+                    r.import_regular(p["rhs"])
+            zero(items)
+            gbs.sync()
+            run()
+            gbs.sync()
+            for g, p in zip(results(items, probs), probs):
+                assert np.array_equal(g, exact(p)), (run.__name__, step, p["lb"], p["rb"])
+    b.close()
