@@ -20,7 +20,8 @@
 //
 // Rows of different requests never mix inside a GEMM row, and integer
 // arithmetic is exact, so a request's result is the same alone or in any
-// mix.  A server is driven by one host thread.
+// mix.  A server is driven by one host thread.  Weights may be re-imported
+// between steps (same shapes); each step multiplies by the current ones.
 #include <string.h>
 
 #include <chrono>
@@ -313,7 +314,10 @@ int gbs_server_step(gbs_server *s, uint32_t *finished_now) {
                        J, units);
     GBS_HIP(hipGetLastError());
   }
-  // 2. every layer's GEMM in one persistent launch
+  // 2. every layer's GEMM in one persistent launch (weights re-imported
+  // since the last step get their int8 copies rebuilt first)
+  for (uint32_t l = 0; l < L; l++)
+    if ((rc = ensure_vals(const_cast<gbs_matrix *>(s->W[l]), st, false))) return rc;
   MfmaOperands views[kMaxLayers];
   int nv = 0;
   for (uint32_t l = 0; l < L; l++) {

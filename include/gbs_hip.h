@@ -319,8 +319,10 @@ typedef struct gbs_server_counters {
 
 /* Nanoseconds of the steady clock the server stamps requests with. */
 int gbs_clock_ns(int64_t *now);
-/* weights[l]: h_{l+1} x h_l, each must be exact on the int8 engine and must
- * not change while the server lives; shifts[l] for l < n_layers - 1. */
+/* weights[l]: h_{l+1} x h_l, each must be exact on the int8 engine and
+ * outlive the server; they may be re-imported between steps (same shapes),
+ * each step multiplies by the current values.  shifts[l] for
+ * l < n_layers - 1. */
 int gbs_server_create(const gbs_matrix *const *weights, const uint32_t *shifts,
                       uint32_t n_layers, uint32_t act_bits,
                       const gbs_server_config *cfg, gbs_server **out);

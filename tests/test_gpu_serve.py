@@ -114,3 +114,24 @@ def test_serve_full_size_mlp(gbs):
     for r in reqs:
         assert np.array_equal(got[r["id"]][0], W.mlp_reference(r["x"], Ws, shifts))
     srv.close()
+
+
+def test_serve_uses_reimported_weights(gbs):
+    """Weights re-imported between steps: requests served after the change
+    get the new weights' exact forward pass."""
+    h, Ws, shifts = small_mlp(7)
+    reqs = W.serving_requests(h[0], 4, 1000.0, seed=8, max_rows=10)
+    srv, mats = make_server(gbs, h, Ws, shifts, max_requests=4, max_rows=10)
+    srv.submit(reqs[0]["id"], reqs[0]["x"])
+    first = drain(srv, [reqs[0]["id"]])
+    assert np.array_equal(first[reqs[0]["id"]][0], W.mlp_reference(reqs[0]["x"], Ws, shifts))
+    rng = np.random.default_rng(9)
+    Ws2 = [W.values(rng, w.shape, 4, True) for w in Ws]
+    for m, w in zip(mats, Ws2):
+        m.import_regular(w)
+    for r in reqs[1:]:
+        srv.submit(r["id"], r["x"])
+    later = drain(srv, [r["id"] for r in reqs[1:]])
+    for r in reqs[1:]:
+        assert np.array_equal(later[r["id"]][0], W.mlp_reference(r["x"], Ws2, shifts)), r["id"]
+    srv.close()
