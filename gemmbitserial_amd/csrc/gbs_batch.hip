@@ -125,12 +125,26 @@ __global__ __launch_bounds__(256, 2) void batch_kernel(const BatchProb *probs, u
 // least padding for its rhs row count (the tuning table's shapes are
 // chosen to fill the chip with one product, which a shared queue does not
 // need), and deep products cut into pieces of at most 4 KiB (nominal) of K
-// so that no item is much longer than the rest.
-static MfmaPlan batch_plan(uint32_t M, uint32_t N, uint32_t kbytes, int fmt) {
+// so that no item is much longer than the rest -- or into up to
+// min_pieces pieces of at least 1 KiB, for products that would otherwise
+// dominate the queue.
+static MfmaPlan batch_plan(uint32_t M, uint32_t N, uint32_t kbytes, int fmt,
+                           uint32_t min_pieces = 1) {
   const int shape = batch_shape(N <= 16 ? 3 : N <= 32 ? 2 : 4, fmt);
   const uint32_t units = kbytes / kMfmaKS;
-  const uint32_t want = (units + 15) / 16;
+  uint32_t want = (units + 15) / 16;
+  const uint32_t most = units / 4 ? units / 4 : 1;  // pieces of at least 1 KiB
+  if (want < min_pieces) want = min_pieces < most ? min_pieces : most;
   return mfma_plan(M, N, kbytes, fmt, shape, (int)(want ? want : 1));
+}
+
+static int step_bytes_of(int fmt) { return fmt == kFmtF6 ? step_bytes<kFmtF6>() : 64; }
+
+// Operand bytes one work item of a plan streams (its tile's rows times its
+// K piece, in the copy format's real bytes).
+static double item_bytes(const MfmaPlan &pl) {
+  const MfmaTileShape sh = kMfmaShape[pl.shape];
+  return (double)(sh.P + sh.Q) * pl.kchunk * step_bytes_of(pl.fmt) / 64.0;
 }
 
 // Plan and launch products given as int8 operand views (the serving loop's
@@ -278,6 +292,23 @@ int gbs_batch_create(const gbs_problem *problems, uint64_t count, gbs_batch **ou
       items.push_back({(uint32_t)i, (double)sh.P * sh.Q * (double)pl.kchunk, pl, ops});
     } else {
       b->separate_idx.push_back((uint32_t)i);
+    }
+  }
+  // Balance: a product whose items each stream more than 1/1024 of the
+  // whole batch's bytes (two per queue slot) would leave its last items
+  // running alone -- the fully connected GEMVs of a layer list are the
+  // case; it is re-planned with proportionally more K pieces.
+  double total = 0;
+  for (const Item &it : items) total += item_bytes(it.plan) * (double)it.plan.tiles();
+  const double target = total / 1024;
+  for (Item &it : items) {
+    const double per_tile = item_bytes(it.plan) * it.plan.splits;  // one tile, all of K
+    if (target > 0 && item_bytes(it.plan) > target) {
+      const MfmaPlan pl = batch_plan(it.ops.M, it.ops.N, it.ops.kbytes, it.plan.fmt,
+                                     (uint32_t)std::min(per_tile / target + 1, 1e6));
+      const MfmaTileShape sh = kMfmaShape[pl.shape];
+      it.plan = pl;
+      it.item_cost = (double)sh.P * sh.Q * (double)pl.kchunk;
     }
   }
   // grouped by format; inside a group the longest items first: they start
