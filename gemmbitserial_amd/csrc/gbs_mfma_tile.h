@@ -57,6 +57,19 @@ struct MfmaFrag<kFmtF6> { v8i x; };  // lanes 0..5 used, loaded in place (no reg
 template <int FMT>
 using MfmaAcc = typename std::conditional<FMT == kFmtI8, v4i, v4f>::type;
 
+// LDS offset of the last 8 bytes of an FP6 fragment, hidden from the
+// optimiser when APART: otherwise it pairs these reads across fragments
+// into ds_read2 instructions whose halves must then be moved into each
+// fragment's 6-register operand (measured: the standalone FP6 tiles'
+// LDS-and-MFMA time drops by a fifth without the pairing; inside the
+// persistent kernel, whose registers are nearly all taken, the pairing
+// measures 3 % faster and is kept there).
+template <bool APART>
+__device__ __forceinline__ int f6_tail_offset(int off) {
+  if constexpr (APART) asm volatile("" : "+v"(off));
+  return off;
+}
+
 template <int FMT>
 __device__ __forceinline__ MfmaAcc<FMT> mfma_op(const MfmaFrag<FMT> &a, const MfmaFrag<FMT> &b,
                                                 const MfmaAcc<FMT> &c) {
@@ -139,7 +152,7 @@ constexpr size_t kLdsPerCU = 160 * 1024;
 // persistent kernel: the largest carried shape (64 x 64, three FP6 stages)
 constexpr size_t kMfmaLdsMax = (size_t)3 * 128 * 192;
 
-template <int P, int Q, int WP, int KSB, int DEPTH, int FMT = kFmtI8>
+template <int P, int Q, int WP, int KSB, int DEPTH, int FMT = kFmtI8, bool TAILS_APART = true>
 __device__ __forceinline__ void mfma_tile(const MfmaOperands &a, uint32_t pt, uint32_t qt,
                                           uint32_t kbeg, uint32_t kend, bool atomic,
                                           int8_t *lds) {
@@ -203,7 +216,7 @@ __device__ __forceinline__ void mfma_tile(const MfmaOperands &a, uint32_t pt, ui
     if constexpr (FMT == kFmtF6) {
       const int8_t *p = base + row * LDR + kk * 96;
       f.x.s0123 = *reinterpret_cast<const v4i *>(p + 16 * g);
-      f.x.s45 = *reinterpret_cast<const v2i *>(p + 64 + 8 * g);
+      f.x.s45 = *reinterpret_cast<const v2i *>(p + f6_tail_offset<TAILS_APART>(64 + 8 * g));
       f.x.s67 = v2i{0, 0};
     } else {
       f.h = *reinterpret_cast<const v4i *>(base + row * LDR +
@@ -338,7 +351,8 @@ __device__ __forceinline__ int dl_chunk(int row, int slot) {  // logical chunk o
   if constexpr (FMT == kFmtF6) return (slot + CPR - ((row >> 2) & 3)) % CPR;
   else return lds_chunk<KSB>(row, slot);
 }
-template <int P, int Q, int WP, int KSB, int NBUF, int FMT = kFmtI8, int KG = 1>
+template <int P, int Q, int WP, int KSB, int NBUF, int FMT = kFmtI8, int KG = 1,
+          bool TAILS_APART = true>
 __device__ __forceinline__ void mfma_tile_dl(const MfmaOperands &a, uint32_t pt, uint32_t qt,
                                              uint32_t kbeg, uint32_t kend, bool atomic,
                                              int8_t *lds) {
@@ -409,7 +423,8 @@ __device__ __forceinline__ void mfma_tile_dl(const MfmaOperands &a, uint32_t pt,
     if constexpr (FMT == kFmtF6) {
       f.x.s0123 = *reinterpret_cast<const v4i *>(p + dl_slot<KSB, FMT>(row, kk * 6 + g) * 16);
       f.x.s45 = *reinterpret_cast<const v2i *>(
-          p + dl_slot<KSB, FMT>(row, kk * 6 + 4 + (g >> 1)) * 16 + (g & 1) * 8);
+          p + f6_tail_offset<TAILS_APART>(dl_slot<KSB, FMT>(row, kk * 6 + 4 + (g >> 1)) * 16 +
+                                           (g & 1) * 8));
       f.x.s67 = v2i{0, 0};
     } else {
       f.h = *reinterpret_cast<const v4i *>(p + dl_slot<KSB, FMT>(row, kk * 4 + g) * 16);
@@ -692,8 +707,8 @@ __device__ __forceinline__ void mfma_tile_by_shape(int shape, const MfmaOperands
                                                    uint32_t pt, uint32_t qt, uint32_t kbeg,
                                                    uint32_t kend, bool atomic, int8_t *lds) {
   if constexpr (FMT == kFmtF6) {
-    if (shape == 2) mfma_tile<32, 256, 1, 64, 2, FMT>(a, pt, qt, kbeg, kend, atomic, lds);
-    else mfma_tile_dl<64, 64, 2, 128, 3, FMT>(a, pt, qt, kbeg, kend, atomic, lds);
+    if (shape == 2) mfma_tile<32, 256, 1, 64, 2, FMT, false>(a, pt, qt, kbeg, kend, atomic, lds);
+    else mfma_tile_dl<64, 64, 2, 128, 3, FMT, 1, false>(a, pt, qt, kbeg, kend, atomic, lds);
   } else {
     switch (shape) {
       case 0:
