@@ -147,6 +147,21 @@ static double item_bytes(const MfmaPlan &pl) {
   return (double)(sh.P + sh.Q) * pl.kchunk * step_bytes_of(pl.fmt) / 64.0;
 }
 
+// Balance: a product whose items each stream more than 1/1024 of the whole
+// launch's bytes (two per queue slot) would leave its last items running
+// alone -- fully connected GEMVs are the case; it is re-planned with
+// proportionally more K pieces (of at least 1 KiB).
+static void rebalance(BatchProb *d, int n, double total_bytes) {
+  const double target = total_bytes / 1024;
+  for (int i = 0; i < n; i++) {
+    const MfmaPlan &pl = d[i].plan;
+    if (target <= 0 || item_bytes(pl) <= target) continue;
+    const double per_tile = item_bytes(pl) * pl.splits;  // one tile, all of K
+    d[i].plan = batch_plan(d[i].ops.M, d[i].ops.N, d[i].ops.kbytes, pl.fmt,
+                           (uint32_t)std::min(per_tile / target + 1, 1e6));
+  }
+}
+
 // Plan and launch products given as int8 operand views (the serving loop's
 // per-step batch).  host_desc must be pinned and hold batch_desc_bytes(n);
 // it is reused only after the stream has passed this launch.
@@ -154,18 +169,28 @@ int launch_views(const MfmaOperands *views, int n, void *host_desc, void *d_desc
                  unsigned int *d_counter, hipStream_t s) {
   BatchProb *h = (BatchProb *)host_desc;
   ZeroSpan *hz = (ZeroSpan *)(h + (n > 0 ? n : 1));
-  uint64_t tiles = 0;
   int m = 0, nz = 0;
+  double total = 0;
   for (int i = 0; i < n; i++) {
     const MfmaOperands &v = views[i];
     if (v.M == 0 || v.N == 0) continue;
     BatchProb d{};
     d.ops = v;
     d.plan = batch_plan(v.M, v.N, v.kbytes, kFmtI8);
-    d.tile_begin = tiles;
-    tiles += d.plan.tiles();
-    if (d.plan.splits > 1) hz[nz++] = ZeroSpan{v.res, (uint64_t)v.M * v.N};
+    total += item_bytes(d.plan) * (double)d.plan.tiles();
     h[m++] = d;
+  }
+  // a step of few GEMV-like products would otherwise run on a few CUs:
+  // the same re-split as a planned batch, then longest items first
+  rebalance(h, m, total);
+  std::sort(h, h + m, [](const BatchProb &x, const BatchProb &y) {
+    return item_bytes(x.plan) > item_bytes(y.plan);
+  });
+  uint64_t tiles = 0;
+  for (int i = 0; i < m; i++) {
+    h[i].tile_begin = tiles;
+    tiles += h[i].plan.tiles();
+    if (h[i].plan.splits > 1) hz[nz++] = ZeroSpan{h[i].ops.res, (uint64_t)h[i].ops.M * h[i].ops.N};
   }
   if (tiles == 0) return GBS_OK;
   GBS_HIP(hipMemcpyAsync(d_desc, h, batch_desc_bytes(n), hipMemcpyHostToDevice, s));
@@ -294,21 +319,20 @@ int gbs_batch_create(const gbs_problem *problems, uint64_t count, gbs_batch **ou
       b->separate_idx.push_back((uint32_t)i);
     }
   }
-  // Balance: a product whose items each stream more than 1/1024 of the
-  // whole batch's bytes (two per queue slot) would leave its last items
-  // running alone -- the fully connected GEMVs of a layer list are the
-  // case; it is re-planned with proportionally more K pieces.
-  double total = 0;
-  for (const Item &it : items) total += item_bytes(it.plan) * (double)it.plan.tiles();
-  const double target = total / 1024;
-  for (Item &it : items) {
-    const double per_tile = item_bytes(it.plan) * it.plan.splits;  // one tile, all of K
-    if (target > 0 && item_bytes(it.plan) > target) {
-      const MfmaPlan pl = batch_plan(it.ops.M, it.ops.N, it.ops.kbytes, it.plan.fmt,
-                                     (uint32_t)std::min(per_tile / target + 1, 1e6));
-      const MfmaTileShape sh = kMfmaShape[pl.shape];
-      it.plan = pl;
-      it.item_cost = (double)sh.P * sh.Q * (double)pl.kchunk;
+  // re-split products whose items would dominate the queue (rebalance)
+  {
+    double total = 0;
+    std::vector<BatchProb> tmp(items.size());
+    for (size_t i = 0; i < items.size(); i++) {
+      tmp[i].ops = items[i].ops;
+      tmp[i].plan = items[i].plan;
+      total += item_bytes(items[i].plan) * (double)items[i].plan.tiles();
+    }
+    rebalance(tmp.data(), (int)tmp.size(), total);
+    for (size_t i = 0; i < items.size(); i++) {
+      const MfmaTileShape sh = kMfmaShape[tmp[i].plan.shape];
+      items[i].plan = tmp[i].plan;
+      items[i].item_cost = (double)sh.P * sh.Q * (double)tmp[i].plan.kchunk;
     }
   }
   // grouped by format; inside a group the longest items first: they start
