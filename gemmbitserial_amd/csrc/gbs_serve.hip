@@ -18,6 +18,14 @@
 //      pool l+1, where cohort l sits next step;
 //   4. the last layer's int32 rows are copied back; those requests finish.
 //
+// A call of gbs_server_step enqueues one step and then completes the step
+// before it (waits for its event, hands its last-layer rows to their
+// requests): the host admits and plans step t+1 while the GPU still runs
+// step t.  The pinned staging buffers (inputs, descriptors, outputs) are
+// doubled for that; device buffers need no doubling, the stream orders
+// their reuse.  A request therefore becomes ready one call after the step
+// that computed it.
+//
 // Rows of different requests never mix inside a GEMM row, and integer
 // arithmetic is exact, so a request's result is the same alone or in any
 // mix.  A server is driven by one host thread.  Weights may be re-imported
@@ -103,10 +111,14 @@ struct gbs_server {
   std::vector<uint32_t> h, shift;
   std::vector<gbs_matrix *> pool;
   std::vector<int32_t *> acc;
-  void *h_desc = nullptr, *d_desc = nullptr;
+  void *h_desc = nullptr, *d_desc = nullptr;  // h_desc: two halves
   unsigned int *d_counter = nullptr;
-  uint8_t *h_in = nullptr, *d_in = nullptr;
-  int32_t *h_out = nullptr;
+  uint8_t *h_in = nullptr, *d_in = nullptr;     // h_in: two halves
+  int32_t *h_out = nullptr;                     // two halves
+  size_t desc_half = 0, in_half = 0, out_half = 0;  // bytes / elements per half
+  hipEvent_t ev[2] = {nullptr, nullptr};        // end of the step using each half
+  std::vector<Req *> completing[2];             // requests whose rows land in h_out half
+  bool pending[2] = {false, false};
   uint64_t bytes_reserved = 0;
   std::deque<Req *> waiting;
   std::vector<std::vector<Req *>> cohort;  // cohort[l]: requests at layer l
@@ -129,6 +141,10 @@ static void server_free(gbs_server *s) {
   if (s->h_desc) (void)hipHostFree(s->h_desc);
   if (s->h_in) (void)hipHostFree(s->h_in);
   if (s->h_out) (void)hipHostFree(s->h_out);
+  for (int k = 0; k < 2; k++) {
+    if (s->ev[k]) (void)hipEventDestroy(s->ev[k]);
+    for (Req *r : s->completing[k]) delete r;
+  }
   for (Req *r : s->waiting) delete r;
   for (auto &c : s->cohort)
     for (Req *r : c) delete r;
@@ -239,9 +255,14 @@ int gbs_server_create(const gbs_matrix *const *weights, const uint32_t *shifts,
     hipError_t e = hipMalloc(&s->d_desc, batch_desc_bytes(kMaxLayers));
     if (e == hipSuccess) e = hipMalloc((void **)&s->d_counter, 256);
     if (e == hipSuccess) e = hipMalloc((void **)&s->d_in, R * s->h[0] + 16);
-    if (e == hipSuccess) e = hipHostMalloc(&s->h_desc, batch_desc_bytes(kMaxLayers));
-    if (e == hipSuccess) e = hipHostMalloc((void **)&s->h_in, R * s->h[0] + 16);
-    if (e == hipSuccess) e = hipHostMalloc((void **)&s->h_out, R * s->h[n_layers] * 4ull + 16);
+    s->desc_half = align_up(batch_desc_bytes(kMaxLayers), 256);
+    s->in_half = align_up(R * s->h[0] + 16, 256);
+    s->out_half = align_up(R * s->h[n_layers] + 4, 64);  // int32 elements
+    if (e == hipSuccess) e = hipHostMalloc(&s->h_desc, 2 * s->desc_half);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&s->h_in, 2 * s->in_half);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&s->h_out, 2 * s->out_half * 4);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&s->ev[0], hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&s->ev[1], hipEventDisableTiming);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) rc = hip_fail(e, "gbs_server_create: buffers");
   }
@@ -284,6 +305,10 @@ int gbs_server_step(gbs_server *s, uint32_t *finished_now) {
   if (rc) return rc;
   hipStream_t st = thread_stream();
   const uint32_t L = s->L;
+  const int cur = (int)(s->steps & 1), prev = cur ^ 1;
+  uint8_t *h_in = s->h_in + (size_t)cur * s->in_half;
+  void *h_desc = (char *)s->h_desc + (size_t)cur * s->desc_half;
+  int32_t *h_out = s->h_out + (size_t)cur * s->out_half;
   // Between steps cohort[l] holds the requests whose next GEMM is layer l
   // (their activations sit in pool l); cohort[0] is empty.
   // 1. admission into layer 0
@@ -295,13 +320,13 @@ int gbs_server_step(gbs_server *s, uint32_t *finished_now) {
     if (n0 + r->rows > s->R || in_flight + s->cohort[0].size() + 1 > s->max_requests) break;
     s->waiting.pop_front();
     r->off = n0;
-    std::copy(r->x.begin(), r->x.end(), s->h_in + (size_t)n0 * s->h[0]);
+    std::copy(r->x.begin(), r->x.end(), h_in + (size_t)n0 * s->h[0]);
     n0 += r->rows;
     s->cohort[0].push_back(r);
   }
   s->cohort_rows[0] = n0;
   if (n0) {
-    GBS_HIP(hipMemcpyAsync(s->d_in, s->h_in, (size_t)n0 * s->h[0], hipMemcpyHostToDevice, st));
+    GBS_HIP(hipMemcpyAsync(s->d_in, h_in, (size_t)n0 * s->h[0], hipMemcpyHostToDevice, st));
     ActJobs J{};
     J.nbits = s->act_bits;
     J.maxv = (1u << s->act_bits) - 1;
@@ -335,7 +360,7 @@ int gbs_server_step(gbs_server *s, uint32_t *finished_now) {
     views[nv++] = v;
     s->ops_done += 2.0 * s->h[l + 1] * s->h[l] * n;
   }
-  if (nv && (rc = launch_views(views, nv, s->h_desc, s->d_desc, s->d_counter, st))) return rc;
+  if (nv && (rc = launch_views(views, nv, h_desc, s->d_desc, s->d_counter, st))) return rc;
   // 3. requantize + pack into the next layer's pool
   ActJobs J{};
   J.nbits = s->act_bits;
@@ -354,24 +379,16 @@ int gbs_server_step(gbs_server *s, uint32_t *finished_now) {
                        J, units);
     GBS_HIP(hipGetLastError());
   }
-  // 4. last layer's rows back to the host
+  // 4. last layer's rows back to the host (read when this step's event has
+  // passed, in the next call)
   const uint32_t nl = s->cohort_rows[L - 1];
   if (nl)
-    GBS_HIP(hipMemcpyAsync(s->h_out, s->acc[L - 1], (size_t)nl * s->h[L] * 4,
+    GBS_HIP(hipMemcpyAsync(h_out, s->acc[L - 1], (size_t)nl * s->h[L] * 4,
                            hipMemcpyDeviceToHost, st));
-  GBS_HIP(hipStreamSynchronize(st));
-  const int64_t t = now_ns();
-  uint32_t fin = 0;
-  // the cohort at the last layer is complete now
-  for (Req *r : s->cohort[L - 1]) {
-    r->y.assign(s->h_out + (size_t)r->off * s->h[L],
-                s->h_out + (size_t)(r->off + r->rows) * s->h[L]);
-    r->done = t;
-    s->finished[r->id] = r;
-    s->done_requests++;
-    s->done_rows += r->rows;
-    fin++;
-  }
+  GBS_HIP(hipEventRecord(s->ev[cur], st));
+  s->completing[cur].swap(s->cohort[L - 1]);
+  s->cohort[L - 1].clear();
+  s->pending[cur] = true;
   // every other cohort moves one layer on
   for (uint32_t l = L - 1; l > 0; l--) {
     s->cohort[l].swap(s->cohort[l - 1]);
@@ -380,6 +397,23 @@ int gbs_server_step(gbs_server *s, uint32_t *finished_now) {
   s->cohort[0].clear();
   s->cohort_rows[0] = 0;
   s->steps++;
+  // 5. complete the previous step
+  uint32_t fin = 0;
+  if (s->pending[prev]) {
+    GBS_HIP(hipEventSynchronize(s->ev[prev]));
+    const int64_t t = now_ns();
+    const int32_t *out = s->h_out + (size_t)prev * s->out_half;
+    for (Req *r : s->completing[prev]) {
+      r->y.assign(out + (size_t)r->off * s->h[L], out + (size_t)(r->off + r->rows) * s->h[L]);
+      r->done = t;
+      s->finished[r->id] = r;
+      s->done_requests++;
+      s->done_rows += r->rows;
+      fin++;
+    }
+    s->completing[prev].clear();
+    s->pending[prev] = false;
+  }
   if (finished_now) *finished_now = fin;
   return GBS_OK;
 }
@@ -419,6 +453,7 @@ int gbs_server_stats(const gbs_server *s, gbs_server_counters *st) {
   st->waiting = (uint32_t)s->waiting.size();
   uint32_t act = 0;
   for (const auto &c : s->cohort) act += (uint32_t)c.size();
+  for (const auto &c : s->completing) act += (uint32_t)c.size();
   st->in_flight = act;
   return GBS_OK;
 }

@@ -331,8 +331,11 @@ int gbs_server_destroy(gbs_server *s);
  * arrival_ns (gbs_clock_ns time base) 0 = now.  Copies x. */
 int gbs_server_submit(gbs_server *s, uint64_t id, const uint8_t *x, uint32_t rows,
                       int64_t arrival_ns);
-/* Admit, compute one layer for every request in flight, finish the ones
- * leaving the last layer.  Synchronous. */
+/* Enqueue one step (admit waiting requests, compute one layer for every
+ * request in flight) and complete the previous step: wait for it and make
+ * the requests that left its last layer ready (*finished_now of them).  The
+ * host thus plans a step while the GPU runs the one before; a request is
+ * ready one call after the step that computed it. */
 int gbs_server_step(gbs_server *s, uint32_t *finished_now);
 /* Collect a finished request: *ready = 0 while it is not done.  out holds
  * rows x h_L int32 (the last layer's accumulators). */
