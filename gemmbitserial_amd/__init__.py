@@ -299,9 +299,14 @@ class DeviceBuffer:
             raise GbsError(2, f"hipMalloc({nbytes}) failed: {rc}")
         self.ptr = p.value
 
+    # The copies below run on the null stream, which the library's
+    # non-blocking streams do not order against: each waits for the device
+    # first (work already enqueued anywhere) and returns when it is done.
     def to_host(self, dtype, shape):
         out = np.zeros(shape, dtype)
-        rc = self.hip().hipMemcpy(_ptr(out), C.c_void_p(self.ptr), out.nbytes, 2)
+        rc = self.hip().hipDeviceSynchronize()
+        if rc == 0:
+            rc = self.hip().hipMemcpy(_ptr(out), C.c_void_p(self.ptr), out.nbytes, 2)
         if rc != 0:
             raise GbsError(3, f"hipMemcpy D2H failed: {rc}")
         return out
@@ -310,18 +315,30 @@ class DeviceBuffer:
         a = np.ascontiguousarray(a)
         if a.nbytes > self.nbytes:
             raise ValueError("array larger than the buffer")
-        rc = self.hip().hipMemcpy(C.c_void_p(self.ptr), _ptr(a), a.nbytes, 1)
+        rc = self.hip().hipDeviceSynchronize()
+        if rc == 0:
+            rc = self.hip().hipMemcpy(C.c_void_p(self.ptr), _ptr(a), a.nbytes, 1)
+        if rc == 0:
+            rc = self.hip().hipDeviceSynchronize()
         if rc != 0:
             raise GbsError(3, f"hipMemcpy H2D failed: {rc}")
 
     def copy_from(self, other):
-        rc = self.hip().hipMemcpy(C.c_void_p(self.ptr), C.c_void_p(other.ptr),
-                                  min(self.nbytes, other.nbytes), 3)
+        rc = self.hip().hipDeviceSynchronize()
+        if rc == 0:
+            rc = self.hip().hipMemcpy(C.c_void_p(self.ptr), C.c_void_p(other.ptr),
+                                      min(self.nbytes, other.nbytes), 3)
+        if rc == 0:
+            rc = self.hip().hipDeviceSynchronize()
         if rc != 0:
             raise GbsError(3, f"hipMemcpy D2D failed: {rc}")
 
     def fill_zero(self):
-        rc = self.hip().hipMemset(C.c_void_p(self.ptr), 0, max(self.nbytes, 8))
+        rc = self.hip().hipDeviceSynchronize()
+        if rc == 0:
+            rc = self.hip().hipMemset(C.c_void_p(self.ptr), 0, max(self.nbytes, 8))
+        if rc == 0:
+            rc = self.hip().hipDeviceSynchronize()
         if rc != 0:
             raise GbsError(3, f"hipMemset failed: {rc}")
 

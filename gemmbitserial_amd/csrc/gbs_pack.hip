@@ -1,10 +1,12 @@
 // gbs_pack.hip -- regular <-> bit-serial conversion kernels for gfx950.
 //
-//   pack (row-major)  one thread per 16 consecutive columns of a row (a
-//                     quarter of a plane word): one 16-byte vector load per
-//                     16 bytes of source when rows are 16-byte aligned,
-//                     quantize, and per plane one 16-bit store of those 16
-//                     bits -- the four threads of a word write its four
+//   pack (row-major)  one-byte elements in 16-byte aligned rows: one
+//                     thread per plane word, four 16-byte loads, bits
+//                     gathered with u8 dot products, one 8-byte store per
+//                     plane.  Other element types or alignments: one thread
+//                     per 16 consecutive columns of a row (a quarter of a
+//                     plane word), quantize, and per plane one 16-bit store
+//                     -- the four threads of a word write its four
 //                     quarters, so a wave stores 128 contiguous bytes per
 //                     plane.  Threads past the real matrix write the zero
 //                     padding.  (The threshold import keeps a ballot kernel:
@@ -200,6 +202,74 @@ __global__ __launch_bounds__(256) void pack_cols_kernel(
   }
 }
 
+// --- row-major pack of one-byte elements, one plane word per thread ------
+// Rows start 16-byte aligned and hold a multiple of 16 bytes.  A thread
+// loads its word's 64 bytes with four 16-byte loads (the 36 threads of a
+// 2304-column row read it contiguously); the codes are the bytes themselves
+// (bipolar: 1 where the element is > 0), and plane b's bits are bit b of
+// each byte, gathered four bytes at a time by a u8 dot product with
+// weights 1, 2, 4, 8 (16, 32, 64, 128 for the next four bytes).  One 8-byte
+// store per plane.  A row's last, partial word and the padding rows take
+// the element loop.
+template <typename T, bool RAW>
+__global__ __launch_bounds__(256) void pack_bytes64_kernel(
+    const T *src, uint64_t nrows, uint64_t ncols, uint64_t nrows_a, uint64_t wpr,
+    uint64_t W, uint32_t nbits, bool bipolar, uint64_t *planes) {
+  static_assert(sizeof(T) == 1, "one-byte elements");
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrows_a * wpr) return;
+  const uint64_t r = i / wpr, w = i - r * wpr, c0 = w * 64;
+  uint64_t out[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  if (r < nrows && c0 + 64 <= ncols) {
+    const uint4 *p = reinterpret_cast<const uint4 *>(src + r * ncols + c0);
+    uint32_t x[16];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const uint4 q = p[v];
+      x[4 * v] = q.x, x[4 * v + 1] = q.y, x[4 * v + 2] = q.z, x[4 * v + 3] = q.w;
+    }
+    if (!RAW && bipolar) {
+#pragma unroll
+      for (int d = 0; d < 16; d++) {
+        // bit 7 of each byte: byte != 0 (and, for signed bytes, >= 0)
+        uint32_t nz = ((x[d] & 0x7f7f7f7fu) + 0x7f7f7f7fu) | x[d];
+        if (T(-1) < T(0)) nz &= ~x[d];
+        x[d] = (nz >> 7) & 0x01010101u;
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      if ((uint32_t)b >= nbits) break;
+      const uint32_t m = 0x01010101u << b;
+      uint32_t half[2];
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        uint32_t acc = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          uint32_t part = __builtin_amdgcn_udot4(x[8 * h + 2 * k] & m, 0x08040201u, 0u, false);
+          part = __builtin_amdgcn_udot4(x[8 * h + 2 * k + 1] & m, 0x80402010u, part, false);
+          acc |= (part >> b) << (8 * k);
+        }
+        half[h] = acc;
+      }
+      out[b] = (uint64_t)half[0] | ((uint64_t)half[1] << 32);
+    }
+  } else if (r < nrows && c0 < ncols) {
+    const T *row = src + r * ncols;
+    const uint32_t jmax = (uint32_t)(ncols - c0 < 64 ? ncols - c0 : 64);
+    for (uint32_t j = 0; j < jmax; j++) {
+      const uint32_t code = CodeOf<T, RAW>::get(row[c0 + j], bipolar);
+#pragma unroll
+      for (int b = 0; b < 8; b++)
+        if ((uint32_t)b < nbits) out[b] |= (uint64_t)((code >> b) & 1u) << j;
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < 8; b++)
+    if ((uint32_t)b < nbits) planes[(uint64_t)b * W + r * wpr + w] = out[b];
+}
+
 template <typename T, bool RAW>
 int pack_typed(const gbs_matrix *m, const void *d_src, bool col_major,
                hipStream_t s) {
@@ -210,6 +280,16 @@ int pack_typed(const gbs_matrix *m, const void *d_src, bool col_major,
     const uint64_t n = m->nrows_a * m->wpr() * 4;
     const bool vec = ((uintptr_t)d_src % 16 == 0) && (m->ncols * sizeof(T)) % 16 == 0;
     const dim3 grid((uint32_t)((n + 255) / 256));
+    if constexpr (sizeof(T) == 1) {
+      if (vec) {
+        const uint64_t words = m->nrows_a * m->wpr();
+        hipLaunchKernelGGL((pack_bytes64_kernel<T, RAW>), dim3((uint32_t)((words + 255) / 256)),
+                           dim3(256), 0, s, (const T *)d_src, m->nrows, m->ncols, m->nrows_a,
+                           m->wpr(), W, nb, m->bipolar(), m->d_planes);
+        GBS_HIP(hipGetLastError());
+        return GBS_OK;
+      }
+    }
     if (vec)
       hipLaunchKernelGGL((pack_rows16_kernel<T, RAW, true>), grid, dim3(256), 0, s,
                          (const T *)d_src, m->nrows, m->ncols, m->nrows_a, m->wpr(), W, nb,

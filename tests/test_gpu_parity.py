@@ -50,6 +50,30 @@ def test_pack_dtypes(gbs, orc, dtype):
         assert np.array_equal(got, O.import_regular(vals, nbits, issigned).planes)
 
 
+@pytest.mark.parametrize("dtype", [np.int8, np.uint8])
+def test_pack_byte_rows_vector_path(gbs, orc, dtype):
+    """Rows of one-byte elements that start 16-byte aligned and hold a
+    multiple of 16 bytes take the vector path (16 codes per thread, bits
+    gathered by u8 dot products): every encoding, the raw overload, zeros,
+    extremes and a ragged last quarter in a second matrix."""
+    rng = np.random.default_rng(77)
+    info = np.iinfo(dtype)
+    for (r, c) in [(33, 256), (7, 1024 + 16)]:
+        vals = rng.integers(info.min, info.max + 1, (r, c)).astype(dtype)
+        vals[0, :16] = 0
+        vals[1, :16] = info.max
+        vals[2, :16] = info.min
+        for nbits, issigned in [(1, True), (1, False), (2, True), (3, False), (5, True), (8, False)]:
+            say("byte rows", dtype, r, c, nbits, issigned)
+            m = gbs.DeviceMatrix(nbits, r, c, issigned)
+            got = m.import_regular(vals, want_planes=True)
+            assert np.array_equal(got, O.import_regular(vals, nbits, issigned).planes)
+            if dtype == np.uint8:
+                got = m.import_regular(vals, raw_u8=True, want_planes=True)
+                assert np.array_equal(got, O.import_regular(vals, nbits, issigned,
+                                                            raw_u8=True).planes)
+
+
 def test_pack_raw_uint8(gbs, orc):
     vals = np.random.default_rng(3).integers(0, 256, (11, 90)).astype(np.uint8)
     for nbits, issigned in [(1, True), (2, False), (3, True)]:
