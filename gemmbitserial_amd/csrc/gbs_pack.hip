@@ -345,21 +345,46 @@ __global__ __launch_bounds__(256) void pack_im2col_kernel(
     uint32_t ch = (uint32_t)(c0 / (g.KH * g.KW));
     const uint32_t kr = (uint32_t)(c0 - (uint64_t)ch * g.KH * g.KW);
     uint32_t ky = kr / g.KW, kx = kr - ky * g.KW;
-    for (uint32_t j = 0; j < jmax; j++) {
-      const int iy = iy0 + (int)(ky * g.dh), ix = ix0 + (int)(kx * g.dw);
-      T e = (T)0;
-      if ((uint32_t)iy < g.H && (uint32_t)ix < g.W)
-        e = img[(ch * g.H + (uint32_t)iy) * g.W + (uint32_t)ix];
-      const uint32_t code = CodeOf<T, RAW>::get(e, bipolar);
+    // G columns at a time: G loads issued before any is used (an element
+    // outside the input loads element 0 of the image and is replaced by
+    // 0), then per plane the G bits are gathered from the codes, four
+    // bytes per u8 dot product
+    constexpr int G = 16;  // columns whose loads are in flight together (32: no faster)
+    for (uint32_t j0 = 0; j0 < jmax; j0 += G) {
+      T e[G];
+      bool ok[G];
 #pragma unroll
-      for (int b = 0; b < 8; b++)
-        if ((uint32_t)b < nbits) out[b] |= (uint64_t)((code >> b) & 1u) << j;
-      if (++kx == g.KW) {
-        kx = 0;
-        if (++ky == g.KH) {
-          ky = 0;
-          ++ch;
+      for (int u = 0; u < G; u++) {
+        const int iy = iy0 + (int)(ky * g.dh), ix = ix0 + (int)(kx * g.dw);
+        ok[u] = j0 + u < jmax && (uint32_t)iy < g.H && (uint32_t)ix < g.W;
+        e[u] = img[ok[u] ? (ch * g.H + (uint32_t)iy) * g.W + (uint32_t)ix : 0u];
+        // next column, without branches (they would split the loads'
+        // basic block and serialise them)
+        const bool wx = kx + 1 == g.KW, wy = wx && ky + 1 == g.KH;
+        kx = wx ? 0u : kx + 1;
+        ky = wy ? 0u : (wx ? ky + 1 : ky);
+        ch += wy ? 1u : 0u;
+      }
+      uint32_t x[G / 4];
+#pragma unroll
+      for (int d = 0; d < G / 4; d++) {
+        uint32_t c[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) c[k] = ok[4 * d + k] ? CodeOf<T, RAW>::get(e[4 * d + k], bipolar) : 0u;
+        x[d] = c[0] | (c[1] << 8) | (c[2] << 16) | (c[3] << 24);
+      }
+#pragma unroll
+      for (int b = 0; b < 8; b++) {
+        if ((uint32_t)b >= nbits) break;
+        const uint32_t m = 0x01010101u << b;
+        uint32_t bits = 0;
+#pragma unroll
+        for (int k = 0; k < G / 8; k++) {
+          uint32_t part = __builtin_amdgcn_udot4(x[2 * k] & m, 0x08040201u, 0u, false);
+          part = __builtin_amdgcn_udot4(x[2 * k + 1] & m, 0x80402010u, part, false);
+          bits |= (part >> b) << (8 * k);
         }
+        out[b] |= (uint64_t)bits << j0;
       }
     }
   }
