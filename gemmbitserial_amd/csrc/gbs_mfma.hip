@@ -54,31 +54,56 @@ void mfma_kernel(MfmaOperands a, MfmaPlan pl) {
     mfma_tile<P, Q, WP, KSB, DEPTH, FMT>(a, pt, qt, kbeg, kend, pl.splits > 1, lds);
 }
 
-// planes -> int8 values, 4 bytes per thread.  Zero outside the real matrix.
+// planes -> int8 values: one thread per 64-value word of a copy row (64
+// bytes, four 16-byte stores).  Four values at a time, bit b of each code
+// is spread into bit b of its byte by one multiply ((x & 0xF) * 0x00204081
+// puts bit i of x at bit 8i, with no two partial products meeting); then
+// bipolar codes 0/1 become 0xFF/0x01 and signed codes are sign-extended
+// bytewise.  Zero outside the real matrix, also for bipolar operands.
 __global__ __launch_bounds__(256) void expand_kernel(
     const uint64_t *planes, uint64_t nrows, uint64_t ncols, uint64_t wpr, uint64_t W,
     uint32_t nbits, bool issigned, uint64_t rows_a, uint64_t ld, int8_t *out) {
+  const uint64_t wpl = ld / 64;
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint64_t per_row = ld / 4;
-  if (i >= rows_a * per_row) return;
-  const uint64_t r = i / per_row, k0 = (i - r * per_row) * 4;
-  uint32_t packed = 0;
-  if (r < nrows) {
+  if (i >= rows_a * wpl) return;
+  const uint64_t r = i / wpl, w = i - r * wpl;
+  uint32_t q[16];
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const uint64_t k = k0 + j;
-      if (k >= ncols) break;
-      uint32_t code = 0;
-      for (uint32_t bb = 0; bb < nbits; bb++)
-        code |= (uint32_t)((planes[(uint64_t)bb * W + r * wpr + (k >> 6)] >> (k & 63)) & 1) << bb;
-      int32_t v;
-      if (nbits == 1 && issigned) v = code ? 1 : -1;
-      else if (issigned && (code >> (nbits - 1))) v = (int32_t)code - (int32_t)(1u << nbits);
-      else v = (int32_t)code;
-      packed |= ((uint32_t)v & 0xffu) << (8 * j);
+  for (int t = 0; t < 16; t++) q[t] = 0;
+  if (r < nrows && w * 64 < ncols) {
+    uint64_t p[8];
+#pragma unroll
+    for (uint32_t b = 0; b < 8; b++) p[b] = b < nbits ? planes[b * W + r * wpr + w] : 0;
+#pragma unroll
+    for (int t = 0; t < 16; t++) {
+      uint32_t v = 0;
+#pragma unroll
+      for (uint32_t b = 0; b < 8; b++)
+        if (b < nbits) v |= ((((uint32_t)(p[b] >> (4 * t)) & 0xFu) * 0x00204081u) & 0x01010101u) << b;
+      q[t] = v;
+    }
+    if (nbits == 1 && issigned) {
+#pragma unroll
+      for (int t = 0; t < 16; t++) q[t] |= (q[t] ^ 0x01010101u) * 0xFFu;  // 0 -> -1
+    } else if (issigned && nbits < 8) {
+      const uint32_t ext = 0xFFu & (0xFFu << nbits);
+#pragma unroll
+      for (int t = 0; t < 16; t++) q[t] |= ((q[t] >> (nbits - 1)) & 0x01010101u) * ext;
+    }
+    const uint64_t valid = ncols - w * 64;
+    if (valid < 64) {
+#pragma unroll
+      for (int t = 0; t < 16; t++) {
+        const uint64_t k = 4 * (uint64_t)t;
+        const uint32_t keep = k >= valid ? 0u : valid - k >= 4 ? 0xFFFFFFFFu
+                                                               : (1u << (8 * (valid - k))) - 1u;
+        q[t] &= keep;
+      }
     }
   }
-  reinterpret_cast<uint32_t *>(out)[i] = packed;
+  uint4 *o = reinterpret_cast<uint4 *>(out + r * ld + w * 64);
+#pragma unroll
+  for (int v = 0; v < 4; v++) o[v] = make_uint4(q[4 * v], q[4 * v + 1], q[4 * v + 2], q[4 * v + 3]);
 }
 
 // planes -> FP4 codes: one thread per 64-value word of a copy row (32
@@ -327,7 +352,7 @@ int ensure_vals(gbs_matrix *m, hipStream_t s, bool may_allocate) {
     m->vals_rows = rows_a;
     m->vals_ld = ld_a;
   }
-  const uint64_t n = m->vals_rows * m->vals_ld / 4;
+  const uint64_t n = m->vals_rows * (m->vals_ld / 64);
   hipLaunchKernelGGL(expand_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s,
                      m->d_planes, m->nrows, m->ncols, m->wpr(), m->wpp(),
                      (uint32_t)m->nbits, m->issigned, m->vals_rows, m->vals_ld, m->d_vals);
