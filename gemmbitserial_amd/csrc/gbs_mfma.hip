@@ -119,20 +119,41 @@ __global__ __launch_bounds__(256) void expand_f4_kernel(
   const uint64_t r = i / wpl, w = i - r * wpl;
   uint32_t q[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (r < nrows && w * 64 < ncols) {
-    uint64_t p0 = planes[r * wpr + w], p1 = 0, p2 = 0;
-    if (nbits > 1) p1 = planes[W + r * wpr + w];
-    if (nbits > 2) p2 = planes[2 * W + r * wpr + w];
-    const uint64_t valid = ncols - w * 64;
-    const bool bip = nbits == 1 && issigned;
+    uint64_t p[3] = {planes[r * wpr + w], 0, 0};
+    if (nbits > 1) p[1] = planes[W + r * wpr + w];
+    if (nbits > 2) p[2] = planes[2 * W + r * wpr + w];
+    // eight values (one dword of nibbles) at a time: bit b of each code
+    // spread into bit b of its nibble by three shift-or-mask steps (bits
+    // 0-3 / 4-7 to the two halves, then pairs, then single bits)
+    const uint32_t sgn = nbits - 1, mag = (1u << nbits) - 1;
 #pragma unroll
-    for (int k = 0; k < 64; k++) {
-      const uint32_t code = (uint32_t)((p0 >> k) & 1) | (uint32_t)((p1 >> k) & 1) << 1 |
-                            (uint32_t)((p2 >> k) & 1) << 2;
-      uint32_t nib;
-      if (bip) nib = code ? 0x1u : 0x9u;
-      else if (issigned && (code >> (nbits - 1))) nib = 0x8u | ((1u << nbits) - code);
-      else nib = code;
-      if ((uint64_t)k < valid) q[k >> 3] |= nib << (4 * (k & 7));
+    for (int t = 0; t < 8; t++) {
+      uint32_t v = 0;
+#pragma unroll
+      for (uint32_t b = 0; b < 3; b++) {
+        if (b >= nbits) break;
+        uint32_t x = (uint32_t)(p[b] >> (8 * t)) & 0xFFu;
+        x = (x | (x << 12)) & 0x000F000Fu;
+        x = (x | (x << 6)) & 0x03030303u;
+        x = (x | (x << 3)) & 0x11111111u;
+        v |= x << b;
+      }
+      if (nbits == 1 && issigned) {
+        v = 0x11111111u | ((v ^ 0x11111111u) << 3);  // 1 -> +1 (0x1), 0 -> -1 (0x9)
+      } else if (issigned) {
+        // negative codes: sign bit 8 | (2^nbits - code), nibblewise
+        const uint32_t s = (v >> sgn) & 0x11111111u, m = s * mag;
+        v = ((v ^ m) + s) | (s << 3);
+      }
+      q[t] = v;
+    }
+    const uint64_t valid = ncols - w * 64;
+    if (valid < 64) {
+#pragma unroll
+      for (int t = 0; t < 8; t++) {
+        const uint64_t k = 8 * (uint64_t)t;
+        q[t] &= k >= valid ? 0u : valid - k >= 8 ? 0xFFFFFFFFu : (1u << (4 * (valid - k))) - 1u;
+      }
     }
   }
   uint4 *o = reinterpret_cast<uint4 *>(out + r * ld + w * 32);
