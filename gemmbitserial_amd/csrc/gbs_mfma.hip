@@ -181,20 +181,41 @@ __global__ __launch_bounds__(256) void expand_f6_kernel(
     for (uint32_t b = 0; b < 5; b++)
       if (b < nbits) p[b] = planes[b * W + r * wpr + w];
     const uint64_t valid = ncols - w * 64;
-    const bool bip = nbits == 1 && issigned;
+    const uint32_t mag = (1u << nbits) - 1;
 #pragma unroll
-    for (int k = 0; k < 64; k++) {
-      uint32_t code = 0;
+    for (int gi = 0; gi < 2; gi++) {
+      uint64_t acc[3] = {0, 0, 0};  // the group's 24 bytes
 #pragma unroll
-      for (int b = 0; b < 5; b++) code |= (uint32_t)((p[b] >> k) & 1) << b;
-      uint32_t c6;
-      if (bip) c6 = code ? 0x01u : 0x21u;
-      else if (issigned && (code >> (nbits - 1))) c6 = 0x20u | ((1u << nbits) - code);
-      else c6 = code;
-      if ((uint64_t)k >= valid) c6 = 0;
-      const int gi = k >> 5, bit = 6 * (k & 31);
-      d[gi][bit >> 5] |= c6 << (bit & 31);
-      if ((bit & 31) > 26) d[gi][(bit >> 5) + 1] |= c6 >> (32 - (bit & 31));
+      for (int c = 0; c < 8; c++) {
+        // four values as four 6-bit fields: bit b of each code spread into
+        // bit b of its field (bits 0-3 of x to bits 0, 6, 12, 18)
+        uint32_t f = 0;
+#pragma unroll
+        for (uint32_t b = 0; b < 5; b++) {
+          if (b >= nbits) break;
+          uint32_t x = (uint32_t)(p[b] >> (32 * gi + 4 * c)) & 0xFu;
+          x = (x | (x << 10)) & 0x3003u;
+          x = (x | (x << 5)) & 0x41041u;
+          f |= x << b;
+        }
+        if (nbits == 1 && issigned) {
+          f = 0x041041u | ((f ^ 0x041041u) << 5);  // 1 -> +1 (0x01), 0 -> -1 (0x21)
+        } else if (issigned) {
+          // negative codes: sign bit 32 | (2^nbits - code), fieldwise
+          const uint32_t sg = (f >> (nbits - 1)) & 0x041041u, m = sg * mag;
+          f = ((f ^ m) + sg) | (sg << 5);
+        }
+        const uint64_t first = 32 * gi + 4 * c;
+        if (first + 4 > valid) f &= first >= valid ? 0u : (1u << (6 * (valid - first))) - 1u;
+        const int pos = 24 * c, wd = pos >> 6, off = pos & 63;
+        acc[wd] |= (uint64_t)f << off;
+        if (off > 40) acc[wd + 1] |= (uint64_t)f >> (64 - off);
+      }
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        d[gi][2 * k] = (uint32_t)acc[k];
+        d[gi][2 * k + 1] = (uint32_t)(acc[k] >> 32);
+      }
     }
   }
   uint8_t *base = out + r * ld + (w >> 1) * 96;
