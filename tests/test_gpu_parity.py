@@ -546,3 +546,35 @@ def test_auto_picks_format_from_support_and_measured_times(gbs, tmp_path):
         assert gbs.select(mat(3, False, M), mat(2, True, N)) == "mfma"
     finally:
         gbs.tune_load(table)
+
+
+def test_random_sweep_every_engine(gbs, orc):
+    """A seeded sweep of random shapes (log-uniform 1..1500 rows, 1..5000
+    columns) and encodings (1..8 bits, either signedness) through AUTO and
+    every engine that holds the product exactly, each against the exact
+    product of the values the oracle's import denotes (float64 BLAS, exact
+    at these magnitudes; no int32 wrap is reachable)."""
+    rng = np.random.default_rng(20261203)
+    engines = ["popcount", "mfma", "mfma_fp4", "mfma_fp6"]
+    ran = {e: 0 for e in engines}
+    for case in range(40):
+        M, N = (int(np.exp(rng.uniform(0, np.log(1500)))) for _ in range(2))
+        K = int(np.exp(rng.uniform(0, np.log(5000))))
+        lb, rb = (int(rng.integers(1, 9)) for _ in range(2))
+        ls, rs = (bool(rng.integers(0, 2)) for _ in range(2))
+        lhs = rand_values(rng, (M, K), lb, ls, np.int16)
+        rhs = rand_values(rng, (N, K), rb, rs, np.int16)
+        l, r = gbs.from_values(lhs, lb, ls), gbs.from_values(rhs, rb, rs)
+        want = O.wrap_i32(O.blas_gemm_exact(O.values_of(lhs, lb, ls), O.values_of(rhs, rb, rs)))
+        say("sweep", case, M, K, N, lb, rb, ls, rs, gbs.select(l, r))
+        assert np.array_equal(gbs.gemm(l, r, "auto"), want), (case, "auto")
+        for e in engines:
+            try:
+                got = gbs.gemm(l, r, e)
+            except gbs.GbsError as err:
+                assert e != "popcount" and err.code == 5, (case, e, err)  # cannot hold it exactly
+                continue
+            ran[e] += 1
+            assert np.array_equal(got, want), (case, e)
+        l.close(), r.close()
+    assert min(ran.values()) >= 3, ran
