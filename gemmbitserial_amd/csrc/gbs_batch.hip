@@ -22,7 +22,8 @@
 // persistent kernel (one kernel carrying every format's tile code would
 // spill registers).
 //
-// Run: a small kernel zeroes the queue counters, then per format group a
+// Run: one small kernel zeroes the queue counters (and the results of
+// split products, which their pieces add to atomically), then per format group a
 // grid of 2 blocks x 256 CUs loops: thread 0 takes the next tile index with
 // one atomicAdd, the block finds its product in a per-tile product table
 // built at plan time (one load; launches planned per call, like the serving
@@ -76,8 +77,11 @@ __global__ void reset_counter_kernel(unsigned int *c) {
 }
 
 // Zero the results of split products (their partial sums are added
-// atomically); blockIdx.y picks the span.
-__global__ __launch_bounds__(256) void zero_spans_kernel(const ZeroSpan *spans) {
+// atomically); blockIdx.y picks the span.  Block (0, 0) also resets the
+// queue counters, which saves the launch of reset_counter_kernel.
+__global__ __launch_bounds__(256) void zero_spans_kernel(const ZeroSpan *spans,
+                                                         unsigned int *counters) {
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 3) counters[threadIdx.x] = 0;
   const ZeroSpan z = spans[blockIdx.y];
   for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < z.n;
        i += (uint64_t)gridDim.x * 256)
@@ -194,11 +198,11 @@ int launch_views(const MfmaOperands *views, int n, void *host_desc, void *d_desc
   }
   if (tiles == 0) return GBS_OK;
   GBS_HIP(hipMemcpyAsync(d_desc, h, batch_desc_bytes(n), hipMemcpyHostToDevice, s));
-  if (nz) {
+  if (nz)
     hipLaunchKernelGGL(zero_spans_kernel, dim3(64, nz), dim3(256), 0, s,
-                       (const ZeroSpan *)((BatchProb *)d_desc + (n > 0 ? n : 1)));
-  }
-  hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, s, d_counter);
+                       (const ZeroSpan *)((BatchProb *)d_desc + (n > 0 ? n : 1)), d_counter);
+  else
+    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, s, d_counter);
   const uint32_t grid = (uint32_t)(tiles < 512 ? tiles : 512);
   hipLaunchKernelGGL((batch_kernel<kFmtI8>), dim3(grid), dim3(256), 0, s,
                      (const BatchProb *)d_desc, (uint32_t)m, tiles, d_counter,
@@ -228,8 +232,9 @@ static int batch_launch(gbs_batch *b, hipStream_t s) {
   if (b->total_tiles) {
     if (b->n_zero)
       hipLaunchKernelGGL(zero_spans_kernel, dim3(64, b->n_zero), dim3(256), 0, s,
-                         (const ZeroSpan *)b->d_zero);
-    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, s, b->d_counter);
+                         (const ZeroSpan *)b->d_zero, b->d_counter);
+    else
+      hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, s, b->d_counter);
     GBS_HIP(hipGetLastError());
     const BatchProb *desc = (const BatchProb *)b->d_desc;
     if (b->tiles[kFmtI8])
