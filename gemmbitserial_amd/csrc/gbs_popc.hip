@@ -28,9 +28,15 @@
 // round-robin over the 8 XCDs), keeping rhs re-reads in that XCD's L2.
 // Small grids split K over blocks and add with integer atomics.
 //
-// N <= 4 goes to gemv_kernel: lanes split K, the few rhs rows are read once
-// per word by every lane, lhs rows stream from HBM (arch-generic.hpp:264-329).
+// N <= 8 with at most 4 planes a side goes to the skinny kernel
+// (gbs_skinny.h: lhs planes streamed from HBM, rhs pieces in LDS, plane
+// pairs folded as they are counted); other N <= 4 products to gemv_kernel:
+// lanes split K, the few rhs rows are read once per word by every lane,
+// lhs rows stream from HBM (arch-generic.hpp:264-329).
+#include <algorithm>
+
 #include "gbs_internal.h"
+#include "gbs_skinny.h"
 
 namespace gbs {
 namespace {
@@ -266,6 +272,11 @@ __global__ __launch_bounds__(256) void gemv_kernel(PopcArgs a) {
   }
 }
 
+__global__ __launch_bounds__(256) void skinny_kernel(SkinnyProb p) {
+  __shared__ uint4 lds[kSkinnyLds / 16];
+  skinny_item(p, blockIdx.x, lds);
+}
+
 // ---------------------------------------------------------------- launch
 template <int PL, int PR, bool XOR, int TM, int TN, int KW>
 int run_tiled(PopcArgs a, int splits_cap, hipStream_t s, bool *used_atomic) {
@@ -361,6 +372,102 @@ bool wants_split(uint32_t M, uint32_t N, uint32_t kwords, int pl, int pr) {
 
 }  // namespace
 
+// Skinny engine: N <= 8 rhs rows and at most 4 planes a side (a bipolar
+// side counts 2: its plane and the ones plane).
+bool skinny_supported(const Operand &l, const Operand &r) {
+  if (r.nrows == 0 || r.nrows > kSkinnyMaxN || l.nrows == 0) return false;
+  if (l.nrows > 0x7fffffffull) return false;
+  return (l.bipolar || l.nbits <= 4) && (r.bipolar || r.nbits <= 4);
+}
+
+static void plane_weights(const Operand &o, bool other_bipolar, int32_t *w, uint32_t *n,
+                          bool *ones) {
+  *ones = false;
+  if (o.bipolar && other_bipolar) {  // handled by the XOR form
+    w[0] = 1;
+    *n = 1;
+    return;
+  }
+  if (o.bipolar) {  // 2b - 1: the plane weighs 2, the ones plane -1
+    w[0] = 2;
+    w[1] = -1;
+    *n = 2;
+    *ones = true;
+    return;
+  }
+  for (uint32_t i = 0; i < o.nbits; i++) w[i] = 1 << i;
+  if (o.issigned) w[o.nbits - 1] = -w[o.nbits - 1];
+  *n = o.nbits;
+}
+
+double skinny_bytes(const Operand &l, const Operand &r, uint64_t K) {
+  (void)r;
+  const uint64_t kwords = ((K + 63) / 64 + 1) & ~1ull;
+  return (double)l.nrows * (double)kwords * 8 * (double)l.nbits;
+}
+
+// Plan of a skinny product.  Lanes per row G: the largest power of two up
+// to 16 that leaves every lane at least 4 slices of 128 bits, doubled (up
+// to 64, at least one slice a lane) while one pass of 512 / G rows streams
+// more than item_bytes of lhs words.  Rows per item: as many passes as
+// keep an item near item_bytes.
+SkinnyProb skinny_plan(const Operand &l, const Operand &r, uint64_t K, int32_t *res,
+                       double item_bytes) {
+  SkinnyProb p{};
+  p.A = l.planes;
+  p.B = r.planes;
+  p.a_wpr = l.wpr; p.a_wpp = l.wpp;
+  p.b_wpr = r.wpr; p.b_wpp = r.wpp;
+  p.res = res;
+  p.M = (uint32_t)l.nrows;
+  p.N = (uint32_t)r.nrows;
+  p.K = (uint32_t)K;
+  p.kw_real = (uint32_t)((K + 63) / 64);
+  p.kwords = (p.kw_real + 1) & ~1u;
+  p.xor_mode = l.bipolar && r.bipolar;
+  int32_t wl[4] = {0, 0, 0, 0}, wr[4] = {0, 0, 0, 0};
+  bool lo = false, ro = false;
+  plane_weights(l, r.bipolar, wl, &p.pl, &lo);
+  plane_weights(r, l.bipolar, wr, &p.pr, &ro);
+  p.ones = lo ? 1 : ro ? 2 : 0;
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++) p.W[i * 4 + j] = p.xor_mode ? -2 : wl[i] * wr[j];
+  const uint32_t kq = p.kwords / 2;
+  const double lplanes = (double)l.nbits;
+  auto pass_bytes = [&](uint32_t gl) {
+    return (double)std::min<uint32_t>(p.M, 512u >> gl) * lplanes * p.kwords * 8;
+  };
+  uint32_t g = 0;
+  while (g < 4 && (2u << g) * 4 <= kq) g++;
+  while (g < 6 && pass_bytes(g) > item_bytes && (1u << (g + 1)) <= kq) g++;
+  p.g_log2 = g;
+  const uint32_t pass = 512u >> g;
+  const uint32_t reps = (uint32_t)std::max(1.0, std::min(item_bytes / pass_bytes(g), 1e6));
+  const uint64_t rows = std::min<uint64_t>((uint64_t)pass * reps, 0x40000000ull);
+  p.rows = (uint32_t)rows;
+  p.items = (uint32_t)((p.M + rows - 1) / rows);
+  return p;
+}
+
+// Modelled time on the skinny kernel: the larger of its VALU work (and,
+// popcount, scaled add per 128-bit slice, row, rhs row and plane pair, at
+// half the 39.3 T lane-op/s peak) and its lhs plane bytes (at 70 % of the
+// 8 TB/s HBM peak).
+double skinny_time(const Operand &l, const Operand &r, uint64_t K) {
+  const double M = (double)l.nrows, N = (double)r.nrows;
+  const double kq = (double)((((K + 63) / 64 + 1) & ~1ull) / 2);
+  const bool x = l.bipolar && r.bipolar;
+  const double pl = x ? 1 : l.bipolar ? 2 : (double)l.nbits;
+  const double pr = x ? 1 : r.bipolar ? 2 : (double)r.nbits;
+  const double valu = M * kq * N * pl * pr * 10 / (0.5 * 39.3e12);
+  const double bytes = M * kq * 16 * (double)l.nbits / (0.7 * 8e12);
+  return valu > bytes ? valu : bytes;
+}
+
+bool popc_needs_rowsum(const Operand &l, const Operand &r) {
+  return l.bipolar != r.bipolar && !skinny_supported(l, r);
+}
+
 int launch_popc_gemm(const Operand &l, const Operand &r, uint64_t kwords,
                      uint64_t K, int32_t *d_res, const int32_t *d_rowsum,
                      hipStream_t s) {
@@ -382,6 +489,14 @@ int launch_popc_gemm(const Operand &l, const Operand &r, uint64_t kwords,
   if (kwords == 0) {
     // empty depth: every dot product is 0 (bipolar x regular: -0 rowsum)
     return launch_fill_i32(d_res, M * N, 0, s);
+  }
+  if (skinny_supported(l, r)) {
+    // items of about 1/1024 of the product (4 blocks per CU), 16 KiB or more
+    const SkinnyProb p = skinny_plan(l, r, K, d_res,
+                                     std::max(16384.0, skinny_bytes(l, r, K) / 1024));
+    hipLaunchKernelGGL(skinny_kernel, dim3(p.items), dim3(256), 0, s, p);
+    GBS_HIP(hipGetLastError());
+    return GBS_OK;
   }
   const int gl = (int)((l.nbits + 3) / 4), gr = (int)((r.nbits + 3) / 4);
   const int ngroups = XOR ? 1 : gl * gr;

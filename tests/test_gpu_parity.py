@@ -229,6 +229,57 @@ def test_split_k_path(gbs, orc):
         assert np.array_equal(got, O.wrap_i32(O.blas_gemm_exact(lv, rv)))
 
 
+ENCODINGS = [(1, True), (1, False), (2, True), (2, False), (3, True), (3, False),
+             (4, True), (4, False)]
+
+
+@pytest.mark.parametrize("lenc", ENCODINGS)
+def test_skinny_engine_every_encoding(gbs, orc, lenc):
+    """Products of 1..8 rhs rows go to the skinny kernel (planes straight
+    from HBM, rhs pieces in LDS, plane pairs folded as they are counted,
+    bipolar sides through a ones plane): every encoding pair, depths
+    around word, 128-bit and 8192-column piece boundaries, one to a few
+    thousand rows."""
+    lb, ls = lenc
+    rng = np.random.default_rng(900 + 2 * lb + ls)
+    shapes = [(1, 1, 1), (7, 63, 2), (100, 65, 3), (1000, 200, 8), (513, 8192, 5),
+              (300, 8193, 4), (64, 20000, 7), (2049, 1000, 1), (33, 128, 6)]
+    for (rb, rs) in ENCODINGS:
+        for (M, K, N) in shapes:
+            say("skinny", lb, ls, rb, rs, M, K, N)
+            lhs = rand_values(rng, (M, K), lb, ls)
+            rhs = rand_values(rng, (N, K), rb, rs)
+            got = run_gemm(gbs, lhs, rhs, lb, rb, ls, rs)
+            want = O.wrap_i32(O.blas_gemm_exact(O.values_of(lhs, lb, ls),
+                                                O.values_of(rhs, rb, rs)))
+            assert np.array_equal(got, want)
+
+
+def test_skinny_engine_fill_and_fallback(gbs, orc):
+    """Many rows (the plan widens the lanes per row to fill the chip) and
+    rows of an odd number of words (the last 128-bit slice of a row holds
+    one real word; the word after it belongs to the next row and must not
+    be counted)."""
+    rng = np.random.default_rng(951)
+    for (lb, ls, rb, rs) in [(1, True, 1, True), (1, False, 1, False), (2, True, 4, False),
+                             (1, True, 3, True), (4, True, 1, True)]:
+        for N in (1, 2, 8):
+            say("skinny fill", lb, ls, rb, rs, N)
+            lhs = rand_values(rng, (12000, 2048), lb, ls)
+            rhs = rand_values(rng, (N, 2048), rb, rs)
+            got = run_gemm(gbs, lhs, rhs, lb, rb, ls, rs)
+            assert np.array_equal(got, O.wrap_i32(O.blas_gemm_exact(
+                O.values_of(lhs, lb, ls), O.values_of(rhs, rb, rs))))
+    for (lb, ls, rb, rs) in [(2, True, 2, False), (1, True, 2, False), (1, True, 1, True),
+                             (3, False, 1, True), (4, True, 4, True)]:
+        for (M, K, N, ka) in [(50, 150, 3, 192), (77, 8190, 8, 8256), (40, 9000, 2, 9024)]:
+            say("skinny odd pitch", lb, ls, rb, rs, M, K, N, ka)
+            lhs = rand_values(rng, (M, K), lb, ls)
+            rhs = rand_values(rng, (N, K), rb, rs)
+            got = run_gemm(gbs, lhs, rhs, lb, rb, ls, rs, geom=(M, N, M, ka, N, ka))
+            assert np.array_equal(got, O.wrap_i32(O.naive_gemm_i64(lhs, rhs)))
+
+
 # ------------------------------------------------------------ int8 MFMA engine
 @pytest.mark.parametrize("lb,rb", list(itertools.product([1, 2, 3, 4], repeat=2)))
 @pytest.mark.parametrize("ls,rs", list(itertools.product([False, True], repeat=2)))
