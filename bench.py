@@ -23,6 +23,9 @@ on).  The other configurations are reported beside it under "configs":
   c5  the CaffeNet GEMM list of the reference benchmark (8 products, 2-bit
       unsigned x 2-bit unsigned) in one batched launch, against one launch
       per product and the reference CPU loop
+  c6  skinny products: lhs 32768 x 32768 2-bit signed (268 MB of planes,
+      more than the 256 MB last-level cache, so every pass reads HBM) times
+      1 and 8 rhs rows of 2-bit unsigned values (not in the default list)
 Every GPU result is checked bit-exact against an exact product before it is
 reported.  The reference CPU path (gemmBitSerial compiled from the
 reference sources, oracle/_ref, single thread) is timed on a bounded sample
@@ -89,15 +92,27 @@ def cfg_c2(rng):
             ("4s x 4u", w4, act4, 4, 4, True, False)]
 
 
+def cfg_c6(rng):
+    a = rng.integers(-2, 2, (32768, 32768), dtype=np.int8)
+    return [("gemv 1 row", a, rng.integers(0, 4, (1, 32768), dtype=np.int8), 2, 2, True, False),
+            ("gemv 8 rows", a, rng.integers(0, 4, (8, 32768), dtype=np.int8), 2, 2, True, False)]
+
+
 CONFIGS = {"c0": ("plain GEMM 256x2048x256, lhs 2-bit unsigned, rhs 2-bit signed", cfg_c0),
            "c1": ("binary layers: lhs 4096x8192 1-bit, rhs 512x8192 1-bit, bipolar "
                   "(Bernoulli 0.5) + unsigned {0,1} (30% ones)", cfg_c1),
            "c2": ("conv-as-GEMM: weights 256x2304, activations 50176x2304 (16x56x56 im2col, "
-                  "50% zeros), 2s x 3u and 4s x 4u", cfg_c2)}
+                  "50% zeros), 2s x 3u and 4s x 4u", cfg_c2),
+           "c6": ("skinny: lhs 32768x32768 2-bit signed (268 MB of planes), rhs 1 and 8 rows "
+                  "2-bit unsigned", cfg_c6)}
 
 
 def valu_lane_ops(M, K, N, lb, rb):
-    """AND/XOR + v_bcnt per 32-bit word per plane pair."""
+    """AND/XOR + v_bcnt per 32-bit word per plane pair (tiled kernel); for
+    products of at most 8 rhs rows and 4 planes a side (the skinny kernel)
+    4 AND + 4 v_bcnt + 2 adds per 128-bit slice per plane pair."""
+    if N <= 8 and lb <= 4 and rb <= 4:
+        return M * N * ((K + 127) // 128) * 10 * lb * rb
     return M * N * ((K + 31) // 32) * 2 * lb * rb
 
 
@@ -107,6 +122,23 @@ def plane_bytes(M, K, N, lb, rb):
 
 def int8_bytes(M, K, N):
     return (M + N) * K + 4 * M * N
+
+
+def operand_bytes(algo, M, K, N, lb, rb):
+    """HBM bytes one call reads and writes at least: the bit-planes for the
+    popcount engine, the value copies for the matrix cores, plus the result."""
+    if algo == "popcount":
+        return plane_bytes(M, K, N, lb, rb)
+    return (M + N) * K * {"mfma": 1.0, "mfma_fp4": 0.5, "mfma_fp6": 0.75}[algo] + 4 * M * N
+
+
+def exact_product(O, a, b, lb, rb, ls, rs):
+    """The exact int32 result, 2048 lhs rows at a time (float64 BLAS on
+    small integers is exact)."""
+    bv = O.values_of(b, rb, rs)
+    parts = [O.blas_gemm_exact(O.values_of(a[i:i + 2048], lb, ls), bv)
+             for i in range(0, a.shape[0], 2048)]
+    return O.wrap_i32(np.concatenate(parts, axis=1))
 
 
 # arithmetic each engine computes in: popcount on 32-bit plane words, int8
@@ -181,25 +213,30 @@ def run_config(gbs, O, name, runs, engines, steps, warmup, stream, ws, cpu_secs)
                 continue
             got = d_res.to_host(np.int32, (N, M))
             if want is None:
-                want = O.wrap_i32(O.blas_gemm_exact(O.values_of(a, lb, ls),
-                                                    O.values_of(b, rb, rs)))
+                want = exact_product(O, a, b, lb, rb, ls, rs)
             if not np.array_equal(got, want):
                 raise SystemExit(f"{name} {enc} {eng}: result differs from the exact product")
             sec = ms * 1e-3
             row["engines"][eng] = {"us": round(ms * 1e3, 2),
                                    "tops": round(2.0 * M * K * N / sec / 1e12, 2),
                                    "roofline": roofline_of(eng, M, K, N, lb, rb, sec),
+                                   "operand_gbs": round(operand_bytes(eng, M, K, N, lb, rb)
+                                                        / sec / 1e9, 1),
                                    "bit_exact": True}
             log(f"[bench] {name} {enc}: {eng} {ms * 1e3:.1f} us, "
                 f"{2.0 * M * K * N / sec / 1e12:.1f} TOPS, bit-exact")
         if cpu_secs > 0 and O.ref_available():
-            ns, reps, res = O.ref_bench_gemm(a, b, lb, rb, ls, rs, min_secs=cpu_secs,
+            # very large lhs: the reference runs on its first rows only
+            ms_ = max(1, min(M, (1 << 26) // max(K, 1)))
+            ns, reps, res = O.ref_bench_gemm(a[:ms_], b, lb, rb, ls, rs, min_secs=cpu_secs,
                                              max_reps=200)
-            if not np.array_equal(res, want):
+            if not np.array_equal(res, want[:, :ms_]):
                 raise SystemExit(f"{name} {enc}: reference CPU result differs")
-            row["cpu_reference"] = {"tops": round(2.0 * M * K * N / ns / 1e3, 4),
+            row["cpu_reference"] = {"tops": round(2.0 * ms_ * K * N / ns / 1e3, 4),
                                     "ms_per_call": round(ns / 1e6, 3), "timed_calls": reps,
                                     "cores": 1}
+            if ms_ < M:
+                row["cpu_reference"]["sample"] = f"first {ms_} of {M} lhs rows"
             log(f"[bench] {name} {enc}: reference CPU {ns / 1e6:.1f} ms/call ({reps} calls)")
         for m in (l, r):
             m.close()
