@@ -89,7 +89,7 @@ int launch_fill_i32(int32_t *d, uint64_t n, int32_t v, hipStream_t s);
 // side is bipolar (nullptr otherwise).
 // does the popcount engine want the regular operand's row sums for this
 // bipolar x regular product (the skinny kernel folds them in itself)
-bool popc_needs_rowsum(const Operand &l, const Operand &r);
+bool popc_needs_rowsum(const Operand &l, const Operand &r, uint64_t K);
 int launch_popc_gemm(const Operand &l, const Operand &r, uint64_t kwords,
                      uint64_t K, int32_t *d_res, const int32_t *d_rowsum,
                      hipStream_t s);

@@ -364,7 +364,7 @@ gbs_algo select_algo(const Operand &l, const Operand &r, uint64_t K) {
   // the 39.3 T lane-op/s VALU, 5 POPS int8 MFMA and 8 TB/s HBM peaks)
   const double valu = M * N * ((K + 31) / 32) * 2 * pl * pr / (0.55 * 39.3e12);
   const double pbytes = (M * Kd * l.nbits + N * Kd * r.nbits) / 8 + 4 * M * N;
-  const double t_popc = skinny_supported(l, r) ? skinny_time(l, r, K)
+  const double t_popc = skinny_supported(l, r) && K < (1ull << 31) ? skinny_time(l, r, K)
                         : (valu > pbytes / (0.6 * 8e12) ? valu : pbytes / (0.6 * 8e12));
   // matrix cores: FP4 / FP6 instructions cover twice the values of int8
   // ones at the same issue rate

@@ -464,8 +464,8 @@ double skinny_time(const Operand &l, const Operand &r, uint64_t K) {
   return valu > bytes ? valu : bytes;
 }
 
-bool popc_needs_rowsum(const Operand &l, const Operand &r) {
-  return l.bipolar != r.bipolar && !skinny_supported(l, r);
+bool popc_needs_rowsum(const Operand &l, const Operand &r, uint64_t K) {
+  return l.bipolar != r.bipolar && !(skinny_supported(l, r) && K < (1ull << 31));
 }
 
 int launch_popc_gemm(const Operand &l, const Operand &r, uint64_t kwords,
@@ -490,7 +490,7 @@ int launch_popc_gemm(const Operand &l, const Operand &r, uint64_t kwords,
     // empty depth: every dot product is 0 (bipolar x regular: -0 rowsum)
     return launch_fill_i32(d_res, M * N, 0, s);
   }
-  if (skinny_supported(l, r)) {
+  if (skinny_supported(l, r) && K < (1ull << 31)) {  // the plan holds K in 32 bits
     // items of about 1/1024 of the product (4 blocks per CU), 16 KiB or more
     const SkinnyProb p = skinny_plan(l, r, K, d_res,
                                      std::max(16384.0, skinny_bytes(l, r, K) / 1024));

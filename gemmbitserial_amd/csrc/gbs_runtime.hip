@@ -141,7 +141,7 @@ int gemm_enqueue(gbs_workspace *ws, const gbs_matrix *l, const gbs_matrix *r,
   }
   if (eng != GBS_ALGO_POPCOUNT) return fail(GBS_EINVAL, "gemm: unknown algo %d", (int)algo);
   const int32_t *d_rowsum = nullptr;
-  if (popc_needs_rowsum(lo, ro) && l->nrows && r->nrows && kwords) {
+  if (popc_needs_rowsum(lo, ro, K) && l->nrows && r->nrows && kwords) {
     const Operand &reg = lo.bipolar ? ro : lo;
     const size_t need = reg.nrows * sizeof(int32_t);
     if (ws->rowsum_cap < need) {
