@@ -60,7 +60,9 @@ typedef enum gbs_dtype {
  * engine returns identical int32 results. */
 typedef enum gbs_algo {
   GBS_ALGO_AUTO = 0,
-  GBS_ALGO_POPCOUNT = 1, /* fused AND/XOR + popcount over bit-planes */
+  GBS_ALGO_POPCOUNT = 1, /* fused AND/XOR + popcount over bit-planes; products
+                            of <= 8 rhs rows and <= 4 planes a side stream
+                            the lhs planes once (the smallest operand form) */
   GBS_ALGO_MFMA = 2,     /* planes expanded to int8, v_mfma_i32_*_i8 */
   GBS_ALGO_MFMA_FP4 = 3, /* planes expanded to FP4 codes (operands of <= 2
                             bits, <= 3 signed), v_mfma_scale_f32_*_f8f6f4 */
