@@ -166,14 +166,60 @@ static void rebalance(BatchProb *d, int n, double total_bytes) {
   }
 }
 
-// Plan and launch products given as int8 operand views (the serving loop's
-// per-step batch).  host_desc must be pinned and hold batch_desc_bytes(n);
-// it is reused only after the stream has passed this launch.
-int launch_views(const MfmaOperands *views, int n, void *host_desc, void *d_desc,
-                 unsigned int *d_counter, hipStream_t s) {
-  BatchProb *h = (BatchProb *)host_desc;
-  ZeroSpan *hz = (ZeroSpan *)(h + (n > 0 ? n : 1));
-  int m = 0, nz = 0;
+// The serving loop's per-step batch: at most kMaxViews products whose
+// descriptors travel in the kernel arguments (no copy ahead of the
+// launch); the queue counter resets itself (the last block to leave zeroes
+// it and the exit counter beside it); split products add their pieces
+// atomically into results the caller keeps zero between launches.
+constexpr int kMaxViews = 16;
+struct ViewBatch {
+  BatchProb p[kMaxViews];
+  uint32_t n;
+  uint64_t total;
+};
+
+__global__ __launch_bounds__(256, 2) void views_kernel(ViewBatch V, unsigned int *counters) {
+  __shared__ __attribute__((aligned(16))) int8_t lds[kMfmaLdsMax];
+  __shared__ uint32_t s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(counters, 1u);
+    __syncthreads();
+    const uint64_t item = s_item;
+    __syncthreads();
+    if (item >= V.total) break;
+    uint32_t lo = 0, hi = V.n - 1;  // largest p with tile_begin <= item
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) / 2;
+      if (V.p[mid].tile_begin <= item) lo = mid;
+      else hi = mid - 1;
+    }
+    const MfmaOperands ops = V.p[lo].ops;
+    const MfmaPlan pl = V.p[lo].plan;
+    uint32_t pt, qt, sp;
+    plan_decode(pl, item - V.p[lo].tile_begin, pt, qt, sp);
+    const uint32_t kbeg = sp * pl.kchunk;
+    const uint32_t kend = min(ops.kbytes, kbeg + pl.kchunk);
+    mfma_tile_by_shape<kFmtI8>(pl.shape, ops, pt, qt, kbeg, kend, pl.splits > 1, lds);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    // every block took its last index from counters[0] before this add
+    __threadfence();
+    if (atomicAdd(counters + 1, 1u) == gridDim.x - 1) {
+      atomicExch(counters, 0u);
+      atomicExch(counters + 1, 0u);
+    }
+  }
+}
+
+// Plan and launch products given as int8 operand views.  counters (two,
+// zero before the first launch) are left zero; the results of products the
+// plan splits along K must be zero before the launch.
+int launch_views(const MfmaOperands *views, int n, unsigned int *counters, hipStream_t s) {
+  if (n > kMaxViews) return fail(GBS_EINVAL, "launch_views: %d products (at most %d)", n, kMaxViews);
+  ViewBatch V{};
+  BatchProb *h = V.p;
+  int m = 0;
   double total = 0;
   for (int i = 0; i < n; i++) {
     const MfmaOperands &v = views[i];
@@ -194,26 +240,16 @@ int launch_views(const MfmaOperands *views, int n, void *host_desc, void *d_desc
   for (int i = 0; i < m; i++) {
     h[i].tile_begin = tiles;
     tiles += h[i].plan.tiles();
-    if (h[i].plan.splits > 1) hz[nz++] = ZeroSpan{h[i].ops.res, (uint64_t)h[i].ops.M * h[i].ops.N};
   }
   if (tiles == 0) return GBS_OK;
-  GBS_HIP(hipMemcpyAsync(d_desc, h, batch_desc_bytes(n), hipMemcpyHostToDevice, s));
-  if (nz)
-    hipLaunchKernelGGL(zero_spans_kernel, dim3(64, nz), dim3(256), 0, s,
-                       (const ZeroSpan *)((BatchProb *)d_desc + (n > 0 ? n : 1)), d_counter);
-  else
-    hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, s, d_counter);
+  V.n = (uint32_t)m;
+  V.total = tiles;
   const uint32_t grid = (uint32_t)(tiles < 512 ? tiles : 512);
-  hipLaunchKernelGGL((batch_kernel<kFmtI8>), dim3(grid), dim3(256), 0, s,
-                     (const BatchProb *)d_desc, (uint32_t)m, tiles, d_counter,
-                     (const uint32_t *)nullptr);
+  hipLaunchKernelGGL(views_kernel, dim3(grid), dim3(256), 0, s, V, counters);
   GBS_HIP(hipGetLastError());
   return GBS_OK;
 }
 
-size_t batch_desc_bytes(int n) {
-  return (size_t)(n > 0 ? n : 1) * (sizeof(BatchProb) + sizeof(ZeroSpan));
-}
 
 // Operands re-imported since the plan get their copies rebuilt before a
 // run, on the run's stream.  The plan stays valid: a copy buffer, once

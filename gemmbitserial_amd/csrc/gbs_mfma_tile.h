@@ -774,9 +774,9 @@ inline int shape_class(uint64_t M, uint64_t N, uint64_t K) {
 // splits 0 = decide from the tile count.
 void mfma_config_for(uint64_t M, uint64_t N, uint64_t K, int fmt, int *shape, int *splits);
 
-// Persistent launch over products given as operand views (gbs_batch.hip).
-int launch_views(const MfmaOperands *views, int n, void *host_desc, void *d_desc,
-                 unsigned int *d_counter, hipStream_t s);
-size_t batch_desc_bytes(int n);
+// Persistent launch over at most 16 products given as int8 operand views
+// (gbs_batch.hip): counters are two words, zero before the first launch
+// and left zero; results of products split along K must be zero before.
+int launch_views(const MfmaOperands *views, int n, unsigned int *counters, hipStream_t s);
 
 }  // namespace gbs

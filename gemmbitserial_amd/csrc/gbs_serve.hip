@@ -12,17 +12,21 @@
 //      inputs are packed on the GPU into pool 0;
 //   2. one persistent launch computes, for every cohort in flight (cohort l
 //      sits at layer l), acc_l = pool_l * W_l^T -- all layers at once, so
-//      requests that arrived at different times share the device;
+//      requests that arrived at different times share the device; its
+//      descriptors ride in the kernel arguments and its queue counter
+//      resets itself, so the launch is the step's only GEMM-side call;
 //   3. one launch requantizes acc_l -> min(2^act_bits - 1, max(0, acc >> s_l))
 //      and packs it (one __ballot per plane word, plus the int8 copy) into
-//      pool l+1, where cohort l sits next step;
+//      pool l+1, where cohort l sits next step, and copies the last layer's
+//      rows to the output buffer; it leaves every accumulator it read zero,
+//      ready for the next step's K-split products to add into;
 //   4. the last layer's int32 rows are copied back; those requests finish.
 //
 // A call of gbs_server_step enqueues one step and then completes the step
 // before it (waits for its event, hands its last-layer rows to their
 // requests): the host admits and plans step t+1 while the GPU still runs
-// step t.  The pinned staging buffers (inputs, descriptors, outputs) are
-// doubled for that; device buffers need no doubling, the stream orders
+// step t.  The pinned staging buffers (inputs, outputs) are doubled for
+// that; device buffers need no doubling, the stream orders
 // their reuse.  A request therefore becomes ready one call after the step
 // that computed it.
 //
@@ -59,15 +63,19 @@ struct Req {
 
 constexpr int kMaxLayers = 16;
 
-// One activation pack job: rows [0, rows) of a source into a pool.
+// One activation job over rows [0, rows): mode 0 packs input codes into a
+// pool; mode 1 requantizes int32 accumulators into a pool; mode 2 copies
+// the last layer's accumulators out.  Modes 1 and 2 leave the accumulators
+// they read zero: the next step's split products add into them.
 struct ActJob {
   const uint8_t *in;     // mode 0: input codes, rows x h (row-major)
-  const int32_t *acc;    // mode 1: int32 accumulators, rows x h
-  uint64_t *planes;      // pool planes [nbits][R][wpr]
-  int8_t *vals;          // pool int8 copy [.][ld]
+  int32_t *acc;          // modes 1, 2: int32 accumulators, rows x h
+  uint64_t *planes;      // modes 0, 1: pool planes [nbits][R][wpr]
+  int8_t *vals;          // modes 0, 1: pool int8 copy [.][ld]
   uint64_t wpr, wpp, ld;
   uint32_t rows, h, mode, shift;
   uint64_t unit_begin;   // prefix of rows * wpr over the jobs
+  int32_t *out;          // mode 2: rows x h int32
 };
 
 struct ActJobs {
@@ -88,11 +96,19 @@ __global__ __launch_bounds__(256) void act_pack_kernel(ActJobs J, uint64_t units
   const uint64_t r = local / jb.wpr, w = local - r * jb.wpr;
   const uint64_t c = w * 64 + lane;
   uint32_t code = 0;
+  if (jb.mode == 2) {
+    if (c < jb.h) {
+      jb.out[r * jb.h + c] = jb.acc[r * jb.h + c];
+      jb.acc[r * jb.h + c] = 0;
+    }
+    return;  // the whole wave: a job's words are one mode
+  }
   if (c < jb.h) {
     if (jb.mode == 0) {
       code = jb.in[r * jb.h + c] & J.maxv;
     } else {
       const int32_t a = jb.acc[r * jb.h + c] >> jb.shift;  // arithmetic shift
+      jb.acc[r * jb.h + c] = 0;
       code = a < 0 ? 0u : (a > (int32_t)J.maxv ? J.maxv : (uint32_t)a);
     }
     jb.vals[r * jb.ld + c] = (int8_t)code;
@@ -110,12 +126,12 @@ struct gbs_server {
   std::vector<const gbs_matrix *> W;
   std::vector<uint32_t> h, shift;
   std::vector<gbs_matrix *> pool;
-  std::vector<int32_t *> acc;
-  void *h_desc = nullptr, *d_desc = nullptr;  // h_desc: two halves
-  unsigned int *d_counter = nullptr;
+  std::vector<int32_t *> acc;                  // zero between steps
+  int32_t *d_out = nullptr;                    // last layer's rows, copied out
+  unsigned int *d_counter = nullptr;           // the launch's two queue words
   uint8_t *h_in = nullptr, *d_in = nullptr;     // h_in: two halves
   int32_t *h_out = nullptr;                     // two halves
-  size_t desc_half = 0, in_half = 0, out_half = 0;  // bytes / elements per half
+  size_t in_half = 0, out_half = 0;  // bytes / elements per half
   hipEvent_t ev[2] = {nullptr, nullptr};        // end of the step using each half
   std::vector<Req *> completing[2];             // requests whose rows land in h_out half
   bool pending[2] = {false, false};
@@ -135,10 +151,9 @@ static void server_free(gbs_server *s) {
   (void)hipDeviceSynchronize();
   for (gbs_matrix *m : s->pool) gbs_matrix_destroy(m);
   for (int32_t *a : s->acc) (void)hipFree(a);
-  if (s->d_desc) (void)hipFree(s->d_desc);
   if (s->d_counter) (void)hipFree(s->d_counter);
   if (s->d_in) (void)hipFree(s->d_in);
-  if (s->h_desc) (void)hipHostFree(s->h_desc);
+  if (s->d_out) (void)hipFree(s->d_out);
   if (s->h_in) (void)hipHostFree(s->h_in);
   if (s->h_out) (void)hipHostFree(s->h_out);
   for (int k = 0; k < 2; k++) {
@@ -205,7 +220,7 @@ int gbs_server_create(const gbs_matrix *const *weights, const uint32_t *shifts,
     s->shift.push_back(l + 1 < n_layers ? shifts[l] : 0);
   }
   // bytes one row of capacity costs across every pool and buffer
-  uint64_t per_row = s->h[0] + 4ull * s->h[n_layers];  // input / output staging
+  uint64_t per_row = s->h[0] + 8ull * s->h[n_layers];  // input / output staging
   for (uint32_t l = 0; l < n_layers; l++)
     per_row += act_bits * align_up(s->h[l], 64) / 8 + align_up(s->h[l], kValsColAlign) +
                4ull * s->h[l + 1];
@@ -245,6 +260,7 @@ int gbs_server_create(const gbs_matrix *const *weights, const uint32_t *shifts,
     p->vals_version = p->planes_version;  // kept current by act_pack_kernel
     int32_t *a = nullptr;
     e = hipMalloc((void **)&a, R * s->h[l + 1] * 4ull);
+    if (e == hipSuccess) e = hipMemset(a, 0, R * s->h[l + 1] * 4ull);
     if (e != hipSuccess) {
       rc = hip_fail(e, "gbs_server_create: accumulators");
       break;
@@ -252,13 +268,12 @@ int gbs_server_create(const gbs_matrix *const *weights, const uint32_t *shifts,
     s->acc.push_back(a);
   }
   if (!rc) {
-    hipError_t e = hipMalloc(&s->d_desc, batch_desc_bytes(kMaxLayers));
-    if (e == hipSuccess) e = hipMalloc((void **)&s->d_counter, 256);
+    hipError_t e = hipMalloc((void **)&s->d_counter, 256);
+    if (e == hipSuccess) e = hipMemset(s->d_counter, 0, 256);
     if (e == hipSuccess) e = hipMalloc((void **)&s->d_in, R * s->h[0] + 16);
-    s->desc_half = align_up(batch_desc_bytes(kMaxLayers), 256);
+    if (e == hipSuccess) e = hipMalloc((void **)&s->d_out, R * s->h[n_layers] * 4ull + 16);
     s->in_half = align_up(R * s->h[0] + 16, 256);
     s->out_half = align_up(R * s->h[n_layers] + 4, 64);  // int32 elements
-    if (e == hipSuccess) e = hipHostMalloc(&s->h_desc, 2 * s->desc_half);
     if (e == hipSuccess) e = hipHostMalloc((void **)&s->h_in, 2 * s->in_half);
     if (e == hipSuccess) e = hipHostMalloc((void **)&s->h_out, 2 * s->out_half * 4);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&s->ev[0], hipEventDisableTiming);
@@ -307,7 +322,6 @@ int gbs_server_step(gbs_server *s, uint32_t *finished_now) {
   const uint32_t L = s->L;
   const int cur = (int)(s->steps & 1), prev = cur ^ 1;
   uint8_t *h_in = s->h_in + (size_t)cur * s->in_half;
-  void *h_desc = (char *)s->h_desc + (size_t)cur * s->desc_half;
   int32_t *h_out = s->h_out + (size_t)cur * s->out_half;
   // Between steps cohort[l] holds the requests whose next GEMM is layer l
   // (their activations sit in pool l); cohort[0] is empty.
@@ -332,7 +346,7 @@ int gbs_server_step(gbs_server *s, uint32_t *finished_now) {
     J.maxv = (1u << s->act_bits) - 1;
     gbs_matrix *p = s->pool[0];
     J.j[0] = ActJob{s->d_in, nullptr, p->d_planes, p->d_vals, p->wpr(), p->wpp(),
-                    p->vals_ld, n0, s->h[0], 0, 0, 0};
+                    p->vals_ld, n0, s->h[0], 0, 0, 0, nullptr};
     J.n = 1;
     const uint64_t units = (uint64_t)n0 * p->wpr();
     hipLaunchKernelGGL(act_pack_kernel, dim3((uint32_t)((units + 3) / 4)), dim3(256), 0, st,
@@ -360,8 +374,9 @@ int gbs_server_step(gbs_server *s, uint32_t *finished_now) {
     views[nv++] = v;
     s->ops_done += 2.0 * s->h[l + 1] * s->h[l] * n;
   }
-  if (nv && (rc = launch_views(views, nv, h_desc, s->d_desc, s->d_counter, st))) return rc;
-  // 3. requantize + pack into the next layer's pool
+  if (nv && (rc = launch_views(views, nv, s->d_counter, st))) return rc;
+  // 3. requantize + pack into the next layer's pool; the last layer's rows
+  // to the output buffer (both leave their accumulators zero)
   ActJobs J{};
   J.nbits = s->act_bits;
   J.maxv = (1u << s->act_bits) - 1;
@@ -371,8 +386,15 @@ int gbs_server_step(gbs_server *s, uint32_t *finished_now) {
     if (!n) continue;
     gbs_matrix *p = s->pool[l + 1];
     J.j[J.n++] = ActJob{nullptr, s->acc[l], p->d_planes, p->d_vals, p->wpr(), p->wpp(),
-                        p->vals_ld, n, s->h[l + 1], 1, s->shift[l], units};
+                        p->vals_ld, n, s->h[l + 1], 1, s->shift[l], units, nullptr};
     units += (uint64_t)n * p->wpr();
+  }
+  const uint32_t nl = s->cohort_rows[L - 1];
+  if (nl) {
+    const uint64_t wpr = align_up(s->h[L], 64) / 64;
+    J.j[J.n++] = ActJob{nullptr, s->acc[L - 1], nullptr, nullptr, wpr, 0, 0, nl, s->h[L], 2,
+                        0, units, s->d_out};
+    units += (uint64_t)nl * wpr;
   }
   if (units) {
     hipLaunchKernelGGL(act_pack_kernel, dim3((uint32_t)((units + 3) / 4)), dim3(256), 0, st,
@@ -381,9 +403,8 @@ int gbs_server_step(gbs_server *s, uint32_t *finished_now) {
   }
   // 4. last layer's rows back to the host (read when this step's event has
   // passed, in the next call)
-  const uint32_t nl = s->cohort_rows[L - 1];
   if (nl)
-    GBS_HIP(hipMemcpyAsync(h_out, s->acc[L - 1], (size_t)nl * s->h[L] * 4,
+    GBS_HIP(hipMemcpyAsync(h_out, s->d_out, (size_t)nl * s->h[L] * 4,
                            hipMemcpyDeviceToHost, st));
   GBS_HIP(hipEventRecord(s->ev[cur], st));
   s->completing[cur].swap(s->cohort[L - 1]);
