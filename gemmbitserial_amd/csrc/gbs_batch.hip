@@ -203,8 +203,10 @@ __global__ __launch_bounds__(256, 2) void views_kernel(ViewBatch V, unsigned int
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    // every block took its last index from counters[0] before this add
-    __threadfence();
+    // this thread's last queue add returned before this add is issued (the
+    // loop left on its value), and both are agent-scope atomics: by the
+    // last exit, no block takes from the queue any more
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     if (atomicAdd(counters + 1, 1u) == gridDim.x - 1) {
       atomicExch(counters, 0u);
       atomicExch(counters + 1, 0u);
