@@ -9,7 +9,8 @@
 //   1. admission: waiting requests join a new cohort while its rows fit the
 //      pool capacity R (set from the memory budget at creation) and the
 //      number of requests in flight stays under max_requests; the cohort's
-//      inputs are packed on the GPU into pool 0;
+//      inputs are packed on the GPU into pool 0, read from pinned host
+//      memory by the packing kernel itself;
 //   2. one persistent launch computes, for every cohort in flight (cohort l
 //      sits at layer l), acc_l = pool_l * W_l^T -- all layers at once, so
 //      requests that arrived at different times share the device; its
@@ -17,10 +18,10 @@
 //      resets itself, so the launch is the step's only GEMM-side call;
 //   3. one launch requantizes acc_l -> min(2^act_bits - 1, max(0, acc >> s_l))
 //      and packs it (one __ballot per plane word, plus the int8 copy) into
-//      pool l+1, where cohort l sits next step, and copies the last layer's
-//      rows to the output buffer; it leaves every accumulator it read zero,
-//      ready for the next step's K-split products to add into;
-//   4. the last layer's int32 rows are copied back; those requests finish.
+//      pool l+1, where cohort l sits next step, and writes the last layer's
+//      rows straight into pinned host memory; it leaves every accumulator
+//      it read zero, ready for the next step's K-split products to add into;
+//   4. those requests finish.
 //
 // A call of gbs_server_step enqueues one step and then completes the step
 // before it (waits for its event, hands its last-layer rows to their
@@ -127,10 +128,11 @@ struct gbs_server {
   std::vector<uint32_t> h, shift;
   std::vector<gbs_matrix *> pool;
   std::vector<int32_t *> acc;                  // zero between steps
-  int32_t *d_out = nullptr;                    // last layer's rows, copied out
   unsigned int *d_counter = nullptr;           // the launch's two queue words
-  uint8_t *h_in = nullptr, *d_in = nullptr;     // h_in: two halves
-  int32_t *h_out = nullptr;                     // two halves
+  // pinned host staging, two halves each, read and written by the kernels
+  // in place (no copies): inputs of an admitted cohort, last-layer rows
+  uint8_t *h_in = nullptr;
+  int32_t *h_out = nullptr;
   size_t in_half = 0, out_half = 0;  // bytes / elements per half
   hipEvent_t ev[2] = {nullptr, nullptr};        // end of the step using each half
   std::vector<Req *> completing[2];             // requests whose rows land in h_out half
@@ -152,8 +154,6 @@ static void server_free(gbs_server *s) {
   for (gbs_matrix *m : s->pool) gbs_matrix_destroy(m);
   for (int32_t *a : s->acc) (void)hipFree(a);
   if (s->d_counter) (void)hipFree(s->d_counter);
-  if (s->d_in) (void)hipFree(s->d_in);
-  if (s->d_out) (void)hipFree(s->d_out);
   if (s->h_in) (void)hipHostFree(s->h_in);
   if (s->h_out) (void)hipHostFree(s->h_out);
   for (int k = 0; k < 2; k++) {
@@ -220,7 +220,7 @@ int gbs_server_create(const gbs_matrix *const *weights, const uint32_t *shifts,
     s->shift.push_back(l + 1 < n_layers ? shifts[l] : 0);
   }
   // bytes one row of capacity costs across every pool and buffer
-  uint64_t per_row = s->h[0] + 8ull * s->h[n_layers];  // input / output staging
+  uint64_t per_row = s->h[0] + 4ull * s->h[n_layers];  // input / output staging
   for (uint32_t l = 0; l < n_layers; l++)
     per_row += act_bits * align_up(s->h[l], 64) / 8 + align_up(s->h[l], kValsColAlign) +
                4ull * s->h[l + 1];
@@ -270,8 +270,6 @@ int gbs_server_create(const gbs_matrix *const *weights, const uint32_t *shifts,
   if (!rc) {
     hipError_t e = hipMalloc((void **)&s->d_counter, 256);
     if (e == hipSuccess) e = hipMemset(s->d_counter, 0, 256);
-    if (e == hipSuccess) e = hipMalloc((void **)&s->d_in, R * s->h[0] + 16);
-    if (e == hipSuccess) e = hipMalloc((void **)&s->d_out, R * s->h[n_layers] * 4ull + 16);
     s->in_half = align_up(R * s->h[0] + 16, 256);
     s->out_half = align_up(R * s->h[n_layers] + 4, 64);  // int32 elements
     if (e == hipSuccess) e = hipHostMalloc((void **)&s->h_in, 2 * s->in_half);
@@ -340,12 +338,11 @@ int gbs_server_step(gbs_server *s, uint32_t *finished_now) {
   }
   s->cohort_rows[0] = n0;
   if (n0) {
-    GBS_HIP(hipMemcpyAsync(s->d_in, h_in, (size_t)n0 * s->h[0], hipMemcpyHostToDevice, st));
     ActJobs J{};
     J.nbits = s->act_bits;
     J.maxv = (1u << s->act_bits) - 1;
     gbs_matrix *p = s->pool[0];
-    J.j[0] = ActJob{s->d_in, nullptr, p->d_planes, p->d_vals, p->wpr(), p->wpp(),
+    J.j[0] = ActJob{h_in, nullptr, p->d_planes, p->d_vals, p->wpr(), p->wpp(),
                     p->vals_ld, n0, s->h[0], 0, 0, 0, nullptr};
     J.n = 1;
     const uint64_t units = (uint64_t)n0 * p->wpr();
@@ -393,7 +390,7 @@ int gbs_server_step(gbs_server *s, uint32_t *finished_now) {
   if (nl) {
     const uint64_t wpr = align_up(s->h[L], 64) / 64;
     J.j[J.n++] = ActJob{nullptr, s->acc[L - 1], nullptr, nullptr, wpr, 0, 0, nl, s->h[L], 2,
-                        0, units, s->d_out};
+                        0, units, h_out};
     units += (uint64_t)nl * wpr;
   }
   if (units) {
@@ -401,11 +398,8 @@ int gbs_server_step(gbs_server *s, uint32_t *finished_now) {
                        J, units);
     GBS_HIP(hipGetLastError());
   }
-  // 4. last layer's rows back to the host (read when this step's event has
-  // passed, in the next call)
-  if (nl)
-    GBS_HIP(hipMemcpyAsync(h_out, s->d_out, (size_t)nl * s->h[L] * 4,
-                           hipMemcpyDeviceToHost, st));
+  // 4. the last layer's rows are in h_out once this step's event has
+  // passed (read in the next call)
   GBS_HIP(hipEventRecord(s->ev[cur], st));
   s->completing[cur].swap(s->cohort[L - 1]);
   s->cohort[L - 1].clear();

@@ -84,10 +84,9 @@ def test_serve_admission_by_memory(gbs):
     h, Ws, shifts = small_mlp(5)
     # a budget that holds only ~10 rows per cohort: requests queue, not fail
     # (per row: every pool's planes and int8 copy, every layer's int32
-    # accumulators, the input staging, and the last layer's rows twice:
-    # host staging and the device output buffer)
+    # accumulators, the input and the last layer's output staging)
     per_row = sum(4 * ((x + 63) // 64 * 64) // 8 + (x + 255) // 256 * 256 for x in h[:-1]) \
-        + sum(4 * x for x in h[1:]) + h[0] + 8 * h[-1]
+        + sum(4 * x for x in h[1:]) + h[0] + 4 * h[-1]
     srv, mats = make_server(gbs, h, Ws, shifts, max_requests=64, max_rows=8,
                             mem_budget=per_row * 10)
     assert srv.stats()["row_capacity"] == 10
