@@ -168,8 +168,8 @@ static void rebalance(BatchProb *d, int n, double total_bytes) {
 
 // The serving loop's per-step batch: at most kMaxViews products whose
 // descriptors travel in the kernel arguments (no copy ahead of the
-// launch); the queue counter resets itself (the last block to leave zeroes
-// it and the exit counter beside it); split products add their pieces
+// launch); the caller zeroes the queue counter after the launch, with
+// work it launches next anyway; split products add their pieces
 // atomically into results the caller keeps zero between launches.
 constexpr int kMaxViews = 16;
 struct ViewBatch {
@@ -202,22 +202,13 @@ __global__ __launch_bounds__(256, 2) void views_kernel(ViewBatch V, unsigned int
     mfma_tile_by_shape<kFmtI8>(pl.shape, ops, pt, qt, kbeg, kend, pl.splits > 1, lds);
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    // this thread's last queue add returned before this add is issued (the
-    // loop left on its value), and both are agent-scope atomics: by the
-    // last exit, no block takes from the queue any more
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    if (atomicAdd(counters + 1, 1u) == gridDim.x - 1) {
-      atomicExch(counters, 0u);
-      atomicExch(counters + 1, 0u);
-    }
-  }
 }
 
-// Plan and launch products given as int8 operand views.  counters (two,
-// zero before the first launch) are left zero; the results of products the
-// plan splits along K must be zero before the launch.
-int launch_views(const MfmaOperands *views, int n, unsigned int *counters, hipStream_t s) {
+// Plan and launch products given as int8 operand views.  *counter must be
+// zero at the launch; the caller zeroes it again after the launch (in
+// stream order).  The results of products the plan splits along K must be
+// zero before the launch.
+int launch_views(const MfmaOperands *views, int n, unsigned int *counter, hipStream_t s) {
   if (n > kMaxViews) return fail(GBS_EINVAL, "launch_views: %d products (at most %d)", n, kMaxViews);
   ViewBatch V{};
   BatchProb *h = V.p;
@@ -247,7 +238,7 @@ int launch_views(const MfmaOperands *views, int n, unsigned int *counters, hipSt
   V.n = (uint32_t)m;
   V.total = tiles;
   const uint32_t grid = (uint32_t)(tiles < 512 ? tiles : 512);
-  hipLaunchKernelGGL(views_kernel, dim3(grid), dim3(256), 0, s, V, counters);
+  hipLaunchKernelGGL(views_kernel, dim3(grid), dim3(256), 0, s, V, counter);
   GBS_HIP(hipGetLastError());
   return GBS_OK;
 }

@@ -775,8 +775,8 @@ inline int shape_class(uint64_t M, uint64_t N, uint64_t K) {
 void mfma_config_for(uint64_t M, uint64_t N, uint64_t K, int fmt, int *shape, int *splits);
 
 // Persistent launch over at most 16 products given as int8 operand views
-// (gbs_batch.hip): counters are two words, zero before the first launch
-// and left zero; results of products split along K must be zero before.
-int launch_views(const MfmaOperands *views, int n, unsigned int *counters, hipStream_t s);
+// (gbs_batch.hip): *counter is zero at the launch and zeroed again by the
+// caller after it; results of products split along K must be zero before.
+int launch_views(const MfmaOperands *views, int n, unsigned int *counter, hipStream_t s);
 
 }  // namespace gbs

@@ -14,8 +14,8 @@
 //   2. one persistent launch computes, for every cohort in flight (cohort l
 //      sits at layer l), acc_l = pool_l * W_l^T -- all layers at once, so
 //      requests that arrived at different times share the device; its
-//      descriptors ride in the kernel arguments and its queue counter
-//      resets itself, so the launch is the step's only GEMM-side call;
+//      descriptors ride in the kernel arguments and the next launch zeroes
+//      its queue counter, so the launch is the step's only GEMM-side call;
 //   3. one launch requantizes acc_l -> min(2^act_bits - 1, max(0, acc >> s_l))
 //      and packs it (one __ballot per plane word, plus the int8 copy) into
 //      pool l+1, where cohort l sits next step, and writes the last layer's
@@ -86,7 +86,11 @@ struct ActJobs {
 };
 
 // One wave per (job, row, plane word): lane = column of that word.
-__global__ __launch_bounds__(256) void act_pack_kernel(ActJobs J, uint64_t units) {
+// reset (optional): the GEMM launch's queue counter, zeroed here because
+// this launch follows that one in stream order.
+__global__ __launch_bounds__(256) void act_pack_kernel(ActJobs J, uint64_t units,
+                                                       unsigned int *reset) {
+  if (reset && blockIdx.x == 0 && threadIdx.x == 0) *reset = 0;
   const uint32_t lane = threadIdx.x & 63;
   const uint64_t u = (uint64_t)blockIdx.x * 4 + threadIdx.x / 64;
   if (u >= units) return;
@@ -128,7 +132,7 @@ struct gbs_server {
   std::vector<uint32_t> h, shift;
   std::vector<gbs_matrix *> pool;
   std::vector<int32_t *> acc;                  // zero between steps
-  unsigned int *d_counter = nullptr;           // the launch's two queue words
+  unsigned int *d_counter = nullptr;           // the GEMM launch's queue counter
   // pinned host staging, two halves each, read and written by the kernels
   // in place (no copies): inputs of an admitted cohort, last-layer rows
   uint8_t *h_in = nullptr;
@@ -347,7 +351,7 @@ int gbs_server_step(gbs_server *s, uint32_t *finished_now) {
     J.n = 1;
     const uint64_t units = (uint64_t)n0 * p->wpr();
     hipLaunchKernelGGL(act_pack_kernel, dim3((uint32_t)((units + 3) / 4)), dim3(256), 0, st,
-                       J, units);
+                       J, units, (unsigned int *)nullptr);
     GBS_HIP(hipGetLastError());
   }
   // 2. every layer's GEMM in one persistent launch (weights re-imported
@@ -393,9 +397,11 @@ int gbs_server_step(gbs_server *s, uint32_t *finished_now) {
                         0, units, h_out};
     units += (uint64_t)nl * wpr;
   }
-  if (units) {
-    hipLaunchKernelGGL(act_pack_kernel, dim3((uint32_t)((units + 3) / 4)), dim3(256), 0, st,
-                       J, units);
+  // (every cohort the GEMM launch served has a job here, so units > 0
+  // whenever that launch ran and this one zeroes its queue counter)
+  if (units || nv) {
+    hipLaunchKernelGGL(act_pack_kernel, dim3((uint32_t)std::max<uint64_t>(1, (units + 3) / 4)),
+                       dim3(256), 0, st, J, units, s->d_counter);
     GBS_HIP(hipGetLastError());
   }
   // 4. the last layer's rows are in h_out once this step's event has
