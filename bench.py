@@ -110,9 +110,9 @@ CONFIGS = {"c0": ("plain GEMM 256x2048x256, lhs 2-bit unsigned, rhs 2-bit signed
 def valu_lane_ops(M, K, N, lb, rb):
     """AND/XOR + v_bcnt per 32-bit word per plane pair (tiled kernel); for
     products of at most 8 rhs rows and 4 planes a side (the skinny kernel)
-    4 AND + 4 v_bcnt + 2 adds per 128-bit slice per plane pair."""
+    4 AND + 4 v_bcnt + 1 multiply-add per 128-bit slice per plane pair."""
     if N <= 8 and lb <= 4 and rb <= 4:
-        return M * N * ((K + 127) // 128) * 10 * lb * rb
+        return M * N * ((K + 127) // 128) * 9 * lb * rb
     return M * N * ((K + 31) // 32) * 2 * lb * rb
 
 

@@ -362,7 +362,7 @@ gbs_algo select_algo(const Operand &l, const Operand &r, uint64_t K) {
   const double pr = l.bipolar && r.bipolar ? 1 : (double)r.nbits;
   // modelled times at the efficiencies these kernels reach (fractions of
   // the 39.3 T lane-op/s VALU, 5 POPS int8 MFMA and 8 TB/s HBM peaks)
-  const double valu = M * N * ((K + 31) / 32) * 2 * pl * pr / (0.55 * 39.3e12);
+  const double valu = M * N * ((K + 31) / 32) * 2 * pl * pr / (0.7 * 39.3e12);
   const double pbytes = (M * Kd * l.nbits + N * Kd * r.nbits) / 8 + 4 * M * N;
   const double t_popc = skinny_supported(l, r) && K < (1ull << 31) ? skinny_time(l, r, K)
                         : (valu > pbytes / (0.6 * 8e12) ? valu : pbytes / (0.6 * 8e12));

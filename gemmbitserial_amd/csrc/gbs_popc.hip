@@ -193,8 +193,8 @@ __global__ __launch_bounds__(256) void popc_kernel(PopcArgs a) {
 #pragma unroll
             for (int q = 0; q < PR; q++) {
               const uint64_t x = XOR ? (av[i][p] ^ bv[j][q]) : (av[i][p] & bv[j][q]);
-              cnt[i][j][p][q] += __builtin_popcount((uint32_t)x) +
-                                 __builtin_popcount((uint32_t)(x >> 32));
+              cnt[i][j][p][q] =
+                  bcnt_add((uint32_t)(x >> 32), bcnt_add((uint32_t)x, cnt[i][j][p][q]));
             }
     }
     if (more) swrite(buf ^ 1);
@@ -245,8 +245,7 @@ __global__ __launch_bounds__(256) void gemv_kernel(PopcArgs a) {
 #pragma unroll
               for (int p = 0; p < PL; p++) {
                 const uint64_t x = XOR ? (av[p] ^ bv) : (av[p] & bv);
-                c[n][p][q] += __builtin_popcount((uint32_t)x) +
-                              __builtin_popcount((uint32_t)(x >> 32));
+                c[n][p][q] = bcnt_add((uint32_t)(x >> 32), bcnt_add((uint32_t)x, c[n][p][q]));
               }
             }
           }
@@ -449,17 +448,17 @@ SkinnyProb skinny_plan(const Operand &l, const Operand &r, uint64_t K, int32_t *
   return p;
 }
 
-// Modelled time on the skinny kernel: the larger of its VALU work (and,
-// popcount, scaled add per 128-bit slice, row, rhs row and plane pair, at
-// half the 39.3 T lane-op/s peak) and its lhs plane bytes (at 70 % of the
-// 8 TB/s HBM peak).
+// Modelled time on the skinny kernel: the larger of its VALU work (4 AND,
+// 4 chained counts and one scaled add per 128-bit slice, row, rhs row and
+// plane pair, at 60 % of the 39.3 T lane-op/s peak) and its lhs plane bytes
+// (at 70 % of the 8 TB/s HBM peak).
 double skinny_time(const Operand &l, const Operand &r, uint64_t K) {
   const double M = (double)l.nrows, N = (double)r.nrows;
   const double kq = (double)((((K + 63) / 64 + 1) & ~1ull) / 2);
   const bool x = l.bipolar && r.bipolar;
   const double pl = x ? 1 : l.bipolar ? 2 : (double)l.nbits;
   const double pr = x ? 1 : r.bipolar ? 2 : (double)r.nbits;
-  const double valu = M * kq * N * pl * pr * 10 / (0.5 * 39.3e12);
+  const double valu = M * kq * N * pl * pr * 9 / (0.6 * 39.3e12);
   const double bytes = M * kq * 16 * (double)l.nbits / (0.7 * 8e12);
   return valu > bytes ? valu : bytes;
 }

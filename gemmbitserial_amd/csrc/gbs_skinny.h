@@ -18,7 +18,8 @@
 // straight from HBM as pairs of 8-byte words (one 16-byte load each),
 // G * 16 consecutive bytes per row, two slices in flight per lane.  Every
 // (lhs plane p, rhs plane q) pair adds W[p][q] * popc(a_p & b_q) to one
-// int32 accumulator per output, with
+// int32 accumulator per output (4 AND, 4 chained v_bcnt, one
+// v_mad_i32_i24 per 128-bit slice), with
 //
 //   W[p][q] = wl[p] * wr[q],  wl / wr = +-2^plane (the top plane of a
 //   signed operand weighs -2^(bits-1)),
@@ -75,6 +76,22 @@ double skinny_bytes(const Operand &l, const Operand &r, uint64_t K);
 double skinny_time(const Operand &l, const Operand &r, uint64_t K);
 
 #if defined(__HIPCC__)
+// acc + popcount(x) in one v_bcnt_u32_b32 (its second operand is an
+// addend).  Left to itself the compiler counts words from zero and adds
+// the counts with v_add3_u32, one VALU instruction more per two words.
+__device__ __forceinline__ uint32_t bcnt_add(uint32_t x, uint32_t acc) {
+  uint32_t r;
+  asm("v_bcnt_u32_b32 %0, %1, %2" : "=v"(r) : "v"(x), "v"(acc));
+  return r;
+}
+
+// acc + t * w for |t|, |w| < 2^23 in one v_mad_i32_i24.
+__device__ __forceinline__ int32_t mad_i24(int32_t t, int32_t w, int32_t acc) {
+  int32_t r;
+  asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"(t), "v"(w), "v"(acc));
+  return r;
+}
+
 __device__ __forceinline__ uint64_t skinny_valid_mask(uint32_t w, uint32_t K) {
   const uint32_t lo = w * 64;
   if (lo + 64 <= K) return ~0ull;
@@ -82,12 +99,11 @@ __device__ __forceinline__ uint64_t skinny_valid_mask(uint32_t w, uint32_t K) {
   return (1ull << (K - lo)) - 1;
 }
 
+// popcount of a & b (x: a ^ b) over 128 bits: four chained counts
 __device__ __forceinline__ uint32_t popc128(uint4 a, uint4 b, bool x) {
   if (x)
-    return __builtin_popcount(a.x ^ b.x) + __builtin_popcount(a.y ^ b.y) +
-           __builtin_popcount(a.z ^ b.z) + __builtin_popcount(a.w ^ b.w);
-  return __builtin_popcount(a.x & b.x) + __builtin_popcount(a.y & b.y) +
-         __builtin_popcount(a.z & b.z) + __builtin_popcount(a.w & b.w);
+    return bcnt_add(a.w ^ b.w, bcnt_add(a.z ^ b.z, bcnt_add(a.y ^ b.y, bcnt_add(a.x ^ b.x, 0u))));
+  return bcnt_add(a.w & b.w, bcnt_add(a.z & b.z, bcnt_add(a.y & b.y, bcnt_add(a.x & b.x, 0u))));
 }
 
 // Words w and w + 1 of a row (8-byte aligned; the compiler makes one
@@ -169,8 +185,8 @@ __device__ __forceinline__ void skinny_slices(const SkinnyProb &p, const int32_t
               qones ? m : lds[(q * kSkinnyMaxN + n) * skinny_piece_slices<RQ>() + su];
 #pragma unroll
           for (int pp = 0; pp < PL; pp++) {
-            acc[0][n] += __mul24((int32_t)popc128(a[u][0][pp], b, XOR), wt[pp][q]);
-            acc[1][n] += __mul24((int32_t)popc128(a[u][1][pp], b, XOR), wt[pp][q]);
+            acc[0][n] = mad_i24((int32_t)popc128(a[u][0][pp], b, XOR), wt[pp][q], acc[0][n]);
+            acc[1][n] = mad_i24((int32_t)popc128(a[u][1][pp], b, XOR), wt[pp][q], acc[1][n]);
           }
         }
       }
