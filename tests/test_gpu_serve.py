@@ -137,3 +137,19 @@ def test_serve_uses_reimported_weights(gbs):
     for r in reqs[1:]:
         assert np.array_equal(later[r["id"]][0], W.mlp_reference(r["x"], Ws2, shifts)), r["id"]
     srv.close()
+
+
+@pytest.mark.parametrize("layers", [1, 2])
+def test_serve_shallow_stacks(gbs, layers):
+    """One layer (its rows go straight out, no requantization) and two
+    layers: every request equals the exact forward pass, over enough steps
+    that accumulators are reused by later cohorts."""
+    h, Ws, shifts = small_mlp(40 + layers, layers=layers)
+    reqs = W.serving_requests(h[0], 9, 1000.0, seed=41, max_rows=12)
+    srv, mats = make_server(gbs, h, Ws, shifts, max_requests=3, max_rows=12)
+    for r in reqs:
+        srv.submit(r["id"], r["x"])
+    got = drain(srv, [r["id"] for r in reqs])
+    for r in reqs:
+        assert np.array_equal(got[r["id"]][0], W.mlp_reference(r["x"], Ws, shifts)), r["id"]
+    srv.close()
