@@ -189,6 +189,28 @@ def time_gemm(gbs, ws, l, r, d_res, algo, steps, warmup, stream):
     return e0.elapsed_ms(e1) / steps
 
 
+def time_dropin(gbs, l, r, M, N, reps):
+    """The drop-in call end to end, as gemmBitSerial makes it: one
+    synchronous gbs_gemm (AUTO engine) into pinned host memory (the drop-in
+    header allocates res with gbs_host_alloc) -- launch, kernels, result
+    copy over PCIe and the wait; host clock.  Returns (seconds, result)."""
+    import ctypes as C
+    p = C.c_void_p()
+    gbs.check(gbs.lib().gbs_host_alloc(4 * M * N, C.byref(p)))
+    try:
+        for _ in range(2):
+            gbs.check(gbs.lib().gbs_gemm(None, l.h, r.h, p, 0, gbs.AUTO))
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            gbs.check(gbs.lib().gbs_gemm(None, l.h, r.h, p, 0, gbs.AUTO))
+        dt = (time.perf_counter() - t0) / reps
+        got = np.ctypeslib.as_array((C.c_int32 * (M * N)).from_address(p.value))
+        got = got.reshape(N, M).copy()
+    finally:
+        gbs.lib().gbs_host_free(p)
+    return dt, got
+
+
 def run_config(gbs, O, name, runs, engines, steps, warmup, stream, ws, cpu_secs):
     out = []
     for enc, a, b, lb, rb, ls, rs in runs:
@@ -225,6 +247,14 @@ def run_config(gbs, O, name, runs, engines, steps, warmup, stream, ws, cpu_secs)
                                    "bit_exact": True}
             log(f"[bench] {name} {enc}: {eng} {ms * 1e3:.1f} us, "
                 f"{2.0 * M * K * N / sec / 1e12:.1f} TOPS, bit-exact")
+        e2e, got = time_dropin(gbs, l, r, M, N, max(3, min(20, steps)))
+        if want is not None and not np.array_equal(got, want):
+            raise SystemExit(f"{name} {enc}: drop-in call result differs from the exact product")
+        row["dropin_call"] = {"us": round(e2e * 1e6, 1), "result_bytes": 4 * M * N,
+                              "note": "synchronous gbs_gemm, AUTO, into pinned host memory: "
+                                      "launch + kernels + result copy + wait, host clock"}
+        log(f"[bench] {name} {enc}: drop-in call end to end {e2e * 1e6:.1f} us "
+            f"({4 * M * N / 1e6:.1f} MB result to the host)")
         if cpu_secs > 0 and O.ref_available():
             # very large lhs: the reference runs on its first rows only
             ms_ = max(1, min(M, (1 << 26) // max(K, 1)))
