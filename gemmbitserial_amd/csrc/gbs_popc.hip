@@ -340,10 +340,24 @@ int run_gemv(PopcArgs a, hipStream_t s) {
 
 constexpr int tdim(int P) { return P == 1 ? 8 : (P == 2 ? 4 : 2); }
 
+// Half the register tile each way when the full one leaves fewer than 512
+// blocks and the half one does not: a product that would otherwise split K
+// four ways (a handful of K tiles per block, a zeroed result and atomic
+// adds) runs unsplit.
+bool half_tiles(uint32_t M, uint32_t N, int pl, int pr) {
+  const uint64_t BM = 16 * tdim(pl), BN = 16 * tdim(pr);
+  const uint64_t full = ((M + BM - 1) / BM) * ((N + BN - 1) / BN);
+  const uint64_t half = ((M + BM / 2 - 1) / (BM / 2)) * ((N + BN / 2 - 1) / (BN / 2));
+  return full < 512 && half >= 512;
+}
+
 template <int PL, int PR, bool XOR>
 int run_pair(PopcArgs a, bool allow_split, hipStream_t s, bool *used_atomic) {
   *used_atomic = false;
   if (a.N <= 4 && a.mode != 2) return run_gemv<PL, PR, XOR>(a, s);
+  if (half_tiles(a.M, a.N, PL, PR))
+    return run_tiled<PL, PR, XOR, tdim(PL) / 2, tdim(PR) / 2, 8>(a, allow_split ? 64 : 1, s,
+                                                                 used_atomic);
   return run_tiled<PL, PR, XOR, tdim(PL), tdim(PR), 8>(a, allow_split ? 64 : 1, s,
                                                        used_atomic);
 }
@@ -364,7 +378,8 @@ int run_group(PopcArgs a, int pl, int pr, bool XOR, bool allow_split, hipStream_
 // Would the tiled launch for this shape split K (and so need atomics)?
 bool wants_split(uint32_t M, uint32_t N, uint32_t kwords, int pl, int pr) {
   if (N <= 4) return false;
-  const uint32_t BM = 16 * tdim(pl), BN = 16 * tdim(pr);
+  const uint32_t hf = half_tiles(M, N, pl, pr) ? 2 : 1;
+  const uint32_t BM = 16 * tdim(pl) / hf, BN = 16 * tdim(pr) / hf;
   const uint64_t tiles = (uint64_t)((M + BM - 1) / BM) * ((N + BN - 1) / BN);
   return tiles < 512 && kwords >= 64 && (uint64_t)kwords / 32 >= 2;
 }
