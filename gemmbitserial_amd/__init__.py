@@ -24,9 +24,9 @@ _NP_DT = {np.dtype(np.int8): I8, np.dtype(np.uint8): U8,
           np.dtype(np.int32): I32, np.dtype(np.uint32): U32,
           np.dtype(np.int64): I64, np.dtype(np.uint64): U64}
 # gbs_algo
-AUTO, POPCOUNT, MFMA, MFMA_FP4, MFMA_FP6 = 0, 1, 2, 3, 4
+AUTO, POPCOUNT, MFMA, MFMA_FP4, MFMA_FP6, MFMA_BITS = 0, 1, 2, 3, 4, 5
 ALGOS = {"auto": AUTO, "popcount": POPCOUNT, "mfma": MFMA, "mfma_fp4": MFMA_FP4,
-         "mfma_fp6": MFMA_FP6}
+         "mfma_fp6": MFMA_FP6, "mfma_bits": MFMA_BITS}
 
 EXPORTED = [
     "gbs_init", "gbs_last_error", "gbs_version", "gbs_host_alloc", "gbs_host_free",

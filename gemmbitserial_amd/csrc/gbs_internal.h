@@ -93,6 +93,12 @@ bool popc_needs_rowsum(const Operand &l, const Operand &r, uint64_t K);
 int launch_popc_gemm(const Operand &l, const Operand &r, uint64_t kwords,
                      uint64_t K, int32_t *d_res, const int32_t *d_rowsum,
                      hipStream_t s);
+// Matrix-core engine of two 1-bit operands on their planes (gbs_bits.hip):
+// no value copies; supported for nbits == 1 on both sides, K < 2^24 - 1024.
+bool bits_supported(const Operand &l, const Operand &r, uint64_t K);
+bool bits_preferred(const Operand &l, const Operand &r, uint64_t K);  // AUTO takes it
+int launch_bits_gemm(const Operand &l, const Operand &r, uint64_t K, int32_t *d_res,
+                     hipStream_t s);
 // Matrix-core engine (gbs_mfma.hip).  ensure_vals / ensure_f4 / ensure_f6
 // make the int8 / FP4 / FP6 copy current (allocating it on first use when
 // may_allocate); ensure_copy picks by format (0 int8, 1 FP4, 2 FP6).

@@ -381,6 +381,7 @@ gbs_algo select_algo(const Operand &l, const Operand &r, uint64_t K) {
   const double t_mfma_c = mops * waste;
   const double t_mfma = (t_mfma_c > ibytes / (0.6 * 8e12) ? t_mfma_c : ibytes / (0.6 * 8e12));
   if (t_mfma >= t_popc) return GBS_ALGO_POPCOUNT;
+  if (fmt == kFmtF4 && bits_preferred(l, r, K)) return GBS_ALGO_MFMA_BITS;
   return fmt == kFmtF4 ? GBS_ALGO_MFMA_FP4 : fmt == kFmtF6 ? GBS_ALGO_MFMA_FP6 : GBS_ALGO_MFMA;
 }
 

@@ -66,8 +66,11 @@ typedef enum gbs_algo {
   GBS_ALGO_MFMA = 2,     /* planes expanded to int8, v_mfma_i32_*_i8 */
   GBS_ALGO_MFMA_FP4 = 3, /* planes expanded to FP4 codes (operands of <= 2
                             bits, <= 3 signed), v_mfma_scale_f32_*_f8f6f4 */
-  GBS_ALGO_MFMA_FP6 = 4  /* planes expanded to FP6 codes (operands of <= 4
+  GBS_ALGO_MFMA_FP6 = 4, /* planes expanded to FP6 codes (operands of <= 4
                             bits, <= 5 signed), v_mfma_scale_f32_*_f8f6f4 */
+  GBS_ALGO_MFMA_BITS = 5 /* 1-bit x 1-bit only: the bit-planes themselves,
+                            each bit turned into an FP4 code in registers
+                            (no value copies), v_mfma_scale_f32_*_f8f6f4 */
 } gbs_algo;
 
 typedef struct gbs_matrix gbs_matrix;       /* device-resident bit-serial matrix */

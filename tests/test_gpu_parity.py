@@ -280,6 +280,55 @@ def test_skinny_engine_fill_and_fallback(gbs, orc):
             assert np.array_equal(got, O.wrap_i32(O.naive_gemm_i64(lhs, rhs)))
 
 
+# ------------------------------------------- 1-bit matrix-core engine on planes
+@pytest.mark.parametrize("ls,rs", list(itertools.product([False, True], repeat=2)))
+def test_bits_engine_every_1bit_encoding(gbs, orc, ls, rs):
+    """The matrix-core engine that reads 1-bit operands straight from their
+    planes (bits turned into FP4 codes in registers): bipolar and {0,1} on
+    either side; ragged tiles of both operands; depths around word,
+    1024-bit step and four-step boundaries (bipolar x bipolar padding is
+    subtracted); rows of an odd number of words; one result per engine."""
+    rng = np.random.default_rng(1300 + 2 * ls + rs)
+    shapes = [(1, 1, 1), (3, 6, 9), (17, 63, 1), (64, 64, 64), (65, 65, 63),
+              (130, 1000, 67), (64, 1024, 200), (200, 1025, 64), (129, 4095, 130),
+              (257, 4097, 70), (100, 8192, 33), (70, 20000, 150)]
+    for (M, K, N) in shapes:
+        say("bits", ls, rs, M, K, N)
+        lhs = rand_values(rng, (M, K), 1, ls)
+        rhs = rand_values(rng, (N, K), 1, rs)
+        want = O.wrap_i32(O.blas_gemm_exact(O.values_of(lhs, 1, ls), O.values_of(rhs, 1, rs)))
+        got = run_gemm(gbs, lhs, rhs, 1, 1, ls, rs, algo="mfma_bits")
+        assert np.array_equal(got, want)
+    for (M, K, N, ka) in [(50, 150, 70, 192), (77, 8190, 65, 8256), (140, 9000, 20, 9024)]:
+        say("bits odd pitch", ls, rs, M, K, N, ka)
+        lhs = rand_values(rng, (M, K), 1, ls)
+        rhs = rand_values(rng, (N, K), 1, rs)
+        got = run_gemm(gbs, lhs, rhs, 1, 1, ls, rs, algo="mfma_bits", geom=(M, N, M, ka, N, ka))
+        assert np.array_equal(got, O.wrap_i32(O.naive_gemm_i64(lhs, rhs)))
+
+
+def test_bits_engine_refuses_wider_operands(gbs, orc):
+    rng = np.random.default_rng(1310)
+    lhs = rand_values(rng, (8, 100), 2, False)
+    rhs = rand_values(rng, (8, 100), 1, True)
+    with pytest.raises(gbs.GbsError):
+        run_gemm(gbs, lhs, rhs, 2, 1, False, True, algo="mfma_bits")
+
+
+def test_bits_engine_mnist_golden(gbs, orc):
+    """The reference's MNIST layer (test/mnistdata.h: bipolar weights x a
+    2-bit unsigned input) through 1-bit products only: the input is its
+    two bit-planes x = x0 + 2 x1, so W x0 + 2 W x1 must equal the
+    reference's golden result."""
+    w, x, golden = mnist()
+    x = x.astype(np.int64).reshape(1, -1)
+    say("bits mnist")
+    y0 = run_gemm(gbs, w, (x & 1).astype(np.int8), 1, 1, True, False, algo="mfma_bits")
+    y1 = run_gemm(gbs, w, (x >> 1).astype(np.int8), 1, 1, True, False, algo="mfma_bits")
+    assert np.array_equal(y0.astype(np.int64) + 2 * y1.astype(np.int64),
+                          golden.reshape(1, -1).astype(np.int64))
+
+
 # ------------------------------------------------------------ int8 MFMA engine
 @pytest.mark.parametrize("lb,rb", list(itertools.product([1, 2, 3, 4], repeat=2)))
 @pytest.mark.parametrize("ls,rs", list(itertools.product([False, True], repeat=2)))
@@ -566,7 +615,9 @@ def test_fp6_every_tile_shape_and_split(gbs, orc):
 
 def test_auto_picks_format_from_support_and_measured_times(gbs, tmp_path):
     import os
-    cases = [((1, True), (1, True), "mfma_fp4"), ((2, False), (3, True), "mfma_fp4"),
+    # 1-bit x 1-bit at this depth and tile count: the matrix cores on the
+    # planes themselves (no copies)
+    cases = [((1, True), (1, True), "mfma_bits"), ((2, False), (3, True), "mfma_fp4"),
              ((3, False), (2, True), "mfma"), ((4, True), (4, False), "mfma"),
              ((6, True), (2, False), "mfma")]
     M, N, K = 2048, 512, 2048
@@ -591,7 +642,7 @@ def test_auto_picks_format_from_support_and_measured_times(gbs, tmp_path):
         assert gbs.select(mat(3, False, M), mat(2, True, N)) == "mfma_fp6"
         assert gbs.select(mat(4, True, M), mat(4, False, N)) == "mfma_fp6"
         assert gbs.select(mat(6, True, M), mat(2, False, N)) == "mfma"      # FP6 cannot hold it
-        assert gbs.select(mat(1, True, M), mat(1, True, N)) == "mfma_fp4"
+        assert gbs.select(mat(1, True, M), mat(1, True, N)) == "mfma_bits"
         t.write_text(f"{cls} 4 1 0 10.0\n{cls} 4 1 2 9.8\n")             # within 5 %: int8
         gbs.tune_load(str(t))
         assert gbs.select(mat(3, False, M), mat(2, True, N)) == "mfma"
@@ -606,7 +657,7 @@ def test_random_sweep_every_engine(gbs, orc):
     product of the values the oracle's import denotes (float64 BLAS, exact
     at these magnitudes; no int32 wrap is reachable)."""
     rng = np.random.default_rng(20261203)
-    engines = ["popcount", "mfma", "mfma_fp4", "mfma_fp6"]
+    engines = ["popcount", "mfma", "mfma_fp4", "mfma_fp6", "mfma_bits"]
     ran = {e: 0 for e in engines}
     for case in range(40):
         M, N = (int(np.exp(rng.uniform(0, np.log(1500)))) for _ in range(2))
@@ -628,4 +679,5 @@ def test_random_sweep_every_engine(gbs, orc):
             ran[e] += 1
             assert np.array_equal(got, want), (case, e)
         l.close(), r.close()
-    assert min(ran.values()) >= 3, ran
+    # the 1-bit engine holds a product only when both sides are 1-bit
+    assert min(v for e, v in ran.items() if e != "mfma_bits") >= 3, ran
