@@ -20,7 +20,7 @@ def test_more_than_one_gpu_is_refused_with_a_message():
     assert "one GPU" in r.stderr and r.stdout == ""
 
 
-@pytest.mark.parametrize("algo", ["mfma", "mfma_fp4", "mfma_fp6", "popcount"])
+@pytest.mark.parametrize("algo", ["mfma", "mfma_fp4", "mfma_fp6", "mfma_bits", "popcount"])
 def test_roofline_objects(algo):
     M, K, N = 4096, 8192, 512
     r = bench.roofline_of(algo, M, K, N, 1, 1, 20e-6)
@@ -55,6 +55,9 @@ def test_work_and_byte_models():
     assert bench.operand_bytes("mfma_fp6", 10, 128, 2, 1, 1) == 12 * 128 * 0.75 + 80
     assert bench.operand_bytes("mfma", 10, 128, 2, 1, 1) == 12 * 128 + 80
     assert bench.operand_bytes("popcount", 10, 128, 2, 3, 1) == bench.plane_bytes(10, 128, 2, 3, 1)
+    # the 1-bit matrix-core engine reads the planes, not copies
+    assert bench.operand_bytes("mfma_bits", 10, 128, 2, 1, 1) == bench.plane_bytes(10, 128, 2, 1, 1)
+    assert bench.ENGINE_DTYPE["mfma_bits"] == "fp4"
 
 
 def test_headline_traffic_comes_from_the_committed_counter_summary():
