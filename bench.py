@@ -51,7 +51,8 @@ MFMA_I8_PEAK = 5.0e15             # dense int8 ops/s
 MFMA_F4_PEAK = 10.1e15            # dense FP4 / FP6 ops/s
 # HBM bytes per launch of the headline kernel, measured with rocprofv3 --pmc
 # (FETCH_SIZE, WRITE_SIZE, scaled by a copy of known size)
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "mfma_fp4_binary_layers_traffic.txt")
+TRAFFIC_FILES = {"mfma_fp4": os.path.join(ROOT, "profiles", "mfma_fp4_binary_layers_traffic.txt"),
+                 "mfma_bits": os.path.join(ROOT, "profiles", "mfma_bits_binary_layers_traffic.txt")}
 
 
 def log(*a):
@@ -148,12 +149,14 @@ ENGINE_DTYPE = {"popcount": "u32", "mfma": "i8", "mfma_fp4": "fp4", "mfma_fp6": 
 
 
 def measured_traffic(engine):
-    """Bytes per launch from the committed counter summary, for the engine
-    it was measured on (the FP4 matrix-core kernel), else None."""
-    if engine != "mfma_fp4" or not os.path.exists(TRAFFIC_FILE):
+    """Bytes per launch from the committed counter summary of the headline
+    workload's kernel for this engine (the FP4-copy kernel or the 1-bit
+    planes kernel), else None."""
+    path = TRAFFIC_FILES.get(engine)
+    if path is None or not os.path.exists(path):
         return None
     kv = {}
-    for ln in open(TRAFFIC_FILE):
+    for ln in open(path):
         if "=" in ln and not ln.lstrip().startswith("#"):
             k, v = ln.split("=", 1)
             kv[k.strip()] = v.strip()

@@ -215,8 +215,8 @@ __global__ __launch_bounds__(256, 2) void bits_kernel(BitsProb p) {
 // conflicts.  The four waves own 32 x 32 quarters of the tile (2 x 2
 // fragments) and every super-step: no K split, no reduction.  Only full
 // super-steps go through the ring; a partly filled last one is loaded into
-// registers with bounds, as in bits_kernel.  Used where the K split of
-// bits_kernel would leave its waves fewer than two super-steps each.
+// registers with bounds, as in bits_kernel.  Used for depths of up to two
+// super-steps, where the K split of bits_kernel leaves waves idle.
 constexpr int kBitsStage = 2 * kBitsTile * 128;  // bytes of one super-step
 constexpr int kBitsRing = 4;                     // stages in LDS
 
@@ -344,10 +344,10 @@ __global__ __launch_bounds__(256, 2) void bits_dl_kernel(BitsProb p) {
 
 template <bool BR, bool BL>
 void run_bits(const BitsProb &p, bool even, uint32_t grid, hipStream_t s) {
-  // bits_kernel splits K over its four waves: below eight super-steps the
-  // waves would hold fewer than two each, and the ring kernel (all four
-  // waves on every super-step, loads three stages ahead) is faster
-  if (p.nss < 2 * kBitsWaves)
+  // bits_kernel splits K over its four waves: up to two super-steps the
+  // ring kernel (all four waves on every super-step, loads three stages
+  // ahead) is faster; from four on, the split
+  if (p.nss <= 2)
     hipLaunchKernelGGL((bits_dl_kernel<BR, BL>), dim3(grid), dim3(256), 0, s, p);
   else if (even)
     hipLaunchKernelGGL((bits_kernel<BR, BL, true>), dim3(grid), dim3(256), 0, s, p);
@@ -364,11 +364,11 @@ bool bits_supported(const Operand &l, const Operand &r, uint64_t K) {
 
 bool bits_preferred(const Operand &l, const Operand &r, uint64_t K) {
   // measured against the FP4-copy kernel on 1-bit products: 9-18 % faster
-  // at depths up to 2048 while the tiles fit the chip at once (at most two
-  // 64 x 64 tiles per CU); from 8192 on about even, and slower where many
-  // rounds of tiles follow each other
+  // at depths up to 2048, 2-10 % at 4096 and 8192, while the tiles fit the
+  // chip at once or twice (at most four 64 x 64 tiles per CU); slower at
+  // 32768 (bipolar) and where many rounds of tiles follow each other
   const uint64_t tiles = ((r.nrows + 63) / 64) * ((l.nrows + 63) / 64);
-  return bits_supported(l, r, K) && K <= 2048 && tiles <= 512;
+  return bits_supported(l, r, K) && K <= 8192 && tiles <= 1024;
 }
 
 int launch_bits_gemm(const Operand &l, const Operand &r, uint64_t K, int32_t *d_res,

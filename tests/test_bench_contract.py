@@ -60,13 +60,18 @@ def test_work_and_byte_models():
     assert bench.ENGINE_DTYPE["mfma_bits"] == "fp4"
 
 
-def test_headline_traffic_comes_from_the_committed_counter_summary():
-    t = bench.measured_traffic("mfma_fp4")
+@pytest.mark.parametrize("engine", ["mfma_fp4", "mfma_bits"])
+def test_headline_traffic_comes_from_the_committed_counter_summary(engine):
+    t = bench.measured_traffic(engine)
     assert isinstance(t, int) and t > 0
-    kv = {k.strip(): v.strip() for k, v in (ln.split("=", 1) for ln in open(bench.TRAFFIC_FILE)
+    kv = {k.strip(): v.strip() for k, v in (ln.split("=", 1) for ln in open(bench.TRAFFIC_FILES[engine])
                                             if "=" in ln and not ln.lstrip().startswith("#"))}
     fetch = float(kv["fetch_kib_per_dispatch"]) * float(kv["fetch_scale"]) * 1024
     write = float(kv["write_kib_per_dispatch"]) * float(kv["write_scale"]) * 1024
     # the summary lists the copy scales rounded to 3 decimals
     assert t == pytest.approx(fetch + write, rel=1e-3)
-    assert bench.measured_traffic("mfma") is None  # measured for the FP4 kernel only
+
+
+def test_traffic_is_null_for_engines_not_measured_on_the_headline():
+    assert bench.measured_traffic("mfma") is None
+    assert bench.measured_traffic("popcount") is None

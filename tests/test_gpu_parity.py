@@ -331,7 +331,7 @@ def test_bits_engine_mnist_golden(gbs, orc):
 
 def test_auto_takes_bits_engine_for_shallow_1bit_products(gbs, orc):
     """AUTO's rule for the 1-bit planes engine: both sides 1-bit, depth at
-    most 2048 and at most 512 tiles of 64 x 64; the FP4 copies otherwise."""
+    most 8192 and at most 1024 tiles of 64 x 64; the FP4 copies otherwise."""
     def sel(M, K, N, lb=1, rb=1):
         l, r = gbs.DeviceMatrix(lb, M, K, True), gbs.DeviceMatrix(rb, N, K, False)
         try:
@@ -339,10 +339,11 @@ def test_auto_takes_bits_engine_for_shallow_1bit_products(gbs, orc):
         finally:
             l.close(), r.close()
     say("bits auto rule")
-    assert sel(2048, 2048, 1024) == "mfma_bits"   # 32 x 16 = 512 tiles
-    assert sel(2048, 2049, 1024) == "mfma_fp4"    # deeper
-    assert sel(2048, 2048, 1025) == "mfma_fp4"    # 544 tiles
-    assert sel(2048, 2048, 1024, rb=2) != "mfma_bits"
+    assert sel(4096, 2048, 1024) == "mfma_bits"   # 64 x 16 = 1024 tiles
+    assert sel(4096, 8192, 512) == "mfma_bits"    # the binary layers
+    assert sel(4096, 8193, 512) == "mfma_fp4"     # deeper
+    assert sel(4096, 2048, 1025) == "mfma_fp4"    # 1088 tiles
+    assert sel(4096, 2048, 1024, rb=2) != "mfma_bits"
 
 
 # ------------------------------------------------------------ int8 MFMA engine
