@@ -49,8 +49,9 @@ VALU_PEAK = 256 * 64 * 2.4e9      # int32 lane-ops/s: 256 CUs x 64 lanes x 2.4 G
 HBM_PEAK = 8.0e12                 # B/s
 MFMA_I8_PEAK = 5.0e15             # dense int8 ops/s
 MFMA_F4_PEAK = 10.1e15            # dense FP4 / FP6 ops/s
-# HBM bytes per launch of the headline kernel, measured with rocprofv3 --pmc
-# (FETCH_SIZE, WRITE_SIZE, scaled by a copy of known size)
+# HBM bytes per launch of the headline workload's kernel, per engine that
+# runs it (the 1-bit planes kernel AUTO takes, the FP4-copy kernel), measured
+# with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE, scaled by a copy of known size)
 TRAFFIC_FILES = {"mfma_fp4": os.path.join(ROOT, "profiles", "mfma_fp4_binary_layers_traffic.txt"),
                  "mfma_bits": os.path.join(ROOT, "profiles", "mfma_bits_binary_layers_traffic.txt")}
 
