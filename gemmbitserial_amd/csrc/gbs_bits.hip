@@ -344,9 +344,10 @@ __global__ __launch_bounds__(256, 2) void bits_dl_kernel(BitsProb p) {
 
 template <bool BR, bool BL>
 void run_bits(const BitsProb &p, bool even, uint32_t grid, hipStream_t s) {
-  // bits_kernel splits K over its four waves: up to two super-steps the
-  // ring kernel (all four waves on every super-step, loads three stages
-  // ahead) is faster; from four on, the split
+  // bits_kernel splits K over its four waves, which leaves some idle up to
+  // three super-steps: up to two the ring kernel (all four waves on every
+  // super-step, loads three stages ahead) measured faster, from four the
+  // split (three: the split, not measured against the ring)
   if (p.nss <= 2)
     hipLaunchKernelGGL((bits_dl_kernel<BR, BL>), dim3(grid), dim3(256), 0, s, p);
   else if (even)
