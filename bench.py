@@ -61,52 +61,16 @@ def log(*a):
 
 
 # ------------------------------------------------------------- workloads
-def u(rng, shape, lo, hi):
-    return rng.integers(lo, hi + 1, shape).astype(np.int8)
+from gemmbitserial_amd import workloads as WL  # noqa: E402
 
-
-def cfg_c0(rng):
-    return [("2u x 2s", u(rng, (256, 2048), 0, 3), u(rng, (256, 2048), -2, 1), 2, 2, False, True)]
-
-
-def cfg_c1(rng):
-    M, K, N = 4096, 8192, 512
-    a_bip = np.where(rng.random((M, K)) < 0.5, 1, -1).astype(np.int8)
-    b_bip = np.where(rng.random((N, K)) < 0.5, 1, -1).astype(np.int8)
-    a_u = (rng.random((M, K)) < 0.3).astype(np.int8)
-    b_u = (rng.random((N, K)) < 0.3).astype(np.int8)
-    return [("bipolar", a_bip, b_bip, 1, 1, True, True),
-            ("unsigned01", a_u, b_u, 1, 1, False, False)]
-
-
-def sparse_act(rng, shape, hi):
-    v = rng.integers(1, hi + 1, shape).astype(np.int8)
-    v[rng.random(shape) < 0.5] = 0
-    return v
-
-
-def cfg_c2(rng):
-    w = u(rng, (256, 2304), -2, 1)
-    act = sparse_act(rng, (50176, 2304), 7)
-    w4 = u(rng, (256, 2304), -8, 7)
-    act4 = sparse_act(rng, (50176, 2304), 15)
-    return [("2s x 3u", w, act, 2, 3, True, False),
-            ("4s x 4u", w4, act4, 4, 4, True, False)]
-
-
-def cfg_c6(rng):
-    a = rng.integers(-2, 2, (32768, 32768), dtype=np.int8)
-    return [("gemv 1 row", a, rng.integers(0, 4, (1, 32768), dtype=np.int8), 2, 2, True, False),
-            ("gemv 8 rows", a, rng.integers(0, 4, (8, 32768), dtype=np.int8), 2, 2, True, False)]
-
-
-CONFIGS = {"c0": ("plain GEMM 256x2048x256, lhs 2-bit unsigned, rhs 2-bit signed", cfg_c0),
+CONFIGS = {"c0": ("plain GEMM 256x2048x256, lhs 2-bit unsigned, rhs 2-bit signed",
+                  WL.plain_gemm),
            "c1": ("binary layers: lhs 4096x8192 1-bit, rhs 512x8192 1-bit, bipolar "
-                  "(Bernoulli 0.5) + unsigned {0,1} (30% ones)", cfg_c1),
+                  "(Bernoulli 0.5) + unsigned {0,1} (30% ones)", WL.binary_layers),
            "c2": ("conv-as-GEMM: weights 256x2304, activations 50176x2304 (16x56x56 im2col, "
-                  "50% zeros), 2s x 3u and 4s x 4u", cfg_c2),
+                  "50% zeros), 2s x 3u and 4s x 4u", WL.conv_gemm),
            "c6": ("skinny: lhs 32768x32768 2-bit signed (268 MB of planes), rhs 1 and 8 rows "
-                  "2-bit unsigned", cfg_c6)}
+                  "2-bit unsigned", WL.skinny)}
 
 
 def valu_lane_ops(M, K, N, lb, rb):

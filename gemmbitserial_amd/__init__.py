@@ -17,12 +17,13 @@ import numpy as np
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libgbs_hip.so")
 
-# gbs_dtype
-I8, U8, U8_RAW, I16, U16, I32, U32, I64, U64 = range(9)
+# gbs_dtype (the floating types: threshold import only)
+I8, U8, U8_RAW, I16, U16, I32, U32, I64, U64, F32, F64 = range(11)
 _NP_DT = {np.dtype(np.int8): I8, np.dtype(np.uint8): U8,
           np.dtype(np.int16): I16, np.dtype(np.uint16): U16,
           np.dtype(np.int32): I32, np.dtype(np.uint32): U32,
-          np.dtype(np.int64): I64, np.dtype(np.uint64): U64}
+          np.dtype(np.int64): I64, np.dtype(np.uint64): U64,
+          np.dtype(np.float32): F32, np.dtype(np.float64): F64}
 # gbs_algo
 AUTO, POPCOUNT, MFMA, MFMA_FP4, MFMA_FP6, MFMA_BITS = 0, 1, 2, 3, 4, 5
 ALGOS = {"auto": AUTO, "popcount": POPCOUNT, "mfma": MFMA, "mfma_fp4": MFMA_FP4,
@@ -32,7 +33,7 @@ EXPORTED = [
     "gbs_init", "gbs_last_error", "gbs_version", "gbs_host_alloc", "gbs_host_free",
     "gbs_matrix_create", "gbs_matrix_destroy", "gbs_matrix_device_planes",
     "gbs_matrix_mark_modified",
-    "gbs_import", "gbs_import_quantize", "gbs_planes_from_host", "gbs_planes_to_host",
+    "gbs_import", "gbs_import_quantize", "gbs_import_quantize_typed", "gbs_planes_from_host", "gbs_planes_to_host",
     "gbs_export", "gbs_sum_rows", "gbs_workspace_create", "gbs_workspace_destroy",
     "gbs_gemm", "gbs_gemm_async", "gbs_workspace_reserve", "gbs_gemm_select",
     "gbs_stream", "gbs_stream_sync", "gbs_event_create", "gbs_event_destroy",
@@ -73,6 +74,7 @@ def lib():
         "gbs_matrix_mark_modified": [_p],
         "gbs_import": [_p, _p, _i, _i, _i, _p],
         "gbs_import_quantize": [_p, _p, _p, _i, _i, _i, _p],
+        "gbs_import_quantize_typed": [_p, _p, _p, _i, _i, _i, _i, _p],
         "gbs_planes_from_host": [_p, _p], "gbs_planes_to_host": [_p, _p],
         "gbs_export": [_p, _p, _i], "gbs_sum_rows": [_p, _p, _i],
         "gbs_workspace_create": [_p], "gbs_workspace_destroy": [_p],
@@ -223,12 +225,18 @@ class DeviceMatrix:
         return out
 
     def import_quantize(self, src, thresholds, col_major=False, want_planes=False):
-        src = np.ascontiguousarray(src, dtype=np.int32)
-        thr = np.ascontiguousarray(thresholds, dtype=np.int32)
+        """importRegularAndQuantize<T> on the GPU (gemmbitserial.hpp:281-308),
+        T = src.dtype; thresholds is nThres x nrows of the same type."""
+        src = np.ascontiguousarray(src)
+        thr = np.ascontiguousarray(thresholds, dtype=src.dtype)
+        if thr.ndim == 1:
+            thr = thr.reshape(0 if thr.size == 0 else 1, -1)
+        if src.size != self.nrows * self.ncols:
+            raise ValueError("source size does not match the matrix")
         out = np.zeros(self.words, np.uint64) if want_planes else None
-        check(lib().gbs_import_quantize(self.h, _ptr(src), _ptr(thr), thr.shape[0],
-                                        int(col_major), 0,
-                                        _ptr(out) if out is not None else None))
+        check(lib().gbs_import_quantize_typed(self.h, _ptr(src), _ptr(thr), _NP_DT[src.dtype],
+                                              thr.shape[0], int(col_major), 0,
+                                              _ptr(out) if out is not None else None))
         return out
 
     def planes(self):

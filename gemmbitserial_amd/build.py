@@ -7,7 +7,7 @@ programs in-tree with hipcc / g++ for gfx950.
 Outputs (git-ignored, shipped with the tree):
     gemmbitserial_amd/libgbs_hip.so
     gemmbitserial_amd/bin/test_dropin      (tests/cpp/test_dropin.cpp)
-    gemmbitserial_amd/bin/bench_dropin     (tests/cpp/bench_dropin.cpp)
+    gemmbitserial_amd/bin/geometry_probe   (tests/cpp/geometry_probe.cpp)
 """
 import argparse
 import concurrent.futures as cf
@@ -26,7 +26,7 @@ BIN = os.path.join(PKG, "bin")
 SOURCES = ["gbs_runtime.hip", "gbs_pack.hip", "gbs_popc.hip", "gbs_mfma.hip",
            "gbs_batch.hip", "gbs_serve.hip", "gbs_bits.hip"]
 CPP_PROGRAMS = {"test_dropin": "tests/cpp/test_dropin.cpp",
-                "bench_dropin": "tests/cpp/bench_dropin.cpp"}
+                "geometry_probe": "tests/cpp/geometry_probe.cpp"}
 
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
@@ -69,8 +69,6 @@ def build_cpp():
     out = []
     for name, src in CPP_PROGRAMS.items():
         path = os.path.join(ROOT, src)
-        if not os.path.exists(path):
-            continue
         exe = os.path.join(BIN, name)
         _run(["g++", "-O2", "-std=c++11", "-Wall", f"-I{INCLUDE}", path, "-o", exe + ".tmp",
               f"-L{PKG}", "-lgbs_hip", f"-Wl,-rpath,$ORIGIN/..", f"-Wl,-rpath,{ROCM}/lib"])

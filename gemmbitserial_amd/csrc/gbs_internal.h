@@ -58,7 +58,9 @@ int hip_fail(hipError_t e, const char *what);
 
 hipStream_t thread_stream();
 int ensure_device();
-int grow(void **p, size_t *cap, size_t need);
+// make *p hold at least need bytes; an outgrown buffer is freed after
+// waiting for stream s, the one the buffer serves
+int grow(void **p, size_t *cap, size_t need, hipStream_t s);
 
 // operand view handed to the GEMM launchers
 struct Operand {
@@ -77,9 +79,8 @@ int launch_pack(const gbs_matrix *m, const void *d_src, gbs_dtype dt,
                 bool col_major, hipStream_t s);
 int launch_pack_im2col(const gbs_matrix *m, const void *d_src, gbs_dtype dt,
                        const gbs_conv2d &g, uint32_t oh, uint32_t ow, hipStream_t s);
-int launch_pack_quantize(const gbs_matrix *m, const int32_t *d_src,
-                         const int32_t *d_thr, int nthres, bool col_major,
-                         hipStream_t s);
+int launch_pack_quantize(const gbs_matrix *m, const void *d_src, const void *d_thr,
+                         gbs_dtype dt, int nthres, bool col_major, hipStream_t s);
 int launch_export(const gbs_matrix *m, int32_t *d_dst, hipStream_t s);
 int launch_sum_rows(const Operand &o, uint64_t kwords, uint64_t ncols,
                     int32_t *d_dst, hipStream_t s);
@@ -126,7 +127,8 @@ constexpr uint64_t kValsRowAlign = 256;  // rows of the int8 copy
 constexpr uint64_t kValsColAlign = 256;  // values per row of the int8 copy (bytes)
 constexpr uint64_t kPackedColAlign = 512;  // values per row of the FP4 / FP6 copies
 
-size_t dtype_size(gbs_dtype dt);
+size_t dtype_size(gbs_dtype dt);      // integral element types (0 otherwise)
+size_t dtype_size_any(gbs_dtype dt);  // also the floating types
 
 // the GEMM enqueue shared by gbs_gemm, gbs_gemm_async and the batch engine
 int gemm_enqueue_public(gbs_workspace *ws, const gbs_matrix *l, const gbs_matrix *r,

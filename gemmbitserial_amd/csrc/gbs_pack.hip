@@ -27,6 +27,8 @@
 // low nbits equal the element modulo 2^nbits (the signed branch of quantize
 // adds 2^nbits before truncating to uint8), bipolar takes "element > 0",
 // and the raw overload takes the element's low bits as they are.
+#include <type_traits>
+
 #include "gbs_internal.h"
 
 namespace gbs {
@@ -65,15 +67,29 @@ struct ColMajorSrc {
   }
 };
 
-// importRegularAndQuantize: index of the first threshold >= e (ascending
-// per row), nthres when e exceeds all of them.
+// importRegularAndQuantize<T>: index of the first threshold >= e (ascending
+// per row), compared in T; nthres when e exceeds all of them; e itself when
+// there are no thresholds.  The value is then converted to uint8_t
+// (gemmbitserial.hpp:289-299).
+template <typename T>
+__device__ inline uint32_t to_u8(T e) {
+  if constexpr (std::is_floating_point<T>::value) {
+    // truncation toward zero; the low 8 bits of the int32 (0 beyond +-2^31, NaN)
+    return (e > (T)-2147483648.0 && e < (T)2147483648.0) ? (uint32_t)(uint8_t)(int32_t)e : 0u;
+  } else {
+    return (uint32_t)(uint8_t)e;
+  }
+}
+
+template <typename T>
 struct ThresholdSrc {
-  const int32_t *p, *thr;
+  const T *p, *thr;
   uint64_t nrows, ncols;
   int nthres;
   bool col_major;
   __device__ inline uint32_t code(uint64_t r, uint64_t c) const {
-    int32_t e = col_major ? p[c * nrows + r] : p[r * ncols + c];
+    const T e = col_major ? p[c * nrows + r] : p[r * ncols + c];
+    if (nthres == 0) return to_u8(e);
     uint32_t q = (uint32_t)nthres;
     for (int t = 0; t < nthres; t++) {
       if (e <= thr[(uint64_t)t * nrows + r]) { q = (uint32_t)t; break; }
@@ -482,8 +498,16 @@ size_t dtype_size(gbs_dtype dt) {
     case GBS_I16: case GBS_U16: return 2;
     case GBS_I32: case GBS_U32: return 4;
     case GBS_I64: case GBS_U64: return 8;
+    default: return 0;
   }
-  return 0;
+}
+
+size_t dtype_size_any(gbs_dtype dt) {
+  switch (dt) {
+    case GBS_F32: return 4;
+    case GBS_F64: return 8;
+    default: return dtype_size(dt);
+  }
 }
 
 int launch_pack(const gbs_matrix *m, const void *d_src, gbs_dtype dt,
@@ -518,25 +542,44 @@ int launch_pack_im2col(const gbs_matrix *m, const void *d_src, gbs_dtype dt,
   return fail(GBS_EINVAL, "gbs_import_im2col: unknown dtype %d", (int)dt);
 }
 
-int launch_pack_quantize(const gbs_matrix *m, const int32_t *d_src,
-                         const int32_t *d_thr, int nthres, bool col_major,
-                         hipStream_t s) {
+template <typename T>
+int pack_quantize_typed(const gbs_matrix *m, const void *d_src, const void *d_thr,
+                        int nthres, bool col_major, hipStream_t s) {
   const uint64_t W = m->wpp();
   if (W == 0) return GBS_OK;
-  ThresholdSrc src{d_src, d_thr, m->nrows, m->ncols, nthres, col_major};
+  ThresholdSrc<T> src{(const T *)d_src, (const T *)d_thr, m->nrows, m->ncols, nthres,
+                      col_major};
   if (!col_major) {
     const uint64_t blocks = ((W + kWave - 1) / kWave + 3) / 4;
-    hipLaunchKernelGGL((pack_rows_kernel<ThresholdSrc>), dim3((uint32_t)blocks), dim3(256),
+    hipLaunchKernelGGL((pack_rows_kernel<ThresholdSrc<T>>), dim3((uint32_t)blocks), dim3(256),
                        0, s, src, m->d_planes, m->nrows, m->ncols, m->wpr(), W,
                        (uint32_t)m->nbits);
   } else {
     const uint64_t waves = (m->nrows_a + kWave - 1) / kWave * m->wpr();
-    hipLaunchKernelGGL((pack_cols_kernel<ThresholdSrc>), dim3((uint32_t)((waves + 3) / 4)),
+    hipLaunchKernelGGL((pack_cols_kernel<ThresholdSrc<T>>), dim3((uint32_t)((waves + 3) / 4)),
                        dim3(256), 0, s, src, m->d_planes, m->nrows, m->ncols,
                        m->nrows_a, m->wpr(), W, (uint32_t)m->nbits);
   }
   GBS_HIP(hipGetLastError());
   return GBS_OK;
+}
+
+int launch_pack_quantize(const gbs_matrix *m, const void *d_src, const void *d_thr,
+                         gbs_dtype dt, int nthres, bool col_major, hipStream_t s) {
+  switch (dt) {
+    case GBS_I8: return pack_quantize_typed<int8_t>(m, d_src, d_thr, nthres, col_major, s);
+    case GBS_U8:
+    case GBS_U8_RAW: return pack_quantize_typed<uint8_t>(m, d_src, d_thr, nthres, col_major, s);
+    case GBS_I16: return pack_quantize_typed<int16_t>(m, d_src, d_thr, nthres, col_major, s);
+    case GBS_U16: return pack_quantize_typed<uint16_t>(m, d_src, d_thr, nthres, col_major, s);
+    case GBS_I32: return pack_quantize_typed<int32_t>(m, d_src, d_thr, nthres, col_major, s);
+    case GBS_U32: return pack_quantize_typed<uint32_t>(m, d_src, d_thr, nthres, col_major, s);
+    case GBS_I64: return pack_quantize_typed<int64_t>(m, d_src, d_thr, nthres, col_major, s);
+    case GBS_U64: return pack_quantize_typed<uint64_t>(m, d_src, d_thr, nthres, col_major, s);
+    case GBS_F32: return pack_quantize_typed<float>(m, d_src, d_thr, nthres, col_major, s);
+    case GBS_F64: return pack_quantize_typed<double>(m, d_src, d_thr, nthres, col_major, s);
+  }
+  return fail(GBS_EINVAL, "gbs_import_quantize_typed: unknown dtype %d", (int)dt);
 }
 
 int launch_export(const gbs_matrix *m, int32_t *d_dst, hipStream_t s) {

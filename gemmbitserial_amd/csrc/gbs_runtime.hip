@@ -87,10 +87,12 @@ hipStream_t thread_stream() {
   return t_stream;
 }
 
-int grow(void **p, size_t *cap, size_t need) {
+int grow(void **p, size_t *cap, size_t need, hipStream_t s) {
   if (*cap >= need && *p) return GBS_OK;
   if (*p) {
-    GBS_HIP(hipDeviceSynchronize());
+    // the buffer's last user is the stream it serves (a workspace serves one
+    // stream at a time): wait for that stream only
+    GBS_HIP(hipStreamSynchronize(s));
     GBS_HIP(hipFree(*p));
     *p = nullptr;
     *cap = 0;
@@ -148,7 +150,7 @@ int gemm_enqueue(gbs_workspace *ws, const gbs_matrix *l, const gbs_matrix *r,
     if (ws->rowsum_cap < need) {
       if (!may_allocate)
         return fail(GBS_ERANGE, "gemm_async: workspace too small, call gbs_workspace_reserve");
-      int rc = grow((void **)&ws->d_rowsum, &ws->rowsum_cap, need);
+      int rc = grow((void **)&ws->d_rowsum, &ws->rowsum_cap, need, s);
       if (rc) return rc;
     }
     int rc = launch_sum_rows(reg, kwords, K, ws->d_rowsum, s);
@@ -266,7 +268,7 @@ int gbs_matrix_device_planes(const gbs_matrix *m, uint64_t **d_planes) {
 // Stage a host source in the thread workspace; returns the device pointer.
 static int stage_host(const void *src, size_t bytes, const void **d_src, hipStream_t s) {
   gbs_workspace *ws = thread_ws();
-  int rc = grow(&ws->d_stage, &ws->stage_cap, bytes);
+  int rc = grow(&ws->d_stage, &ws->stage_cap, bytes, s);
   if (rc) return rc;
   GBS_HIP(hipMemcpyAsync(ws->d_stage, src, bytes, hipMemcpyHostToDevice, s));
   *d_src = ws->d_stage;
@@ -414,49 +416,51 @@ int gbs_matrix_load(const char *path, gbs_matrix **out) {
   return GBS_OK;
 }
 
+int gbs_import_quantize_typed(gbs_matrix *m, const void *src, const void *thresholds,
+                              gbs_dtype dt, int nthres, int read_col_major, int src_on_device,
+                              uint64_t *host_planes) {
+  int rc = check_matrix(m, "gbs_import_quantize_typed");
+  if (rc) return rc;
+  if (m->issigned)
+    return fail(GBS_EINVAL, "gbs_import_quantize_typed: threshold quantization needs an "
+                            "unsigned matrix (gemmbitserial.hpp:283)");
+  if (nthres < 0) return fail(GBS_EINVAL, "gbs_import_quantize_typed: nthres < 0");
+  const size_t es = dtype_size_any(dt);
+  if (es == 0) return fail(GBS_EINVAL, "gbs_import_quantize_typed: unknown dtype %d", (int)dt);
+  const size_t bytes = (size_t)(m->nrows * m->ncols) * es;
+  const size_t tbytes = (size_t)nthres * m->nrows * es;
+  if ((bytes && !src) || (tbytes && !thresholds))
+    return fail(GBS_EINVAL, "gbs_import_quantize_typed: null source or thresholds");
+  if ((rc = ensure_device())) return rc;
+  hipStream_t s = thread_stream();
+  const void *d_src = src, *d_thr = thresholds;
+  if (!src_on_device && (bytes || tbytes)) {
+    // both arrays in the thread's staging buffer, thresholds 16-byte aligned
+    gbs_workspace *ws = thread_ws();
+    const size_t off = (bytes + 15) & ~(size_t)15;
+    if ((rc = grow(&ws->d_stage, &ws->stage_cap, off + tbytes, s))) return rc;
+    if (bytes) GBS_HIP(hipMemcpyAsync(ws->d_stage, src, bytes, hipMemcpyHostToDevice, s));
+    if (tbytes)
+      GBS_HIP(hipMemcpyAsync((char *)ws->d_stage + off, thresholds, tbytes,
+                             hipMemcpyHostToDevice, s));
+    d_src = ws->d_stage;
+    d_thr = (const char *)ws->d_stage + off;
+  }
+  if ((rc = launch_pack_quantize(m, d_src, d_thr, dt, nthres, read_col_major != 0, s)))
+    return rc;
+  m->planes_version++;
+  if (host_planes && m->words())
+    GBS_HIP(hipMemcpyAsync(host_planes, m->d_planes, m->words() * 8,
+                           hipMemcpyDeviceToHost, s));
+  GBS_HIP(hipStreamSynchronize(s));
+  return GBS_OK;
+}
+
 int gbs_import_quantize(gbs_matrix *m, const int32_t *src, const int32_t *thresholds,
                         int nthres, int read_col_major, int src_on_device,
                         uint64_t *host_planes) {
-  int rc = check_matrix(m, "gbs_import_quantize");
-  if (rc) return rc;
-  if (m->issigned)
-    return fail(GBS_EINVAL, "gbs_import_quantize: threshold quantization needs an "
-                            "unsigned matrix (gemmbitserial.hpp:283)");
-  if (nthres < 0) return fail(GBS_EINVAL, "gbs_import_quantize: nthres < 0");
-  const size_t bytes = (size_t)(m->nrows * m->ncols) * 4;
-  const size_t tbytes = (size_t)nthres * m->nrows * 4;
-  if ((bytes && !src) || (tbytes && !thresholds))
-    return fail(GBS_EINVAL, "gbs_import_quantize: null source or thresholds");
-  if ((rc = ensure_device())) return rc;
-  hipStream_t s = thread_stream();
-  const int32_t *d_src = src, *d_thr = thresholds;
-  void *tmp = nullptr;
-  if (!src_on_device && (bytes || tbytes)) {
-    GBS_HIP(hipMalloc(&tmp, bytes + tbytes + 8));
-    hipError_t e = bytes ? hipMemcpyAsync(tmp, src, bytes, hipMemcpyHostToDevice, s)
-                         : hipSuccess;
-    if (e == hipSuccess && tbytes)
-      e = hipMemcpyAsync((char *)tmp + bytes, thresholds, tbytes, hipMemcpyHostToDevice, s);
-    if (e != hipSuccess) {
-      (void)hipStreamSynchronize(s);
-      (void)hipFree(tmp);
-      return hip_fail(e, "gbs_import_quantize: staging");
-    }
-    d_src = (const int32_t *)tmp;
-    d_thr = (const int32_t *)((char *)tmp + bytes);
-  }
-  rc = launch_pack_quantize(m, d_src, d_thr, nthres, read_col_major != 0, s);
-  m->planes_version++;
-  if (!rc && host_planes && m->words()) {
-    hipError_t e = hipMemcpyAsync(host_planes, m->d_planes, m->words() * 8,
-                                  hipMemcpyDeviceToHost, s);
-    if (e != hipSuccess) rc = hip_fail(e, "gbs_import_quantize: copy back");
-  }
-  hipError_t e = hipStreamSynchronize(s);
-  if (tmp) (void)hipFree(tmp);
-  if (rc) return rc;
-  if (e != hipSuccess) return hip_fail(e, "gbs_import_quantize");
-  return GBS_OK;
+  return gbs_import_quantize_typed(m, src, thresholds, GBS_I32, nthres, read_col_major,
+                                   src_on_device, host_planes);
 }
 
 int gbs_planes_from_host(gbs_matrix *m, const uint64_t *host_planes) {
@@ -505,7 +509,7 @@ static int to_i32(size_t n, int32_t *dst, int dst_on_device, F &&launch) {
   int32_t *d = dst;
   gbs_workspace *ws = thread_ws();
   if (!dst_on_device) {
-    if ((rc = grow((void **)&ws->d_res, &ws->res_cap, n * 4))) return rc;
+    if ((rc = grow((void **)&ws->d_res, &ws->res_cap, n * 4, s))) return rc;
     d = ws->d_res;
   }
   if ((rc = launch(d, s))) return rc;
@@ -558,7 +562,8 @@ int gbs_workspace_reserve(gbs_workspace *ws, const gbs_matrix *lhs, const gbs_ma
   if (rc) return rc;
   if ((rc = ensure_device())) return rc;
   const size_t rows = lhs->nrows > rhs->nrows ? lhs->nrows : rhs->nrows;
-  if ((rc = grow((void **)&ws->d_rowsum, &ws->rowsum_cap, rows * 4))) return rc;
+  if ((rc = grow((void **)&ws->d_rowsum, &ws->rowsum_cap, rows * 4, thread_stream())))
+    return rc;
   if (lhs->ncols == rhs->ncols) {
     hipStream_t s = thread_stream();
     if (mfma_supported(view(lhs), view(rhs), lhs->ncols)) {
@@ -590,7 +595,7 @@ int gbs_gemm(gbs_workspace *ws, const gbs_matrix *lhs, const gbs_matrix *rhs,
   hipStream_t s = thread_stream();
   int32_t *d = res;
   if (!res_on_device && n) {
-    if ((rc = grow((void **)&ws->d_res, &ws->res_cap, n * 4))) return rc;
+    if ((rc = grow((void **)&ws->d_res, &ws->res_cap, n * 4, s))) return rc;
     d = ws->d_res;
   }
   if ((rc = gemm_enqueue(ws, lhs, rhs, d, algo, s, true))) return rc;

@@ -16,6 +16,61 @@ def values(rng, shape, nbits, issigned):
     return rng.integers(0, 1 << nbits, shape).astype(np.int8)
 
 
+def uniform_i8(rng, shape, lo, hi):
+    """Uniform integers in [lo, hi] as int8."""
+    return rng.integers(lo, hi + 1, shape).astype(np.int8)
+
+
+def sparse_act(rng, shape, hi):
+    """Activations: 0 with probability 0.5, else uniform in [1, hi]."""
+    v = rng.integers(1, hi + 1, shape).astype(np.int8)
+    v[rng.random(shape) < 0.5] = 0
+    return v
+
+
+# The single-product configurations.  Each returns a list of runs
+# (encoding, lhs M x K, rhs N x K, lhs bits, rhs bits, lhs signed, rhs
+# signed), drawn from np.random.default_rng(SEED0 + config number).
+def plain_gemm(rng):
+    """Plain GEMM 256 x 2048 x 256: lhs 2-bit unsigned in [0, 3], rhs 2-bit
+    signed in [-2, 1]."""
+    return [("2u x 2s", uniform_i8(rng, (256, 2048), 0, 3), uniform_i8(rng, (256, 2048), -2, 1),
+             2, 2, False, True)]
+
+
+def binary_layers(rng):
+    """Binary layers 4096 x 8192 x 512, once bipolar (Bernoulli 0.5) and
+    once unsigned {0, 1} with 30 % ones."""
+    M, K, N = 4096, 8192, 512
+    a_bip = np.where(rng.random((M, K)) < 0.5, 1, -1).astype(np.int8)
+    b_bip = np.where(rng.random((N, K)) < 0.5, 1, -1).astype(np.int8)
+    a_u = (rng.random((M, K)) < 0.3).astype(np.int8)
+    b_u = (rng.random((N, K)) < 0.3).astype(np.int8)
+    return [("bipolar", a_bip, b_bip, 1, 1, True, True),
+            ("unsigned01", a_u, b_u, 1, 1, False, False)]
+
+
+def conv_gemm(rng):
+    """Conv-as-GEMM (im2col of a 3 x 3 conv over 256 channels, 16 x 56 x 56
+    outputs): weights 256 x 2304 2-bit signed in [-2, 1], activations
+    50176 x 2304 3-bit unsigned (50 % zeros, else [1, 7]); then at 4 x 4
+    bits (weights [-8, 7], activations 50 % zeros, else [1, 15])."""
+    w = uniform_i8(rng, (256, 2304), -2, 1)
+    act = sparse_act(rng, (50176, 2304), 7)
+    w4 = uniform_i8(rng, (256, 2304), -8, 7)
+    act4 = sparse_act(rng, (50176, 2304), 15)
+    return [("2s x 3u", w, act, 2, 3, True, False),
+            ("4s x 4u", w4, act4, 4, 4, True, False)]
+
+
+def skinny(rng):
+    """Skinny products: lhs 32768 x 32768 2-bit signed times 1 and 8 rhs
+    rows of 2-bit unsigned values."""
+    a = rng.integers(-2, 2, (32768, 32768), dtype=np.int8)
+    return [("gemv 1 row", a, rng.integers(0, 4, (1, 32768), dtype=np.int8), 2, 2, True, False),
+            ("gemv 8 rows", a, rng.integers(0, 4, (8, 32768), dtype=np.int8), 2, 2, True, False)]
+
+
 def log_uniform_int(rng, lo, hi):
     return int(round(float(np.exp(rng.uniform(np.log(lo), np.log(hi))))))
 

@@ -25,7 +25,8 @@
  *     calling thread's library stream).
  *   - Thread safety: calls on distinct objects from distinct threads are
  *     safe; each thread owns its own library stream.  Calls that share a
- *     gbs_workspace must not overlap.
+ *     gbs_workspace must not overlap, on the host or on the device (one
+ *     stream at a time per workspace).
  */
 #ifndef GBS_HIP_H
 #define GBS_HIP_H
@@ -50,10 +51,14 @@ typedef enum gbs_status {
  * GBS_U8 follows the reference's template importRegular<T> with
  * T = (const) uint8_t; GBS_U8_RAW follows its non-template
  * importRegular(uint8_t*) overload, which takes the raw low bits without the
- * bipolar "> 0" test (gemmbitserial.hpp:212-245 vs 249-275). */
+ * bipolar "> 0" test (gemmbitserial.hpp:212-245 vs 249-275).  The floating
+ * types are accepted by the threshold import only (its comparisons are
+ * meaningful for any ordered T; the plain import's quantize is defined for
+ * integral T only, gemmbitserial.hpp:169-186). */
 typedef enum gbs_dtype {
   GBS_I8 = 0, GBS_U8 = 1, GBS_U8_RAW = 2, GBS_I16 = 3, GBS_U16 = 4,
-  GBS_I32 = 5, GBS_U32 = 6, GBS_I64 = 7, GBS_U64 = 8
+  GBS_I32 = 5, GBS_U32 = 6, GBS_I64 = 7, GBS_U64 = 8,
+  GBS_F32 = 9, GBS_F64 = 10
 } gbs_dtype;
 
 /* GEMM engine.  AUTO picks per shape from the measured crossover; every
@@ -118,8 +123,10 @@ int gbs_matrix_device_planes(const gbs_matrix *m, uint64_t **d_planes);
  * reference counterpart (the reference has no derived copies). */
 int gbs_matrix_mark_modified(gbs_matrix *m);
 
-/* Pack a regular matrix into the planes on the GPU (one __ballot per plane
- * word).  src holds nrows x ncols elements of type dt, row-major, or
+/* Pack a regular matrix into the planes on the GPU (one thread per plane
+ * word or quarter word for row-major sources, plane bits gathered by u8
+ * dot products; one lane per row for column-major ones).  src holds
+ * nrows x ncols elements of type dt, row-major, or
  * column-major (src[c*nrows + r]) when read_col_major; src_on_device says
  * whether src is a device pointer.  When host_planes is non-NULL the packed
  * words (all nbits*nrows_a*ncols_a/64 of them) are copied there too.
@@ -128,11 +135,24 @@ int gbs_matrix_mark_modified(gbs_matrix *m);
 int gbs_import(gbs_matrix *m, const void *src, gbs_dtype dt, int read_col_major,
                int src_on_device, uint64_t *host_planes);
 
-/* Threshold-quantizing import for unsigned matrices: element e of row r
- * becomes the index of the first threshold t with e <= thr[t*nrows + r]
- * (nthres when none); int32 inputs and thresholds.
- * Replaces BitSerialMatrix::importRegularAndQuantize
+/* Threshold-quantizing import for unsigned matrices.  src and thresholds
+ * are both of element type dt (any gbs_dtype; GBS_U8_RAW reads as GBS_U8);
+ * thresholds holds thr[t*nrows + r] for t < nthres, ascending per row.
+ * Element e of row r becomes the index of the first threshold t with
+ * e <= thr[t*nrows + r], compared in dt, or nthres when e exceeds all of
+ * them; with nthres == 0 it stays e itself.  The low nbits of that value
+ * converted to uint8_t are packed: integers modulo 256; floating values
+ * truncated toward zero first (C++ leaves a float outside (-1, 256)
+ * undefined there; here it is the low 8 bits of the truncated int32, 0
+ * beyond +-2^31 and for NaN, which is what x86-64 code computes).
+ * Replaces BitSerialMatrix::importRegularAndQuantize<T>
  * (gemmbitserial.hpp:281-308). */
+int gbs_import_quantize_typed(gbs_matrix *m, const void *src,
+                              const void *thresholds, gbs_dtype dt, int nthres,
+                              int read_col_major, int src_on_device,
+                              uint64_t *host_planes);
+
+/* gbs_import_quantize_typed with int32 elements and thresholds. */
 int gbs_import_quantize(gbs_matrix *m, const int32_t *src,
                         const int32_t *thresholds, int nthres,
                         int read_col_major, int src_on_device,
@@ -197,8 +217,12 @@ int gbs_sum_rows(const gbs_matrix *m, int32_t *dst, int dst_on_device);
 
 /* A workspace holds the device result buffer and the row-sum scratch one
  * GEMM needs; it grows on demand.  NULL workspaces mean the calling
- * thread's own.  No reference counterpart (the reference GEMMContext owns
- * a host result, gemmbitserial.hpp:379-411). */
+ * thread's own.  A workspace serves one stream at a time: the popcount
+ * engine's bipolar x regular products write row sums into it on the
+ * call's stream, so calls that share one (NULL included) are enqueued on
+ * the same stream or separated by a synchronisation.  No reference
+ * counterpart (the reference GEMMContext owns a host result,
+ * gemmbitserial.hpp:379-411). */
 int gbs_workspace_create(gbs_workspace **out);
 int gbs_workspace_destroy(gbs_workspace *ws);
 

@@ -11,19 +11,23 @@
 //   deallocGEMMContext(ctx);
 //
 // What changes underneath:
-//   - importRegular packs on the GPU (one __ballot per plane word) into a
-//     device copy of the planes and copies the packed words back into
-//     `data`, so host accessors (get, word, printHex, ...) see them;
-//   - gemmBitSerial runs one fused kernel over all plane pairs (or the exact
-//     int8 MFMA engine) and copies the N x M result into `res`;
+//   - importRegular packs on the GPU (one thread per plane word or quarter
+//     word, plane bits gathered by byte dot products) into the device planes;
+//   - gemmBitSerial runs one kernel over all plane pairs (a matrix-core
+//     engine where it holds the product exactly, else the popcount engine)
+//     and copies the N x M result into `res`;
 //   - `data` and `res` are pinned host memory (gbs_host_alloc).
 //
 // Coherence.  BitSerialMatrix and GEMMContext are copied by value, as in the
 // reference; copies share `data` and a small control block holding the
-// device planes.  Host-side mutators (set, unset, clearAll, word, rowptr,
-// bitplaneptr, import32As4x8) mark the host copy newer; gemmBitSerial,
-// sumRows and exportRegular upload it first.  Code that writes through
-// `data` directly calls markHostDirty() (or syncToDevice()) afterwards.
+// device planes.  The host mirror `data` is filled lazily: after an import
+// the packed words stay on the device until a host accessor (get, set,
+// unset, word, rowptr, bitplaneptr, printHex, import32As4x8) or
+// syncToHost() needs them, so an import followed by gemmBitSerial moves no
+// planes over PCIe.  Host-side mutators mark the host copy newer;
+// gemmBitSerial, sumRows and exportRegular upload it first.  Code that reads
+// `data` directly calls syncToHost() first; code that writes it directly
+// calls markHostDirty() (or syncToDevice()) afterwards.
 //
 // Errors.  The reference asserts; here a failed device call throws
 // gemmbitserial::Error with the library's message.
@@ -70,10 +74,30 @@ inline gbs_dtype dtype_of() {
                         : (std::is_signed<U>::value ? GBS_I64 : GBS_U64);
 }
 
+// Element type -> gbs_dtype for importRegularAndQuantize<T>, which also
+// takes floating element types.
+template <typename U>
+inline gbs_dtype quantize_dtype_impl(std::true_type) {
+  static_assert(sizeof(U) == 4 || sizeof(U) == 8, "float or double");
+  return sizeof(U) == 4 ? GBS_F32 : GBS_F64;
+}
+template <typename U>
+inline gbs_dtype quantize_dtype_impl(std::false_type) {
+  return dtype_of<U>();
+}
+template <typename T>
+inline gbs_dtype quantize_dtype_of() {
+  typedef typename std::remove_cv<T>::type U;
+  static_assert(std::is_arithmetic<U>::value,
+                "importRegularAndQuantize needs an arithmetic element type");
+  return quantize_dtype_impl<U>(std::is_floating_point<U>());
+}
+
 // Shared by every copy of one BitSerialMatrix.
 struct Shared {
   gbs_matrix *dev;
-  bool host_dirty;  // host `data` was changed after the last device write
+  bool host_dirty;     // host `data` was changed after the last device write
+  bool device_newer;   // the device planes were written after `data` was filled
 };
 
 }  // namespace detail
@@ -101,6 +125,7 @@ class BitSerialMatrix {
     bsm.dev = new detail::Shared();
     bsm.dev->dev = nullptr;
     bsm.dev->host_dirty = false;
+    bsm.dev->device_newer = false;
     detail::check(gbs_matrix_create(nbits, nrows, ncols, issigned, bsm.nrows_a,
                                     bsm.ncols_a, &bsm.dev->dev),
                   "BitSerialMatrix::alloc");
@@ -139,6 +164,7 @@ class BitSerialMatrix {
   }
 
   void printHex() {
+    syncToHost();
     for (uint64_t i = 0; i < nbits; i++) {
       std::cout << "Bit " << i << ":" << std::endl;
       for (uint64_t j = 0; j < nrows_a; j++) {
@@ -156,11 +182,13 @@ class BitSerialMatrix {
   inline uint64_t wordsPerBitplane() const { return nrows_a * wordsPerRow(); }
 
   inline bool get(uint64_t bit, uint64_t row, uint64_t col) {
+    if (dev && dev->device_newer) syncToHost();
     return ((data[bit * wordsPerBitplane() + row * wordsPerRow() + (col >> 6)] >>
              bitpos(col)) & 1) == 1;
   }
   inline void clearAll() {
     memset(data, 0, nbits * wordsPerBitplane() * sizeof(uint64_t));
+    if (dev) dev->device_newer = false;  // every word overwritten
     markHostDirty();
   }
   inline void set(uint64_t bit, uint64_t row, uint64_t col) {
@@ -171,14 +199,17 @@ class BitSerialMatrix {
   }
   // Writable reference into the host mirror; the host copy becomes newer.
   inline uint64_t &word(uint64_t bit, uint64_t row, uint64_t col) {
+    syncToHost();
     markHostDirty();
     return data[bit * wordsPerBitplane() + row * wordsPerRow() + (col >> 6)];
   }
   inline uint64_t *rowptr(uint64_t bit, uint64_t row) {
+    syncToHost();
     markHostDirty();
     return &data[bit * wordsPerBitplane() + row * wordsPerRow()];
   }
   inline uint64_t *bitplaneptr(uint64_t bit) {
+    syncToHost();
     markHostDirty();
     return &data[bit * wordsPerBitplane()];
   }
@@ -193,6 +224,13 @@ class BitSerialMatrix {
     if (dev && dev->host_dirty) {
       detail::check(gbs_planes_from_host(dev->dev, data), "syncToDevice");
       dev->host_dirty = false;
+    }
+  }
+  // Fill the host mirror if the device planes are newer (after an import).
+  inline void syncToHost() {
+    if (dev && dev->device_newer) {
+      detail::check(gbs_planes_to_host(dev->dev, data), "syncToHost");
+      dev->device_newer = false;
     }
   }
 
@@ -235,25 +273,19 @@ class BitSerialMatrix {
   }
 
   // Threshold quantization for unsigned matrices; thresholds[t*nrows + r]
-  // ascending per row (gemmbitserial.hpp:281-308).
+  // ascending per row, compared in T (gemmbitserial.hpp:281-308).
   template <typename T>
   void importRegularAndQuantize(T *matrix, T *thresholds, int nThres,
                                 bool readColMajor = false) {
-    static_assert(std::is_integral<typename std::remove_cv<T>::type>::value &&
-                      sizeof(T) <= 4,
-                  "importRegularAndQuantize needs an integral type of <= 32 bits");
     if (issigned)
       throw Error("gemmbitserial: importRegularAndQuantize needs an unsigned matrix",
                   GBS_EINVAL);
-    // widen to the kernel's int32 element type
-    const uint64_t n = nrows * ncols, nt = (uint64_t)nThres * nrows;
-    int32_t *buf = new int32_t[n + nt + 1];
-    for (uint64_t i = 0; i < n; i++) buf[i] = (int32_t)matrix[i];
-    for (uint64_t i = 0; i < nt; i++) buf[n + i] = (int32_t)thresholds[i];
-    int rc = gbs_import_quantize(dev->dev, buf, buf + n, nThres, readColMajor, 0, data);
-    delete[] buf;
-    detail::check(rc, "importRegularAndQuantize");
+    detail::check(gbs_import_quantize_typed(dev->dev, matrix, thresholds,
+                                            detail::quantize_dtype_of<T>(), nThres,
+                                            readColMajor, 0, nullptr),
+                  "importRegularAndQuantize");
     dev->host_dirty = false;
+    dev->device_newer = true;
   }
 
   // Unpack into a row-major nrows x ncols matrix (gemmbitserial.hpp:312-333).
@@ -270,8 +302,9 @@ class BitSerialMatrix {
 
  private:
   void importFrom(const void *matrix, gbs_dtype dt, bool readColMajor) {
-    detail::check(gbs_import(dev->dev, matrix, dt, readColMajor, 0, data), "importRegular");
+    detail::check(gbs_import(dev->dev, matrix, dt, readColMajor, 0, nullptr), "importRegular");
     dev->host_dirty = false;
+    dev->device_newer = true;
   }
 };
 
@@ -338,6 +371,40 @@ class GEMMContext {
   inline bool isMatrixVector() const { return rhs.nrows == 1; }
 };
 
+// Block sizes and padded sizes allocGEMMContext_base chooses; a pure
+// function of the shape, so the layout can be checked without a device.
+struct ContextGeometry {
+  uint64_t lhsBlock, rhsBlock;
+  uint64_t lhs_nrows_a, lhs_ncols_a, rhs_nrows_a, rhs_ncols_a;
+};
+
+// The blocking steps of allocGEMMContext_base (gemmbitserial.hpp:415-447).
+static inline ContextGeometry contextGeometry_base(const uint64_t lhsRows, const uint64_t depth,
+                                            const uint64_t rhsRows,
+                                            const uint64_t regblock_lhs,
+                                            const uint64_t regblock_d,
+                                            const uint64_t regblock_rhs,
+                                            const uint64_t cacheBits) {
+  ContextGeometry g;
+  const uint64_t depth_al = alignTo(depth, regblock_d * 64);
+  computeBlockSize(regblock_lhs, regblock_rhs, cacheBits, depth_al, g.lhsBlock, g.rhsBlock);
+  if (g.lhsBlock > lhsRows || g.rhsBlock > rhsRows) {
+    g.lhsBlock = alignTo(lhsRows, regblock_lhs);
+    g.rhsBlock = alignTo(rhsRows, regblock_rhs);
+  } else {
+    if ((alignTo(lhsRows, g.lhsBlock) - lhsRows) > 0.1 * lhsRows)
+      g.lhsBlock = finetuneBlockSize(lhsRows, g.lhsBlock, regblock_lhs);
+    if ((alignTo(rhsRows, g.rhsBlock) - rhsRows) > 0.1 * rhsRows)
+      g.rhsBlock = finetuneBlockSize(rhsRows, g.rhsBlock, regblock_rhs);
+  }
+  // zero-row operands: keep alignment well defined (the reference divides by 0)
+  const uint64_t la = g.lhsBlock ? g.lhsBlock : 1, ra = g.rhsBlock ? g.rhsBlock : 1;
+  g.lhs_nrows_a = alignTo(lhsRows, la);
+  g.rhs_nrows_a = alignTo(rhsRows, ra);
+  g.lhs_ncols_a = g.rhs_ncols_a = alignTo(depth, regblock_d * 64);
+  return g;
+}
+
 // gemmbitserial.hpp:415-452
 static GEMMContext allocGEMMContext_base(const uint64_t lhsRows, const uint64_t depth,
                                          const uint64_t rhsRows, const uint64_t lhsBits,
@@ -347,26 +414,25 @@ static GEMMContext allocGEMMContext_base(const uint64_t lhsRows, const uint64_t 
                                          const uint64_t regblock_rhs,
                                          const uint64_t cacheBits) {
   GEMMContext ret;
-  const uint64_t depth_al = alignTo(depth, regblock_d * 64);
-  computeBlockSize(regblock_lhs, regblock_rhs, cacheBits, depth_al, ret.lhsBlock,
-                   ret.rhsBlock);
-  if (ret.lhsBlock > lhsRows || ret.rhsBlock > rhsRows) {
-    ret.lhsBlock = alignTo(lhsRows, regblock_lhs);
-    ret.rhsBlock = alignTo(rhsRows, regblock_rhs);
-  } else {
-    if ((alignTo(lhsRows, ret.lhsBlock) - lhsRows) > 0.1 * lhsRows)
-      ret.lhsBlock = finetuneBlockSize(lhsRows, ret.lhsBlock, regblock_lhs);
-    if ((alignTo(rhsRows, ret.rhsBlock) - rhsRows) > 0.1 * rhsRows)
-      ret.rhsBlock = finetuneBlockSize(rhsRows, ret.rhsBlock, regblock_rhs);
-  }
-  // zero-row operands: keep alignment well defined (the reference divides by 0)
-  const uint64_t la = ret.lhsBlock ? ret.lhsBlock : 1, ra = ret.rhsBlock ? ret.rhsBlock : 1;
+  const ContextGeometry g = contextGeometry_base(lhsRows, depth, rhsRows, regblock_lhs,
+                                                 regblock_d, regblock_rhs, cacheBits);
+  ret.lhsBlock = g.lhsBlock;
+  ret.rhsBlock = g.rhsBlock;
+  const uint64_t la = g.lhsBlock ? g.lhsBlock : 1, ra = g.rhsBlock ? g.rhsBlock : 1;
   ret.lhs = BitSerialMatrix::alloc(lhsBits, lhsRows, depth, lhsSigned, la, regblock_d * 64);
   ret.rhs = BitSerialMatrix::alloc(rhsBits, rhsRows, depth, rhsSigned, ra, regblock_d * 64);
   void *p = nullptr;
   detail::check(gbs_host_alloc(lhsRows * rhsRows * sizeof(int32_t), &p), "allocGEMMContext");
   ret.res = static_cast<int32_t *>(p);
   return ret;
+}
+
+// The generic register blocking of arch-generic.hpp:78-100 (2x1x2, or 1x4x1
+// for matrix-vector), as allocGEMMContext applies it.
+static inline ContextGeometry contextGeometry(uint64_t lhsRows, uint64_t depth, uint64_t rhsRows) {
+  const uint64_t cacheBits = 32 * 1024 * 8;
+  if (rhsRows == 1) return contextGeometry_base(lhsRows, depth, rhsRows, 1, 4, 1, cacheBits);
+  return contextGeometry_base(lhsRows, depth, rhsRows, 2, 1, 2, cacheBits);
 }
 
 // The generic register blocking of arch-generic.hpp:78-100 (2x1x2, or 1x4x1

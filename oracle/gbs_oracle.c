@@ -50,7 +50,8 @@
  * include/gbs_hip.h */
 enum {
   ORC_I8 = 0, ORC_U8 = 1, ORC_U8_RAW = 2, ORC_I16 = 3, ORC_U16 = 4,
-  ORC_I32 = 5, ORC_U32 = 6, ORC_I64 = 7, ORC_U64 = 8
+  ORC_I32 = 5, ORC_U32 = 6, ORC_I64 = 7, ORC_U64 = 8,
+  ORC_F32 = 9, ORC_F64 = 10  /* orc_import_and_quantize only */
 };
 
 static int dtype_size(int dt) {
@@ -202,30 +203,65 @@ int orc_import(const void *src, int dt, uint64_t nbits, int issigned,
   return 0;
 }
 
-/* gemmbitserial.hpp:281-308: unsigned-only threshold quantization.
- * thresholds[t * nrows + r] for t < nThres, sorted ascending per row.
- * src and thresholds are int64 here (the reference templates both on T). */
-int orc_import_and_quantize(const int64_t *src, const int64_t *thresholds,
+/* a[i] <= b[j], compared in the element type dt (gemmbitserial.hpp:292 compares
+ * two T values) */
+static int le_typed(const void *a, uint64_t i, const void *b, uint64_t j, int dt) {
+  switch (dt) {
+    case ORC_I8: return ((const int8_t *)a)[i] <= ((const int8_t *)b)[j];
+    case ORC_U8: case ORC_U8_RAW: return ((const uint8_t *)a)[i] <= ((const uint8_t *)b)[j];
+    case ORC_I16: return ((const int16_t *)a)[i] <= ((const int16_t *)b)[j];
+    case ORC_U16: return ((const uint16_t *)a)[i] <= ((const uint16_t *)b)[j];
+    case ORC_I32: return ((const int32_t *)a)[i] <= ((const int32_t *)b)[j];
+    case ORC_U32: return ((const uint32_t *)a)[i] <= ((const uint32_t *)b)[j];
+    case ORC_I64: return ((const int64_t *)a)[i] <= ((const int64_t *)b)[j];
+    case ORC_U64: return ((const uint64_t *)a)[i] <= ((const uint64_t *)b)[j];
+    case ORC_F32: return ((const float *)a)[i] <= ((const float *)b)[j];
+    case ORC_F64: return ((const double *)a)[i] <= ((const double *)b)[j];
+  }
+  return 0;
+}
+
+/* (uint8_t) of element i of type dt (gemmbitserial.hpp:298): integers modulo
+ * 256; floating values truncated toward zero, then the low 8 bits of the
+ * int32 (C leaves a float outside (-1, 256) undefined; x86-64 code converts
+ * through int32, giving 0 beyond +-2^31 and for NaN, which this follows). */
+static uint8_t u8_typed(const void *a, uint64_t i, int dt) {
+  double f;
+  switch (dt) {
+    case ORC_F32: f = ((const float *)a)[i]; break;
+    case ORC_F64: f = ((const double *)a)[i]; break;
+    default: return (uint8_t)load_elem(a, dt, i);
+  }
+  if (!(f > -2147483648.0 && f < 2147483648.0)) return 0;
+  return (uint8_t)(int32_t)f;
+}
+
+/* gemmbitserial.hpp:281-308: unsigned-only threshold quantization of an
+ * nrows x ncols matrix of element type dt (row-major, or col-major when
+ * read_col_major); thresholds[t * nrows + r] of the same type for
+ * t < nThres, sorted ascending per row.  The element becomes the index of
+ * the first threshold it does not exceed, nThres when it exceeds them all,
+ * and stays itself when nThres == 0 (the loop at :291 never runs); the low
+ * nbits of its uint8_t conversion are set. */
+int orc_import_and_quantize(const void *src, const void *thresholds, int dt,
                             int nThres, uint64_t nbits, uint64_t nrows,
                             uint64_t ncols, uint64_t nrows_a,
                             uint64_t ncols_a, int read_col_major,
                             uint64_t *planes) {
-  if (nbits < 1 || nbits > 8) return -1;
+  if (nbits < 1 || nbits > 8 || nThres < 0 || dt < ORC_I8 || dt > ORC_F64) return -1;
   const uint64_t wpr = ncols_a / 64;
   const uint64_t wpp = nrows_a * wpr;
   memset(planes, 0, nbits * wpp * sizeof(uint64_t));
   for (uint64_t r = 0; r < nrows; r++) {
     for (uint64_t c = 0; c < ncols; c++) {
-      int64_t e = read_col_major ? src[c * nrows + r] : src[r * ncols + c];
+      const uint64_t idx = read_col_major ? c * nrows + r : r * ncols + c;
+      uint8_t code = nThres == 0 ? u8_typed(src, idx, dt) : (uint8_t)nThres;
       for (int t = 0; t < nThres; t++) {
-        if (e <= thresholds[(uint64_t)t * nrows + r]) {
-          e = t;
+        if (le_typed(src, idx, thresholds, (uint64_t)t * nrows + r, dt)) {
+          code = (uint8_t)t;
           break;
-        } else if (t == nThres - 1) {
-          e = t + 1;
         }
       }
-      uint8_t code = (uint8_t)e;
       for (uint64_t b = 0; b < nbits; b++)
         if (code & (1u << b))
           planes[b * wpp + r * wpr + (c >> 6)] |= (uint64_t)1 << (c & 63);

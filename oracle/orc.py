@@ -20,11 +20,12 @@ ORC_SO = os.path.join(HERE, "liborc.so")
 REF_SO = os.path.join(HERE, "_ref", "libgbs_ref.so")
 
 # element-type codes shared with include/gbs_hip.h (gbs_dtype)
-I8, U8, U8_RAW, I16, U16, I32, U32, I64, U64 = range(9)
+I8, U8, U8_RAW, I16, U16, I32, U32, I64, U64, F32, F64 = range(11)
 _NP_DT = {np.dtype(np.int8): I8, np.dtype(np.uint8): U8,
           np.dtype(np.int16): I16, np.dtype(np.uint16): U16,
           np.dtype(np.int32): I32, np.dtype(np.uint32): U32,
-          np.dtype(np.int64): I64, np.dtype(np.uint64): U64}
+          np.dtype(np.int64): I64, np.dtype(np.uint64): U64,
+          np.dtype(np.float32): F32, np.dtype(np.float64): F64}
 
 
 def dtype_code(arr, raw_u8=False):
@@ -71,8 +72,8 @@ def orc():
         lib.orc_import.argtypes = [_p, C.c_int, _u64, C.c_int, _u64, _u64, _u64,
                                    _u64, C.c_int, _p]
         lib.orc_import_and_quantize.restype = C.c_int
-        lib.orc_import_and_quantize.argtypes = [_p, _p, C.c_int, _u64, _u64, _u64,
-                                                _u64, _u64, C.c_int, _p]
+        lib.orc_import_and_quantize.argtypes = [_p, _p, C.c_int, C.c_int, _u64, _u64,
+                                                _u64, _u64, _u64, C.c_int, _p]
         lib.orc_export.argtypes = [_p, _u64, C.c_int, _u64, _u64, _u64, _u64, _p]
         lib.orc_sum_rows.argtypes = [_p, _u64, C.c_int, _u64, _u64, _u64, _u64, _p]
         lib.orc_gemm.restype = C.c_int
@@ -95,6 +96,9 @@ def ref():
         lib.gbsref_import.restype = C.c_int
         lib.gbsref_import.argtypes = [_p, C.c_int, _u64, C.c_int, _u64, _u64, _u64,
                                       _u64, C.c_int, _p, _p]
+        lib.gbsref_import_quantize.restype = C.c_int
+        lib.gbsref_import_quantize.argtypes = [_p, _p, C.c_int, C.c_int, _u64, _u64, _u64,
+                                               _u64, _u64, C.c_int, _p, _p]
         lib.gbsref_export.argtypes = [_p, _u64, C.c_int, _u64, _u64, _u64, _u64, _p]
         lib.gbsref_sum_rows.argtypes = [_p, _u64, C.c_int, _u64, _u64, _u64, _u64, _p]
         lib.gbsref_alloc_geometry.argtypes = [_u64, _u64, _u64, _p]
@@ -180,18 +184,42 @@ def ref_import_regular(src, nbits, issigned, rowalign=1, colalign=64,
 
 def import_and_quantize(src, thresholds, nbits, nrows_a=None, ncols_a=None,
                         col_major=False):
-    src = np.ascontiguousarray(src, dtype=np.int64)
-    thr = np.ascontiguousarray(thresholds, dtype=np.int64)
+    """importRegularAndQuantize<T> (gemmbitserial.hpp:281-308) with T the
+    arrays' dtype; thresholds is nThres x nrows (thr[t, r]), same dtype."""
+    src = np.ascontiguousarray(src)
+    thr = np.ascontiguousarray(thresholds, dtype=src.dtype)
+    if thr.ndim == 1:
+        thr = thr.reshape(0 if thr.size == 0 else 1, -1)
     nrows, ncols = (src.shape[1], src.shape[0]) if col_major else src.shape
     nrows_a = nrows if nrows_a is None else nrows_a
     ncols_a = align_to(ncols, 64) if ncols_a is None else ncols_a
     planes = np.zeros(nbits * nrows_a * (ncols_a // 64), np.uint64)
-    rc = orc().orc_import_and_quantize(_ptr(src), _ptr(thr), thr.shape[0], nbits,
-                                       nrows, ncols, nrows_a, ncols_a,
+    rc = orc().orc_import_and_quantize(_ptr(src), _ptr(thr), dtype_code(src), thr.shape[0],
+                                       nbits, nrows, ncols, nrows_a, ncols_a,
                                        int(col_major), _ptr(planes))
     if rc != 0:
         raise ValueError("orc_import_and_quantize rejected the arguments")
     return Planes(planes, nbits, False, nrows, ncols, nrows_a, ncols_a)
+
+
+def ref_import_and_quantize(src, thresholds, nbits, rowalign=1, colalign=64,
+                            col_major=False):
+    """The compiled reference's importRegularAndQuantize<T> on the same input."""
+    lib = ref()
+    src = np.ascontiguousarray(src)
+    thr = np.ascontiguousarray(thresholds, dtype=src.dtype)
+    if thr.ndim == 1:
+        thr = thr.reshape(0 if thr.size == 0 else 1, -1)
+    nrows, ncols = (src.shape[1], src.shape[0]) if col_major else src.shape
+    geom = np.zeros(2, np.uint64)
+    args = (dtype_code(src), thr.shape[0], nbits, nrows, ncols, rowalign, colalign,
+            int(col_major))
+    lib.gbsref_import_quantize(_ptr(src), _ptr(thr), *args, None, _ptr(geom))
+    nra, nca = int(geom[0]), int(geom[1])
+    planes = np.zeros(nbits * nra * (nca // 64), np.uint64)
+    if lib.gbsref_import_quantize(_ptr(src), _ptr(thr), *args, _ptr(planes), _ptr(geom)):
+        raise ValueError("gbsref_import_quantize rejected the dtype")
+    return Planes(planes, nbits, False, nrows, ncols, nra, nca)
 
 
 # ---------------------------------------------------------- export / sumrows

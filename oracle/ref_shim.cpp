@@ -18,7 +18,7 @@ using namespace gemmbitserial;
 
 namespace {
 enum { I8 = 0, U8 = 1, U8_RAW = 2, I16 = 3, U16 = 4, I32 = 5, U32 = 6,
-       I64 = 7, U64 = 8 };
+       I64 = 7, U64 = 8, F32 = 9, F64 = 10 };
 
 // dispatch importRegular on the element type; U8 goes through the template
 // (T = const uint8_t), U8_RAW through the non-template uint8_t* overload
@@ -33,6 +33,29 @@ int do_import(BitSerialMatrix &m, const void *src, int dt, bool cm) {
     case U32: m.importRegular((const uint32_t *)src, cm); return 0;
     case I64: m.importRegular((const int64_t *)src, cm); return 0;
     case U64: m.importRegular((const uint64_t *)src, cm); return 0;
+  }
+  return -1;
+}
+template <typename T>
+void quantize_into(BitSerialMatrix &m, const void *src, const void *thr, int nthres,
+                   bool cm) {
+  m.importRegularAndQuantize((T *)const_cast<void *>(src), (T *)const_cast<void *>(thr),
+                             nthres, cm);
+}
+
+int do_import_quantize(BitSerialMatrix &m, const void *src, const void *thr, int dt,
+                       int nthres, bool cm) {
+  switch (dt) {
+    case I8: quantize_into<int8_t>(m, src, thr, nthres, cm); return 0;
+    case U8: case U8_RAW: quantize_into<uint8_t>(m, src, thr, nthres, cm); return 0;
+    case I16: quantize_into<int16_t>(m, src, thr, nthres, cm); return 0;
+    case U16: quantize_into<uint16_t>(m, src, thr, nthres, cm); return 0;
+    case I32: quantize_into<int32_t>(m, src, thr, nthres, cm); return 0;
+    case U32: quantize_into<uint32_t>(m, src, thr, nthres, cm); return 0;
+    case I64: quantize_into<int64_t>(m, src, thr, nthres, cm); return 0;
+    case U64: quantize_into<uint64_t>(m, src, thr, nthres, cm); return 0;
+    case F32: quantize_into<float>(m, src, thr, nthres, cm); return 0;
+    case F64: quantize_into<double>(m, src, thr, nthres, cm); return 0;
   }
   return -1;
 }
@@ -56,6 +79,25 @@ int gbsref_import(const void *src, int dt, uint64_t nbits, int issigned,
     rc = do_import(m, src, dt, col_major != 0);
     if (rc == 0)
       std::memcpy(planes, m.data, m.nbits * m.wordsPerBitplane() * 8);
+  }
+  BitSerialMatrix::dealloc(m);
+  return rc;
+}
+
+// BitSerialMatrix::importRegularAndQuantize<T> (gemmbitserial.hpp:281-308) on
+// an unsigned standalone matrix; src and thresholds of element type dt.
+int gbsref_import_quantize(const void *src, const void *thr, int dt, int nthres,
+                           uint64_t nbits, uint64_t nrows, uint64_t ncols,
+                           uint64_t rowalign, uint64_t colalign, int col_major,
+                           uint64_t *planes, uint64_t geom[2]) {
+  BitSerialMatrix m = BitSerialMatrix::alloc(nbits, nrows, ncols, false, rowalign,
+                                             colalign);
+  geom[0] = m.nrows_a;
+  geom[1] = m.ncols_a;
+  int rc = 0;
+  if (planes) {
+    rc = do_import_quantize(m, src, thr, dt, nthres, col_major != 0);
+    if (rc == 0) std::memcpy(planes, m.data, m.nbits * m.wordsPerBitplane() * 8);
   }
   BitSerialMatrix::dealloc(m);
   return rc;

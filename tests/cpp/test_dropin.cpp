@@ -6,6 +6,10 @@
 // semantics and host-side edits after import.
 //
 //   test_dropin [mnist.bin]     exit status 0 = all passed
+//   test_dropin --geometry      for each "M K N" on stdin, allocGEMMContext
+//                               and print "M K N lhsBlock rhsBlock
+//                               lhs.nrows_a lhs.ncols_a rhs.nrows_a
+//                               rhs.ncols_a" as allocated
 //
 // mnist.bin (optional): int32 n_out, n_in, then n_out*n_in int8 weights,
 // n_in uint8 inputs, n_out int32 golden outputs (written by the pytest
@@ -265,6 +269,97 @@ static void test_layout_and_coherence() {
   report("Layout and coherence tests", ok, nok);
 }
 
+// importRegularAndQuantize<T> (gemmbitserial.hpp:281-308): comparisons in
+// T (uint32 values and thresholds above 2^31, floats), no thresholds (the
+// element's own low bits), row- and column-major sources.
+template <typename T>
+static int threshold_case(size_t R, size_t C, int nthres, bool colmajor, T base, T step) {
+  const uint64_t nbits = 3;
+  std::vector<T> vals(R * C), thr((size_t)nthres * R + 1);
+  for (size_t i = 0; i < R * C; i++) vals[i] = (T)(base + (T)(rnd() % 40) * step);
+  for (size_t r = 0; r < R; r++) {
+    T t0 = (T)(base + (T)(rnd() % 10) * step);
+    for (int t = 0; t < nthres; t++) {
+      thr[(size_t)t * R + r] = t0;
+      t0 = (T)(t0 + (T)(1 + rnd() % 10) * step);
+    }
+  }
+  std::vector<T> src(R * C);
+  for (size_t r = 0; r < R; r++)
+    for (size_t c = 0; c < C; c++) src[colmajor ? c * R + r : r * C + c] = vals[r * C + c];
+  BitSerialMatrix m = BitSerialMatrix::alloc(nbits, R, C, false);
+  m.importRegularAndQuantize(src.data(), thr.data(), nthres, colmajor);
+  int bad = 0;
+  for (size_t r = 0; r < R; r++)
+    for (size_t c = 0; c < C; c++) {
+      const T e = vals[r * C + c];
+      uint32_t q;
+      if (nthres == 0) {
+        q = (uint8_t)e;
+      } else {
+        q = (uint32_t)nthres;
+        for (int t = 0; t < nthres; t++)
+          if (e <= thr[(size_t)t * R + r]) { q = (uint32_t)t; break; }
+      }
+      uint32_t got = 0;
+      for (uint64_t b = 0; b < nbits; b++) got |= (uint32_t)m.get(b, r, c) << b;
+      bad += got != (q & 7u);
+    }
+  BitSerialMatrix::dealloc(m);
+  return bad;
+}
+
+static void test_threshold_import() {
+  int ok = 0, nok = 0;
+  for (int nthres : {0, 1, 3})
+    for (int cm = 0; cm < 2; cm++) {
+      (threshold_case<uint32_t>(19, 77, nthres, cm, 0x80000000u, 1000u) ? nok : ok)++;
+      (threshold_case<uint32_t>(5, 130, nthres, cm, 0xfffff000u, 97u) ? nok : ok)++;
+      (threshold_case<int16_t>(7, 65, nthres, cm, -200, 11) ? nok : ok)++;
+      (threshold_case<float>(9, 100, nthres, cm, 0.25f, 0.5f) ? nok : ok)++;
+      (threshold_case<double>(3, 64, nthres, cm, 1.0, 0.125) ? nok : ok)++;
+      (threshold_case<uint8_t>(11, 70, nthres, cm, 3, 5) ? nok : ok)++;
+    }
+  report("Threshold import tests", ok, nok);
+}
+
+// The host mirror is filled lazily: after an import `data` is filled by
+// syncToHost() or the first host accessor, and equals the device planes.
+static void test_lazy_mirror() {
+  int ok = 0, nok = 0;
+  const size_t M = 70, K = 300;
+  std::vector<int8_t> a(M * K);
+  random_values(2, a.size(), a.data(), true);
+  BitSerialMatrix m = BitSerialMatrix::alloc(2, M, K, true, 4, 128);
+  m.importRegular(a.data());
+  std::vector<uint64_t> dev(m.nbits * m.wordsPerBitplane());
+  (gbs_planes_to_host(m.dev->dev, dev.data()) == GBS_OK ? ok : nok)++;
+  BitSerialMatrix copy = m;  // copies share the mirror and its state
+  copy.syncToHost();
+  (memcmp(m.data, dev.data(), dev.size() * 8) == 0 ? ok : nok)++;
+  // a fresh import marks the mirror stale again; word() fills it first
+  for (auto &v : a) v = (int8_t)(-v);
+  m.importRegular(a.data());
+  (gbs_planes_to_host(m.dev->dev, dev.data()) == GBS_OK ? ok : nok)++;
+  const uint64_t w = m.word(1, 3, 64);
+  (w == dev[1 * m.wordsPerBitplane() + 3 * m.wordsPerRow() + 1] ? ok : nok)++;
+  (memcmp(m.data, dev.data(), dev.size() * 8) == 0 ? ok : nok)++;
+  BitSerialMatrix::dealloc(m);
+  report("Lazy host mirror tests", ok, nok);
+}
+
+static void print_geometry() {
+  unsigned long long M, K, N;
+  while (scanf("%llu %llu %llu", &M, &K, &N) == 3) {
+    GEMMContext ctx = allocGEMMContext(M, K, N, 1, 1, false, false);
+    printf("%llu %llu %llu %llu %llu %llu %llu %llu %llu\n", M, K, N,
+           (unsigned long long)ctx.lhsBlock, (unsigned long long)ctx.rhsBlock,
+           (unsigned long long)ctx.lhs.nrows_a, (unsigned long long)ctx.lhs.ncols_a,
+           (unsigned long long)ctx.rhs.nrows_a, (unsigned long long)ctx.rhs.ncols_a);
+    deallocGEMMContext(ctx);
+  }
+}
+
 static void test_errors() {
   int ok = 0, nok = 0;
   GEMMContext ctx = allocGEMMContext(4, 10, 4, 2, 2, false, false);
@@ -283,6 +378,10 @@ static void test_errors() {
 }
 
 int main(int argc, char **argv) {
+  if (argc > 1 && strcmp(argv[1], "--geometry") == 0) {
+    print_geometry();
+    return 0;
+  }
   test_conversions();
   test_rowwise_sum();
   if (argc > 1) test_mnist(argv[1]);
@@ -291,6 +390,8 @@ int main(int argc, char **argv) {
   test_bipolar_times_bipolar();
   test_signed_mixed();
   test_layout_and_coherence();
+  test_threshold_import();
+  test_lazy_mirror();
   test_errors();
   printf(g_fail ? "Some tests failed\n" : "All tests completed successfully\n");
   return g_fail;

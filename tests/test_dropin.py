@@ -39,3 +39,23 @@ def test_dropin_suite(tmp_path):
     assert r.returncode == 0, r.stdout + r.stderr
     assert "All tests completed successfully" in r.stdout
     assert "MNIST matrix-vector using bipolar times regular: OK" in r.stdout
+
+
+@pytest.mark.gpu
+def test_dropin_allocated_geometry_matches_reference():
+    """allocGEMMContext through the header on the device: lhsBlock, rhsBlock
+    and both sides' nrows_a / ncols_a equal the compiled reference's for the
+    benchmark configurations' shapes, MNIST and the reference test grid."""
+    from oracle import orc as O
+    from test_oracle import GEOMETRY_SHAPES
+    assert os.path.exists(BIN)
+    text = "".join(f"{m} {k} {n}\n" for m, k, n in GEOMETRY_SHAPES)
+    r = subprocess.run([BIN, "--geometry"], input=text, capture_output=True, text=True,
+                       timeout=110)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = [tuple(int(x) for x in ln.split()) for ln in r.stdout.strip().splitlines()]
+    assert [ln[:3] for ln in lines] == GEOMETRY_SHAPES
+    for ln in lines:
+        assert ln[3:] == O.alloc_geometry(*ln[:3])
+        if O.ref_available():
+            assert ln[3:] == O.ref_alloc_geometry(*ln[:3])

@@ -9,6 +9,10 @@ import pytest
 from conftest import mnist, rand_values
 from oracle import orc as O
 
+
+def align(x, a):
+    return (x + a - 1) // a * a
+
 pytestmark = pytest.mark.gpu
 DIMS = [3, 5, 7, 16, 17, 18, 30, 31, 32, 100, 177, 256]  # test.cpp:112
 
@@ -90,16 +94,31 @@ def test_pack_overwrites_previous_content(gbs, orc):
     assert np.array_equal(got, O.import_regular(vals, 2, True, nrows_a=8, ncols_a=128).planes)
 
 
-def test_import_quantize(gbs, orc):
-    rng = np.random.default_rng(2)
-    for col_major in [False, True]:
-        vals = rng.integers(-50, 50, (6, 90))
-        thr = np.sort(rng.integers(-40, 40, (3, 6)), axis=0)
-        src = vals.T.copy() if col_major else vals
-        want = O.import_and_quantize(src, thr, 2, col_major=col_major)
-        m = gbs.DeviceMatrix(2, 6, 90, False)
-        got = m.import_quantize(src, thr, col_major=col_major, want_planes=True)
-        assert np.array_equal(got, want.planes)
+QUANT_DTYPES = [np.int8, np.uint8, np.int16, np.uint16, np.int32, np.uint32, np.int64,
+                np.uint64, np.float32, np.float64]
+
+
+@pytest.mark.parametrize("dtype", QUANT_DTYPES, ids=lambda d: np.dtype(d).name)
+@pytest.mark.parametrize("nthres", [0, 1, 3])
+def test_import_quantize(gbs, orc, dtype, nthres):
+    """importRegularAndQuantize<T> on the GPU equals the oracle word for
+    word (the oracle equals the compiled reference, tests/test_oracle.py):
+    comparisons in T (uint32 / uint64 values straddling 2^31 / 2^63,
+    floats), no thresholds (raw low bits), row- and column-major, ragged
+    shapes, padding rows."""
+    from test_oracle import threshold_inputs
+    rng = np.random.default_rng(nthres * 13 + np.dtype(dtype).itemsize)
+    for nrows, ncols in [(6, 90), (37, 200), (1, 64), (70, 3)]:
+        vals, thr = threshold_inputs(rng, dtype, nrows, ncols, nthres)
+        for col_major in (False, True):
+            src = np.ascontiguousarray(vals.T) if col_major else vals
+            for nbits in (2, 8):
+                want = O.import_and_quantize(src, thr, nbits, nrows_a=nrows + 3,
+                                             ncols_a=align(ncols, 128), col_major=col_major)
+                m = gbs.DeviceMatrix(nbits, nrows, ncols, False, nrows + 3, align(ncols, 128))
+                got = m.import_quantize(src, thr, col_major=col_major, want_planes=True)
+                m.close()
+                assert np.array_equal(got, want.planes), (nrows, ncols, col_major, nbits)
 
 
 @pytest.mark.parametrize("nbits,issigned", list(itertools.product([1, 2, 3, 4, 8], [True, False])))

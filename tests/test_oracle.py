@@ -3,11 +3,12 @@ against the reference compiled from its own sources (oracle/_ref), against
 independent naive products, against the MNIST known-answer vector the
 reference's tests hold, and against closed-form identities."""
 import itertools
+import os
 
 import numpy as np
 import pytest
 
-from conftest import mnist, rand_values
+from conftest import ROOT, mnist, rand_values
 from oracle import orc as O
 
 needs_ref = pytest.mark.skipif(not O.ref_available(),
@@ -115,15 +116,107 @@ def test_sum_rows_closed_form(orc):
         assert np.array_equal(O.sum_rows(p), vals.astype(np.int64).sum(1))
 
 
-def test_import_and_quantize(orc):
-    rng = np.random.default_rng(2)
-    vals = rng.integers(-50, 50, (6, 90))
-    thr = np.sort(rng.integers(-40, 40, (3, 6)), axis=0)
-    p = O.import_and_quantize(vals, thr, 2)
-    got = O.export_regular(p)
-    want = np.array([[int(np.searchsorted(thr[:, r], vals[r, c], side="left"))
-                      for c in range(90)] for r in range(6)])
-    assert np.array_equal(got, want)
+def threshold_inputs(rng, dtype, nrows, ncols, nthres):
+    """Values and ascending per-row thresholds of a dtype, spread over the
+    type's range so that comparisons differ from int32-widened ones (uint32
+    and uint64 values above the signed maximum, negative and fractional
+    floats)."""
+    dt = np.dtype(dtype)
+    if dt.kind == "f":
+        vals = rng.uniform(-3.0, 300.0, (nrows, ncols)).astype(dt)
+        vals[rng.random((nrows, ncols)) < 0.1] = 0.5
+        thr = np.sort(rng.uniform(-3.0, 300.0, (nthres, nrows)), axis=0).astype(dt)
+        return vals, thr
+    info = np.iinfo(dt)
+    lo, hi = int(info.min), int(info.max)
+    if dt.kind == "u" and dt.itemsize >= 4:
+        lo = hi // 2 - 4096        # straddle 2^31 (2^63): int32 widening flips these
+        hi = hi // 2 + 4096
+    vals = rng.integers(lo, hi, (nrows, ncols), dtype=dt, endpoint=True)
+    thr = np.sort(rng.integers(lo, hi, (nthres, nrows), dtype=dt, endpoint=True), axis=0)
+    if nthres:
+        vals[:, 0] = thr[0]          # equal to a threshold: "<=" takes it
+    return vals, thr
+
+
+QUANT_DTYPES = [np.int8, np.uint8, np.int16, np.uint16, np.int32, np.uint32, np.int64,
+                np.uint64, np.float32, np.float64]
+
+
+@pytest.mark.parametrize("dtype", QUANT_DTYPES, ids=lambda d: np.dtype(d).name)
+@pytest.mark.parametrize("nthres", [0, 1, 3, 5])
+def test_import_and_quantize_matches_reference(orc, dtype, nthres):
+    """Word for word against the compiled reference's importRegularAndQuantize<T>
+    (gemmbitserial.hpp:281-308), row- and column-major, ragged shapes."""
+    if not O.ref_available():
+        pytest.skip("oracle/_ref not built")
+    rng = np.random.default_rng(nthres * 31 + np.dtype(dtype).itemsize)
+    for nrows, ncols in [(6, 90), (17, 129), (1, 64), (33, 3)]:
+        vals, thr = threshold_inputs(rng, dtype, nrows, ncols, nthres)
+        for col_major in (False, True):
+            src = np.ascontiguousarray(vals.T) if col_major else vals
+            for nbits in (1, 3, 8):
+                got = O.import_and_quantize(src, thr, nbits, col_major=col_major)
+                want = O.ref_import_and_quantize(src, thr, nbits, col_major=col_major)
+                assert (got.nrows_a, got.ncols_a) == (want.nrows_a, want.ncols_a)
+                assert np.array_equal(got.planes, want.planes), (nrows, ncols, col_major, nbits)
+
+
+@pytest.mark.parametrize("dtype", QUANT_DTYPES, ids=lambda d: np.dtype(d).name)
+def test_import_and_quantize_closed_form(orc, dtype):
+    """With ascending thresholds the code is the number of thresholds below
+    the element (searchsorted, left side); with none it is the element
+    itself modulo 256 (truncated toward zero for floats)."""
+    rng = np.random.default_rng(np.dtype(dtype).itemsize * 7)
+    for nthres in (0, 1, 3, 7):
+        vals, thr = threshold_inputs(rng, dtype, 9, 150, nthres)
+        p = O.import_and_quantize(vals, thr, 8)
+        got = O.export_regular(p)
+        if nthres:
+            want = np.array([[np.searchsorted(thr[:, r], vals[r, c], side="left")
+                              for c in range(vals.shape[1])] for r in range(vals.shape[0])])
+        elif np.dtype(dtype).kind == "f":
+            want = np.trunc(vals.astype(np.float64)).astype(np.int64) % 256
+        else:
+            want = vals.astype(object) % 256
+        assert np.array_equal(got, np.asarray(want, dtype=np.int64))
+
+
+def test_import_and_quantize_catches_int32_widening(orc):
+    """A planted mistake -- comparing uint32 values as int32 -- changes the
+    planes, so the reference comparison above would catch it."""
+    vals = np.array([[0x80000005, 7, 0xfffffff0]], np.uint32)
+    thr = np.array([[0x80000000], [0x90000000]], np.uint32)
+    good = O.export_regular(O.import_and_quantize(vals, thr, 2))
+    widened = O.export_regular(O.import_and_quantize(vals.view(np.int32), thr.view(np.int32), 2))
+    assert good.tolist() == [[1, 0, 2]]
+    assert not np.array_equal(good, widened)
+
+
+# ------------------------------------------------------------ geometry probe
+GEOMETRY_SHAPES = [(256, 2048, 256), (4096, 8192, 512), (256, 2304, 50176), (256, 784, 1),
+                   (64, 256, 1), (4096, 8192, 256), (1089, 1808, 64), (3886, 2374, 1),
+                   (3, 6, 9), (5, 10, 15), (17, 34, 51), (177, 354, 531), (256, 512, 768),
+                   (100, 200, 1), (2, 3, 4), (1, 1, 1), (9216, 4096, 1), (96, 363, 3025)]
+
+
+def test_header_geometry_matches_reference(orc):
+    """The drop-in header's allocGEMMContext geometry (lhsBlock, rhsBlock,
+    nrows_a, ncols_a of both sides), computed by tests/cpp/geometry_probe.cpp
+    from include/gemmbitserial.hpp, equals the compiled reference's."""
+    import subprocess
+    probe = os.path.join(ROOT, "gemmbitserial_amd", "bin", "geometry_probe")
+    assert os.path.exists(probe), "build the C++ programs first (python -m gemmbitserial_amd.build)"
+    text = "".join(f"{m} {k} {n}\n" for m, k, n in GEOMETRY_SHAPES)
+    r = subprocess.run([probe], input=text, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    lines = [tuple(int(x) for x in ln.split()) for ln in r.stdout.strip().splitlines()]
+    assert len(lines) == len(GEOMETRY_SHAPES)
+    for (m, k, n), got in zip(GEOMETRY_SHAPES, lines):
+        assert got[:3] == (m, k, n)
+        assert got[3:] == O.alloc_geometry(m, k, n), (m, k, n)
+        if O.ref_available():
+            assert got[3:] == O.ref_alloc_geometry(m, k, n), (m, k, n)
 
 
 # -------------------------------------------------------------------- gemm
