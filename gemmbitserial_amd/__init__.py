@@ -43,7 +43,7 @@ EXPORTED = [
     "gbs_tune_classes", "gbs_tune_class_of", "gbs_tune_set", "gbs_tune_load",
     "gbs_clock_ns", "gbs_server_create", "gbs_server_destroy", "gbs_server_submit",
     "gbs_server_step", "gbs_server_take", "gbs_server_stats",
-    "gbs_gemm_async_config",
+    "gbs_gemm_async_config", "gbs_gemm_async_noalloc",
     "gbs_conv2d_shape", "gbs_import_im2col", "gbs_matrix_save", "gbs_matrix_load",
 ]
 
@@ -89,6 +89,7 @@ def lib():
         "gbs_tune_classes": [_p, _p], "gbs_tune_class_of": [_u64, _u64, _u64, _p],
         "gbs_tune_set": [_p, _p, _i], "gbs_tune_load": [C.c_char_p],
         "gbs_gemm_async_config": [_p, _p, _p, _p, _i, _i, _i, _p],
+        "gbs_gemm_async_noalloc": [_p, _p, _p, _i, _p],
         "gbs_clock_ns": [_p], "gbs_server_create": [_p, _p, _i, _i, _p, _p],
         "gbs_server_destroy": [_p], "gbs_server_submit": [_p, _u64, _p, _i, C.c_int64],
         "gbs_server_step": [_p, _p],

@@ -23,6 +23,20 @@ thread_local int t_stream_device = -1;
 thread_local gbs_workspace *t_ws = nullptr;
 }  // namespace
 
+// Scratch of the allocation-free GEMM entry point (gbs_gemm_async_noalloc):
+// the regular operand's row sums of a bipolar x regular product on the
+// popcount engine.  Every device the library runs on holds its own copy
+// until the process ends; host code uses the address hipGetSymbolAddress
+// returns for the current device, never the array's name.
+constexpr size_t kScratchInts = GBS_NOALLOC_MAX_ROWSUM_ROWS;
+static __device__ __align__(256) int32_t scratch[kScratchInts];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+int32_t *g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
 int fail(int code, const char *fmt, ...) {
   char buf[512];
   va_list ap;
@@ -113,10 +127,13 @@ static int check_matrix(const gbs_matrix *m, const char *fn) {
   return GBS_OK;
 }
 
-// Shared body of gbs_gemm / gbs_gemm_async / the batch fallbacks.
-int gemm_enqueue(gbs_workspace *ws, const gbs_matrix *l, const gbs_matrix *r,
-                        int32_t *d_res, gbs_algo algo, hipStream_t s,
-                        bool may_allocate) {
+// Shared body of gbs_gemm / gbs_gemm_async / the batch fallbacks and the
+// allocation-free entry point.  Row sums of the regular operand (popcount
+// engine, bipolar x regular) go to ws's scratch, grown when may_allocate,
+// or, when ws is null, to `rowsum_buf` (rowsum_cap bytes, never grown).
+static int gemm_enqueue_into(gbs_workspace *ws, const gbs_matrix *l, const gbs_matrix *r,
+                             int32_t *d_res, gbs_algo algo, hipStream_t s,
+                             bool may_allocate, int32_t *rowsum_buf, size_t rowsum_cap) {
   if (l->ncols != r->ncols)
     return fail(GBS_EINVAL, "gemm: lhs has %llu columns, rhs %llu (depths must match)",
                 (unsigned long long)l->ncols, (unsigned long long)r->ncols);
@@ -147,17 +164,46 @@ int gemm_enqueue(gbs_workspace *ws, const gbs_matrix *l, const gbs_matrix *r,
   if (popc_needs_rowsum(lo, ro, K) && l->nrows && r->nrows && kwords) {
     const Operand &reg = lo.bipolar ? ro : lo;
     const size_t need = reg.nrows * sizeof(int32_t);
-    if (ws->rowsum_cap < need) {
-      if (!may_allocate)
-        return fail(GBS_ERANGE, "gemm_async: workspace too small, call gbs_workspace_reserve");
-      int rc = grow((void **)&ws->d_rowsum, &ws->rowsum_cap, need, s);
-      if (rc) return rc;
+    int32_t *buf = rowsum_buf;
+    if (ws) {
+      if (ws->rowsum_cap < need) {
+        if (!may_allocate)
+          return fail(GBS_ERANGE, "gemm_async: workspace too small, call gbs_workspace_reserve");
+        int rc = grow((void **)&ws->d_rowsum, &ws->rowsum_cap, need, s);
+        if (rc) return rc;
+      }
+      buf = ws->d_rowsum;
+    } else if (need > rowsum_cap) {
+      return fail(GBS_ERANGE, "gemm: the regular operand's %llu rows exceed the %llu of the "
+                              "row-sum scratch", (unsigned long long)reg.nrows,
+                  (unsigned long long)(rowsum_cap / sizeof(int32_t)));
     }
-    int rc = launch_sum_rows(reg, kwords, K, ws->d_rowsum, s);
+    int rc = launch_sum_rows(reg, kwords, K, buf, s);
     if (rc) return rc;
-    d_rowsum = ws->d_rowsum;
+    d_rowsum = buf;
   }
   return launch_popc_gemm(lo, ro, kwords, K, d_res, d_rowsum, s);
+}
+
+int gemm_enqueue(gbs_workspace *ws, const gbs_matrix *l, const gbs_matrix *r,
+                 int32_t *d_res, gbs_algo algo, hipStream_t s, bool may_allocate) {
+  return gemm_enqueue_into(ws, l, r, d_res, algo, s, may_allocate, nullptr, 0);
+}
+
+// Device address of the current device's scratch array, looked up once per
+// device (with that device current).
+static int scratch_of_device(int dev, int32_t **out) {
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(GBS_ERANGE, "device %d beyond the %d the library tracks", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[dev] = static_cast<int32_t *>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return GBS_OK;
 }
 
 int gemm_enqueue_public(gbs_workspace *ws, const gbs_matrix *l, const gbs_matrix *r,
@@ -182,7 +228,8 @@ int gbs_init(int device) {
   t_device = device;
   GBS_HIP(hipSetDevice(device));
   if (!thread_stream()) return fail(GBS_EDEVICE, "gbs_init: stream creation failed");
-  return GBS_OK;
+  int32_t *sp = nullptr;
+  return scratch_of_device(device, &sp);
 }
 
 const char *gbs_last_error(void) { return t_err.c_str(); }
@@ -615,6 +662,38 @@ int gbs_gemm_async(gbs_workspace *ws, const gbs_matrix *lhs, const gbs_matrix *r
   if ((rc = ensure_device())) return rc;
   hipStream_t s = stream ? (hipStream_t)stream : thread_stream();
   return gemm_enqueue(ws, lhs, rhs, d_res, algo, s, false);
+}
+
+int gbs_gemm_async_noalloc(const gbs_matrix *lhs, const gbs_matrix *rhs, int32_t *d_res,
+                           gbs_algo algo, void *stream) {
+  int rc = check_matrix(lhs, "gbs_gemm_async_noalloc");
+  if (!rc) rc = check_matrix(rhs, "gbs_gemm_async_noalloc");
+  if (rc) return rc;
+  if (lhs->nrows * rhs->nrows && !d_res)
+    return fail(GBS_EINVAL, "gbs_gemm_async_noalloc: null result pointer");
+  if (lhs->ncols != rhs->ncols)
+    return fail(GBS_EINVAL, "gbs_gemm_async_noalloc: depths differ (%llu, %llu)",
+                (unsigned long long)lhs->ncols, (unsigned long long)rhs->ncols);
+  if ((rc = ensure_device())) return rc;
+  int32_t *sp = nullptr;
+  if ((rc = scratch_of_device(t_device, &sp))) return rc;
+  hipStream_t s = stream ? (hipStream_t)stream : thread_stream();
+  gbs_algo eng = algo;
+  if (algo == GBS_ALGO_AUTO) {
+    // AUTO among the engines that need nothing allocated: a matrix-core
+    // engine whose value copies both operands already hold, else the
+    // engines that read the planes themselves
+    const Operand lo = view(lhs), ro = view(rhs);
+    eng = select_algo(lo, ro, lhs->ncols);
+    const bool have =
+        eng == GBS_ALGO_MFMA_FP4   ? (lhs->d_f4 && rhs->d_f4)
+        : eng == GBS_ALGO_MFMA_FP6 ? (lhs->d_f6 && rhs->d_f6)
+        : eng == GBS_ALGO_MFMA     ? (lhs->d_vals && rhs->d_vals)
+                                   : true;
+    if (!have) eng = bits_supported(lo, ro, lhs->ncols) ? GBS_ALGO_MFMA_BITS : GBS_ALGO_POPCOUNT;
+  }
+  return gemm_enqueue_into(nullptr, lhs, rhs, d_res, eng, s, false, sp,
+                           kScratchInts * sizeof(int32_t));
 }
 
 int gbs_gemm_select(const gbs_matrix *lhs, const gbs_matrix *rhs, gbs_algo *chosen) {

@@ -244,6 +244,28 @@ int gbs_gemm_async(gbs_workspace *ws, const gbs_matrix *lhs,
                    const gbs_matrix *rhs, int32_t *d_res, gbs_algo algo,
                    void *stream);
 
+/* Rows of the regular operand the allocation-free entry point's row-sum
+ * scratch holds (1 MiB of int32 per device). */
+#define GBS_NOALLOC_MAX_ROWSUM_ROWS 262144
+
+/* The product of gbs_gemm_async, enqueued without allocating anything: no
+ * hipMalloc, no workspace.  No reference counterpart; it is for callers
+ * that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation).  The caller passes the operands and
+ * the device result buffer d_res (N*M int32).  Row sums of the regular
+ * operand of a bipolar x regular product on the popcount engine go to a
+ * fixed per-device scratch array (at most GBS_NOALLOC_MAX_ROWSUM_ROWS rows,
+ * else GBS_ERANGE) that every call on that device shares: calls that
+ * overlap on one GPU (other streams, other threads) race on it, so
+ * serialise them.  A matrix-core engine asked for explicitly needs the
+ * operands' value copies to exist already (gbs_workspace_reserve, or any
+ * earlier allocating call on them), else GBS_ERANGE; GBS_ALGO_AUTO picks
+ * among the engines that need nothing allocated (a matrix-core engine
+ * whose copies both operands hold, else the 1-bit planes engine or the
+ * popcount engine), all of which give the same int32 results. */
+int gbs_gemm_async_noalloc(const gbs_matrix *lhs, const gbs_matrix *rhs,
+                           int32_t *d_res, gbs_algo algo, void *stream);
+
 /* Grow ws for a product with these two operands, so that gbs_gemm_async
  * allocates nothing. */
 int gbs_workspace_reserve(gbs_workspace *ws, const gbs_matrix *lhs,
