@@ -93,7 +93,7 @@ def int8_bytes(M, K, N):
 def operand_bytes(algo, M, K, N, lb, rb):
     """HBM bytes one call reads and writes at least: the bit-planes for the
     popcount engine, the value copies for the matrix cores, plus the result."""
-    if algo in ("popcount", "mfma_bits"):
+    if algo in ("popcount", "mfma_bits", "mfma_planes"):
         return plane_bytes(M, K, N, lb, rb)
     return (M + N) * K * {"mfma": 1.0, "mfma_fp4": 0.5, "mfma_fp6": 0.75}[algo] + 4 * M * N
 
@@ -110,7 +110,7 @@ def exact_product(O, a, b, lb, rb, ls, rs):
 # arithmetic each engine computes in: popcount on 32-bit plane words, int8
 # products, or FP4 / FP6 codes of the integer values with fp32 accumulation
 ENGINE_DTYPE = {"popcount": "u32", "mfma": "i8", "mfma_fp4": "fp4", "mfma_fp6": "fp6",
-                "mfma_bits": "fp4"}
+                "mfma_bits": "fp4", "mfma_planes": "fp4"}
 
 
 def measured_traffic(engine):
@@ -129,10 +129,11 @@ def measured_traffic(engine):
 
 
 def roofline_of(algo, M, K, N, lb, rb, sec):
-    if algo in ("mfma", "mfma_fp4", "mfma_fp6", "mfma_bits"):
+    if algo in ("mfma", "mfma_fp4", "mfma_fp6", "mfma_bits", "mfma_planes"):
         ach = 2.0 * M * K * N / sec
         peak, unit = ((MFMA_I8_PEAK, "TOP/s (int8)") if algo == "mfma"
-                      else (MFMA_F4_PEAK, "TOP/s (fp4 operands)") if algo == "mfma_bits"
+                      else (MFMA_F4_PEAK, "TOP/s (fp4 operands)")
+                      if algo in ("mfma_bits", "mfma_planes")
                       else (MFMA_F4_PEAK, f"TOP/s ({algo[-3:]} operands)"))
         return {"bound": "mfma", "achieved": round(ach / 1e12, 2),
                 "peak": peak / 1e12, "unit": unit, "frac": round(ach / peak, 4)}
@@ -195,7 +196,8 @@ def run_config(gbs, O, name, runs, engines, steps, warmup, stream, ws, cpu_secs)
         want = None
         row = {"encoding": enc, "M": M, "K": K, "N": N, "bits": [lb, rb],
                "signed": [ls, rs], "auto": auto, "engines": {}}
-        for eng in (["popcount", "mfma", "mfma_fp4", "mfma_fp6", "mfma_bits"] if engines == "all"
+        for eng in (["popcount", "mfma", "mfma_fp4", "mfma_fp6", "mfma_bits", "mfma_planes"]
+                    if engines == "all"
                     else [auto]):
             log(f"[bench] {name} {enc}: {eng} x {steps}")
             try:

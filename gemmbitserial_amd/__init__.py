@@ -25,9 +25,9 @@ _NP_DT = {np.dtype(np.int8): I8, np.dtype(np.uint8): U8,
           np.dtype(np.int64): I64, np.dtype(np.uint64): U64,
           np.dtype(np.float32): F32, np.dtype(np.float64): F64}
 # gbs_algo
-AUTO, POPCOUNT, MFMA, MFMA_FP4, MFMA_FP6, MFMA_BITS = 0, 1, 2, 3, 4, 5
+AUTO, POPCOUNT, MFMA, MFMA_FP4, MFMA_FP6, MFMA_BITS, MFMA_PLANES = 0, 1, 2, 3, 4, 5, 6
 ALGOS = {"auto": AUTO, "popcount": POPCOUNT, "mfma": MFMA, "mfma_fp4": MFMA_FP4,
-         "mfma_fp6": MFMA_FP6, "mfma_bits": MFMA_BITS}
+         "mfma_fp6": MFMA_FP6, "mfma_bits": MFMA_BITS, "mfma_planes": MFMA_PLANES}
 
 EXPORTED = [
     "gbs_init", "gbs_last_error", "gbs_version", "gbs_host_alloc", "gbs_host_free",

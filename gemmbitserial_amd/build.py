@@ -24,7 +24,7 @@ OBJ = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libgbs_hip.so")
 BIN = os.path.join(PKG, "bin")
 SOURCES = ["gbs_runtime.hip", "gbs_pack.hip", "gbs_popc.hip", "gbs_mfma.hip",
-           "gbs_batch.hip", "gbs_serve.hip", "gbs_bits.hip"]
+           "gbs_batch.hip", "gbs_serve.hip", "gbs_bits.hip", "gbs_planes.hip"]
 CPP_PROGRAMS = {"test_dropin": "tests/cpp/test_dropin.cpp",
                 "geometry_probe": "tests/cpp/geometry_probe.cpp"}
 

@@ -100,6 +100,13 @@ bool bits_supported(const Operand &l, const Operand &r, uint64_t K);
 bool bits_preferred(const Operand &l, const Operand &r, uint64_t K);  // AUTO takes it
 int launch_bits_gemm(const Operand &l, const Operand &r, uint64_t K, int32_t *d_res,
                      hipStream_t s);
+// Matrix-core engine of multi-bit operands on their planes (gbs_planes.hip):
+// no value copies; one side of <= 4 planes in 2-bit chunks, the other
+// whole-valued FP4 codes (bipolar, <= 2 bits unsigned, <= 3 bits signed),
+// every partial sum below 2^24.
+bool planes_supported(const Operand &l, const Operand &r, uint64_t K);
+int launch_planes_gemm(const Operand &l, const Operand &r, uint64_t K, int32_t *d_res,
+                       hipStream_t s);
 // Matrix-core engine (gbs_mfma.hip).  ensure_vals / ensure_f4 / ensure_f6
 // make the int8 / FP4 / FP6 copy current (allocating it on first use when
 // may_allocate); ensure_copy picks by format (0 int8, 1 FP4, 2 FP6).

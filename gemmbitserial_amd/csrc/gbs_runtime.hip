@@ -142,6 +142,7 @@ static int gemm_enqueue_into(gbs_workspace *ws, const gbs_matrix *l, const gbs_m
   const uint64_t kwords = (K + 63) / 64;
   gbs_algo eng = algo == GBS_ALGO_AUTO ? select_algo(lo, ro, K) : algo;
   if (eng == GBS_ALGO_MFMA_BITS) return launch_bits_gemm(lo, ro, K, d_res, s);
+  if (eng == GBS_ALGO_MFMA_PLANES) return launch_planes_gemm(lo, ro, K, d_res, s);
   if (eng == GBS_ALGO_MFMA || eng == GBS_ALGO_MFMA_FP4 || eng == GBS_ALGO_MFMA_FP6) {
     const int fmt = eng == GBS_ALGO_MFMA_FP4 ? kFmtF4 : eng == GBS_ALGO_MFMA_FP6 ? kFmtF6 : kFmtI8;
     const bool ok = fmt == kFmtF4   ? f4_supported(lo, ro, K)

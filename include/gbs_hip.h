@@ -73,9 +73,15 @@ typedef enum gbs_algo {
                             bits, <= 3 signed), v_mfma_scale_f32_*_f8f6f4 */
   GBS_ALGO_MFMA_FP6 = 4, /* planes expanded to FP6 codes (operands of <= 4
                             bits, <= 5 signed), v_mfma_scale_f32_*_f8f6f4 */
-  GBS_ALGO_MFMA_BITS = 5 /* 1-bit x 1-bit only: the bit-planes themselves,
+  GBS_ALGO_MFMA_BITS = 5, /* 1-bit x 1-bit only: the bit-planes themselves,
                             each bit turned into an FP4 code in registers
                             (no value copies), v_mfma_scale_f32_*_f8f6f4 */
+  GBS_ALGO_MFMA_PLANES = 6 /* the bit-planes themselves (no value copies):
+                            one operand of <= 4 planes taken two planes at a
+                            time as FP4 codes of 2-bit chunks, the other
+                            (bipolar, <= 2 bits unsigned, <= 3 bits signed)
+                            as FP4 codes of its whole values;
+                            v_mfma_scale_f32_*_f8f6f4 */
 } gbs_algo;
 
 typedef struct gbs_matrix gbs_matrix;       /* device-resident bit-serial matrix */

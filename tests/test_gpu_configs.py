@@ -18,7 +18,7 @@ from oracle import orc as O
 
 pytestmark = pytest.mark.gpu
 
-ENGINES = ["auto", "popcount", "mfma", "mfma_fp4", "mfma_fp6", "mfma_bits"]
+ENGINES = ["auto", "popcount", "mfma", "mfma_fp4", "mfma_fp6", "mfma_bits", "mfma_planes"]
 CONFIGS = {"plain_gemm": (0, W.plain_gemm), "binary_layers": (1, W.binary_layers),
            "conv_gemm": (2, W.conv_gemm)}
 
