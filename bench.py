@@ -53,11 +53,25 @@ MFMA_F4_PEAK = 10.1e15            # dense FP4 / FP6 ops/s
 # runs it (the 1-bit planes kernel AUTO takes, the FP4-copy kernel), measured
 # with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE, scaled by a copy of known size)
 TRAFFIC_FILES = {"mfma_fp4": os.path.join(ROOT, "profiles", "mfma_fp4_binary_layers_traffic.txt"),
-                 "mfma_bits": os.path.join(ROOT, "profiles", "mfma_bits_binary_layers_traffic.txt")}
+                 "mfma_bits": os.path.join(ROOT, "profiles", "mfma_bits_binary_layers_traffic.txt"),
+                 "c3": os.path.join(ROOT, "profiles", "batched_uneven_512_traffic.txt")}
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo) and logical CPU count."""
+    name = "unknown CPU"
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                name = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return f"{name}, {os.cpu_count()} logical CPUs on the host"
 
 
 # ------------------------------------------------------------- workloads
@@ -348,46 +362,64 @@ def bench_batched(gbs, O, steps, warmup, stream, log, name="c3", cpu_secs=0.0):
         d0.fill_zero()
     out["speedup_graph_vs_per_call"] = round(out["per_call"]["ms"] / out["graph"]["ms"], 2)
     out["bit_exact"] = True
+    # roofline of the persistent launch: it reads every operand's bit-planes
+    # and writes every int32 result at least once
+    alg = sum(plane_bytes(p["M"], p["K"], p["N"], p["lb"], p["rb"]) for p in probs)
+    sec = out["persistent"]["ms"] * 1e-3
+    peak_of = {0: MFMA_I8_PEAK, 1: MFMA_F4_PEAK, 2: MFMA_F4_PEAK}
+    plans = [b.plan_of(i) for i in range(len(probs))]
+    mfma_floor = sum(2.0 * p["M"] * p["K"] * p["N"] / peak_of[pl[0]]
+                     for p, pl in zip(probs, plans) if pl[0] >= 0)
+    out["roofline"] = {"bound": "hbm", "achieved": round(alg / sec / 1e9, 1),
+                       "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                       "frac": round(alg / sec / HBM_PEAK, 4), "algorithmic_bytes": alg,
+                       "hbm_floor_us": round(alg / HBM_PEAK * 1e6, 1),
+                       "mfma_floor_us": round(mfma_floor * 1e6, 1),
+                       "traffic": measured_traffic(cname)}
+    tabled = sum(1 for p, pl in zip(probs, plans) if pl[0] >= 0 and
+                 gbs.batch_tune_get(pl[0], gbs.batch_tune_class_of(p["N"], p["K"]))[0] == pl[1])
+    out["tiles_from_table"] = f"{tabled} of {sum(1 for pl in plans if pl[0] >= 0)} products"
+    out["formats"] = {f: sum(1 for pl in plans if pl[0] == i)
+                      for i, f in enumerate(("int8", "fp4", "fp6"))}
     if cpu_secs > 0 and O.ref_available():
-        # the reference loop of benchmark.cpp:213-220: every product once per pass
-        ms = 0.0
-        for p in probs:
+        # the reference loop of benchmark.cpp:213-220 (every product once per
+        # pass), timed on every product of the list or, for the 512-product
+        # batch, on every 16th product and scaled by the ops
+        stride = 1 if len(probs) <= 64 else 16
+        sample = probs[::stride]
+        ms, sops = 0.0, 0.0
+        for p in sample:
             ns, reps, res = O.ref_bench_gemm(p["lhs"], p["rhs"], p["lb"], p["rb"], p["ls"],
-                                             p["rs"], min_secs=cpu_secs / len(probs),
+                                             p["rs"], min_secs=cpu_secs / len(sample),
                                              max_reps=200)
             want = O.wrap_i32(O.blas_gemm_exact(O.values_of(p["lhs"], p["lb"], p["ls"]),
                                                 O.values_of(p["rhs"], p["rb"], p["rs"])))
             if not np.array_equal(res, want):
                 raise SystemExit(f"{cname}: reference CPU result differs")
             ms += ns / 1e6
-        out["cpu_reference"] = {"ms_per_pass": round(ms, 3),
-                                "tops": round(ops / (ms * 1e-3) / 1e12, 4), "cores": 1}
-        log(f"[bench] {cname}: reference CPU {ms:.1f} ms per pass")
+            sops += 2.0 * p["M"] * p["K"] * p["N"]
+        tops = sops / (ms * 1e-3) / 1e12
+        out["cpu_reference"] = {"ms_per_pass": round(ops / (tops * 1e12) * 1e3, 3),
+                                "tops": round(tops, 4), "cores": 1, "cpu": cpu_model(),
+                                "sample": (f"all {len(probs)} products" if stride == 1 else
+                                           f"every {stride}th product ({len(sample)} of "
+                                           f"{len(probs)}, {sops / ops * 100:.1f} % of the ops), "
+                                           "pass time scaled by the ops")}
+        log(f"[bench] {cname}: reference CPU {out['cpu_reference']['ms_per_pass']:.1f} ms per pass")
     b.close()
     for l, r, d in items:
         l.close(), r.close(), d.close()
     return out
 
 
-def bench_serving(gbs, rate, count, log):
-    from gemmbitserial_amd import workloads as W
-    h, Ws, shifts = W.serving_mlp()
-    mats = [gbs.from_values(w, 4, True) for w in Ws]
-    srv = gbs.Server(mats, shifts, act_bits=4, max_requests=64, max_rows=64)
-    reqs = W.serving_requests(h[0], count, rate)
-    # warm-up requests (not measured)
-    for i in range(8):
-        srv.submit(10**6 + i, reqs[i]["x"])
-    pend = {10**6 + i for i in range(8)}
-    while pend:
-        srv.step()
-        for i in list(pend):
-            if srv.take(i) is not None:
-                pend.discard(i)
+def serve_run(gbs, srv, reqs, speedup, log):
+    """Feed requests at their Poisson arrival times divided by speedup (None:
+    all at once, the server's capacity) and step until every one finished.
+    Returns (latencies ms, results, wall s, ops, hbm bytes, steps)."""
+    count = len(reqs)
     st0 = srv.stats()
-    log(f"[bench] c4: serving {count} requests at {rate:.0f}/s, hidden sizes {h}")
     t0 = gbs.clock_ns() + 1_000_000
-    arrivals = [t0 + int(r["t"] * 1e9) for r in reqs]
+    arrivals = [t0 + (int(r["t"] / speedup * 1e9) if speedup else 0) for r in reqs]
     nxt, lat, res, pending = 0, [], {}, set()
     while len(res) < count:
         now = gbs.clock_ns()
@@ -398,31 +430,89 @@ def bench_serving(gbs, rate, count, log):
         if not pending:
             continue  # idle until the next arrival
         if srv.step():
-            for i in list(pending):
+            for i in srv.finished():
                 t = srv.take(i)
-                if t is not None:
-                    res[i] = t
-                    lat.append((t[2] - t[1]) / 1e6)
-                    pending.discard(i)
+                res[i] = t
+                lat.append((t[2] - t[1]) / 1e6)
+                pending.discard(i)
     t_end = max(t[2] for t in res.values())
     st = srv.stats()
-    ops = st["ops"] - st0["ops"]
-    wall = (t_end - arrivals[0]) / 1e9
-    # bit-exactness of a sample of requests against the exact forward pass
-    sample = list(range(0, count, max(1, count // 16)))
-    for i in sample:
-        if not np.array_equal(res[i][0], W.mlp_reference(reqs[i]["x"], Ws, shifts)):
-            raise SystemExit(f"c4: request {i} differs from the exact forward pass")
-    lat = np.array(lat)
+    return (np.array(lat), res, (t_end - arrivals[0]) / 1e9, st["ops"] - st0["ops"],
+            st["hbm_bytes"] - st0["hbm_bytes"], st["steps"] - st0["steps"])
+
+
+def bench_serving(gbs, O, rate, count, log, cpu_secs=0.0):
+    from gemmbitserial_amd import workloads as W
+    h, Ws, shifts = W.serving_mlp()
+    mats = [gbs.from_values(w, 4, True) for w in Ws]
+    srv = gbs.Server(mats, shifts, act_bits=4, max_requests=64, max_rows=64)
+    # warm-up requests (not measured)
+    warm = W.serving_requests(h[0], 8, rate, seed=W.SEED0 + 55)
+    for i, r in enumerate(warm):
+        srv.submit(10**6 + i, r["x"])
+    pend = {10**6 + i for i in range(len(warm))}
+    while pend:
+        srv.step()
+        for i in list(pend):
+            if srv.take(i) is not None:
+                pend.discard(i)
+    reqs = W.serving_requests(h[0], count, rate)
     out = {"workload": "8-layer MLP serving, continuous batching", "hidden": h,
            "shifts": shifts, "requests": count, "rate_per_s": rate,
-           "rows_total": int(sum(r["x"].shape[0] for r in reqs)),
-           "p50_ms": round(float(np.percentile(lat, 50)), 3),
-           "p99_ms": round(float(np.percentile(lat, 99)), 3),
-           "tops": round(ops / wall / 1e12, 3), "steps": st["steps"] - st0["steps"],
-           "wall_s": round(wall, 4), "row_capacity": st["row_capacity"],
-           "bit_exact_sample": len(sample)}
-    log(f"[bench] c4: p50 {out['p50_ms']} ms, p99 {out['p99_ms']} ms, {out['tops']} TOPS")
+           "rows_total": int(sum(r["x"].shape[0] for r in reqs))}
+    sweep = []
+    for r_ in (rate, 4 * rate, 16 * rate, None):
+        log(f"[bench] c4: serving {count} requests at "
+            f"{'once (capacity)' if r_ is None else f'{r_:.0f}/s'}, hidden sizes {h}")
+        lat, res, wall, ops, hbm, steps = serve_run(gbs, srv, reqs, r_ and r_ / rate, log)
+        sample = list(range(0, count, max(1, count // 16)))
+        for i in sample:
+            if not np.array_equal(res[i][0], W.mlp_reference(reqs[i]["x"], Ws, shifts)):
+                raise SystemExit(f"c4: request {i} differs from the exact forward pass")
+        row = {"rate_per_s": r_ if r_ else "all at once",
+               "p50_ms": round(float(np.percentile(lat, 50)), 3),
+               "p99_ms": round(float(np.percentile(lat, 99)), 3),
+               "tops": round(ops / wall / 1e12, 3), "steps": steps, "wall_s": round(wall, 4),
+               "hbm_gbs": round(hbm / wall / 1e9, 1), "bit_exact_sample": len(sample)}
+        sweep.append(row)
+        log(f"[bench] c4: p50 {row['p50_ms']} ms, p99 {row['p99_ms']} ms, {row['tops']} TOPS, "
+            f"{steps} steps")
+    base, cap = sweep[0], sweep[-1]
+    out.update({k: base[k] for k in ("p50_ms", "p99_ms", "tops", "steps", "wall_s")})
+    out["bit_exact_sample"] = base["bit_exact_sample"]
+    out["sweep"] = sweep
+    out["capacity_tops"] = cap["tops"]
+    st = srv.stats()
+    out["row_capacity"] = st["row_capacity"]
+    # roofline at capacity: the step launches' algorithmic bytes (weight
+    # planes, input and output activation planes, int32 rows out of the last
+    # layer) over the wall time of the all-at-once run
+    out["roofline"] = {"bound": "hbm", "achieved": cap["hbm_gbs"], "peak": HBM_PEAK / 1e9,
+                       "unit": "GB/s", "frac": round(cap["hbm_gbs"] * 1e9 / HBM_PEAK, 4),
+                       "at": "capacity (all requests at once), wall clock"}
+    if cpu_secs > 0 and O.ref_available():
+        # the reference forward pass of sample requests: per layer one
+        # gemmBitSerial (weights 4-bit signed x activations 4-bit unsigned),
+        # requantized between layers; timed per call like benchmark.cpp:93-111
+        ns_tot, ops_tot, picks = 0.0, 0.0, reqs[: 4]
+        for r in picks:
+            a = r["x"].astype(np.int8)
+            for l, w in enumerate(Ws):
+                ns, reps, acc = O.ref_bench_gemm(w, a, 4, 4, True, False,
+                                                 min_secs=cpu_secs / (len(picks) * len(Ws)),
+                                                 max_reps=50)
+                ns_tot += ns
+                ops_tot += 2.0 * w.shape[0] * w.shape[1] * a.shape[0]
+                if l + 1 < len(Ws):
+                    a = np.clip(acc.astype(np.int64) >> shifts[l], 0, 15).astype(np.int8)
+            if not np.array_equal(acc, W.mlp_reference(r["x"], Ws, shifts)):
+                raise SystemExit("c4: reference CPU forward pass differs")
+        out["cpu_reference"] = {"tops": round(ops_tot / ns_tot / 1e3, 4),
+                                "ms_per_request": round(ns_tot / len(picks) / 1e6, 3),
+                                "cores": 1, "cpu": cpu_model(),
+                                "sample": f"forward pass of the first {len(picks)} requests "
+                                          f"({sum(r['x'].shape[0] for r in picks)} rows)"}
+        log(f"[bench] c4: reference CPU {out['cpu_reference']['ms_per_request']} ms per request")
     srv.close()
     for m in mats:
         m.close()
@@ -470,10 +560,11 @@ def main():
     for i, name in enumerate(names):
         if name in ("c3", "c5"):
             results[name] = bench_batched(gbs, O, args.steps, args.warmup, stream, log, name,
-                                          cpu_secs if name == "c5" else 0.0)
+                                          cpu_secs)
             continue
         if name == "c4":
-            results[name] = bench_serving(gbs, args.serve_rate, args.serve_requests, log)
+            results[name] = bench_serving(gbs, O, args.serve_rate, args.serve_requests, log,
+                                          cpu_secs)
             continue
         desc, make = CONFIGS[name]
         runs = make(np.random.default_rng(SEED0 + int(name[1])))
@@ -499,7 +590,7 @@ def main():
                "kind": "reference",
                "sample": "the two full binary-layer GEMMs, " + ", ".join(
                    f"{r['encoding']}: {r['cpu_reference']['timed_calls']} timed calls after 1 "
-                   f"warm-up" for r in c1) + f"; host logical CPUs {os.cpu_count()}"}
+                   f"warm-up" for r in c1) + f"; {cpu_model()}"}
     line = {"metric": METRIC, "value": round(value, 2), "unit": "TOPS",
             "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "ms_per_step": round(step_ms, 5), "higher_is_better": True,

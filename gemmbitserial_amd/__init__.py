@@ -39,10 +39,13 @@ EXPORTED = [
     "gbs_stream", "gbs_stream_sync", "gbs_event_create", "gbs_event_destroy",
     "gbs_event_record", "gbs_event_elapsed_ms",
     "gbs_batch_create", "gbs_batch_destroy", "gbs_batch_run", "gbs_batch_run_graph",
-    "gbs_batch_run_per_call", "gbs_batch_info",
+    "gbs_batch_run_per_call", "gbs_batch_info", "gbs_batch_plan_of",
+    "gbs_batch_tune_classes", "gbs_batch_tune_class_of", "gbs_batch_tune_shapes",
+    "gbs_batch_tune_set", "gbs_batch_tune_get", "gbs_batch_tune_load",
     "gbs_tune_classes", "gbs_tune_class_of", "gbs_tune_set", "gbs_tune_load",
     "gbs_clock_ns", "gbs_server_create", "gbs_server_destroy", "gbs_server_submit",
-    "gbs_server_step", "gbs_server_take", "gbs_server_stats",
+    "gbs_server_step", "gbs_server_take", "gbs_server_stats", "gbs_server_layer_input",
+    "gbs_server_finished",
     "gbs_gemm_async_config", "gbs_gemm_async_noalloc",
     "gbs_conv2d_shape", "gbs_import_im2col", "gbs_matrix_save", "gbs_matrix_load",
 ]
@@ -86,6 +89,10 @@ def lib():
         "gbs_batch_create": [_p, _u64, _p], "gbs_batch_destroy": [_p],
         "gbs_batch_run": [_p, _p], "gbs_batch_run_graph": [_p, _p],
         "gbs_batch_run_per_call": [_p, _p], "gbs_batch_info": [_p, _p, _p, _p],
+        "gbs_batch_plan_of": [_p, _u64, _p, _p, _p, _p],
+        "gbs_batch_tune_classes": [_p], "gbs_batch_tune_class_of": [_u64, _u64, _p],
+        "gbs_batch_tune_shapes": [_i, _p, _i, _p], "gbs_batch_tune_set": [_i, _i, _i, _i],
+        "gbs_batch_tune_get": [_i, _i, _p, _p], "gbs_batch_tune_load": [C.c_char_p],
         "gbs_tune_classes": [_p, _p], "gbs_tune_class_of": [_u64, _u64, _u64, _p],
         "gbs_tune_set": [_p, _p, _i], "gbs_tune_load": [C.c_char_p],
         "gbs_gemm_async_config": [_p, _p, _p, _p, _i, _i, _i, _p],
@@ -94,7 +101,8 @@ def lib():
         "gbs_server_destroy": [_p], "gbs_server_submit": [_p, _u64, _p, _i, C.c_int64],
         "gbs_server_step": [_p, _p],
         "gbs_server_take": [_p, _u64, _p, _u64, _p, _p, _p, _p],
-        "gbs_server_stats": [_p, _p],
+        "gbs_server_stats": [_p, _p], "gbs_server_layer_input": [_p, _i, _p, _p],
+        "gbs_server_finished": [_p, _p, _i, _p],
         "gbs_conv2d_shape": [_p, _p, _p, _p, _p],
         "gbs_import_im2col": [_p, _p, _i, _p, _i, _p],
         "gbs_matrix_save": [_p, C.c_char_p], "gbs_matrix_load": [C.c_char_p, _p],
@@ -438,6 +446,15 @@ class Batch:
         check(lib().gbs_batch_info(self.h, C.byref(t), C.byref(a), C.byref(b)))
         return {"tiles": t.value, "persistent": a.value, "separate": b.value}
 
+    def plan_of(self, i):
+        """(format, shape, splits, K piece units) the batch planned for
+        problem i; format -1 for a popcount product."""
+        f, s = C.c_int(), C.c_int()
+        sp, pc = C.c_uint32(), C.c_uint32()
+        check(lib().gbs_batch_plan_of(self.h, i, C.byref(f), C.byref(s), C.byref(sp),
+                                      C.byref(pc)))
+        return f.value, s.value, sp.value, pc.value
+
     def close(self):
         if self.h:
             check(lib().gbs_batch_destroy(self.h))
@@ -448,6 +465,38 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def batch_tune_classes():
+    n = C.c_int()
+    check(lib().gbs_batch_tune_classes(C.byref(n)))
+    return n.value
+
+
+def batch_tune_class_of(N, K):
+    c = C.c_int()
+    check(lib().gbs_batch_tune_class_of(N, K, C.byref(c)))
+    return c.value
+
+
+def batch_tune_shapes(fmt):
+    buf, n = (C.c_int * 64)(), C.c_int()
+    check(lib().gbs_batch_tune_shapes(fmt, buf, 64, C.byref(n)))
+    return list(buf[:n.value])
+
+
+def batch_tune_set(fmt, cls, shape, piece_units=16):
+    check(lib().gbs_batch_tune_set(fmt, cls, shape, piece_units))
+
+
+def batch_tune_get(fmt, cls):
+    s, p = C.c_int(), C.c_int()
+    check(lib().gbs_batch_tune_get(fmt, cls, C.byref(s), C.byref(p)))
+    return s.value, p.value
+
+
+def batch_tune_load(path):
+    check(lib().gbs_batch_tune_load(path.encode()))
 
 
 def tune_classes():
@@ -484,7 +533,8 @@ class _ServerCounters(C.Structure):
     _fields_ = [("steps", C.c_uint64), ("done_requests", C.c_uint64),
                 ("done_rows", C.c_uint64), ("ops", C.c_double),
                 ("row_capacity", C.c_uint32), ("bytes_reserved", C.c_uint64),
-                ("waiting", C.c_uint32), ("in_flight", C.c_uint32)]
+                ("waiting", C.c_uint32), ("in_flight", C.c_uint32),
+                ("layer_steps", C.c_uint64), ("hbm_bytes", C.c_double)]
 
 
 def clock_ns():
@@ -509,6 +559,7 @@ class Server:
         h = C.c_void_p()
         check(L.gbs_server_create(wa, sh, len(self.weights), act_bits, C.byref(cfg), C.byref(h)))
         self.s = h
+        self.act_bits = act_bits
 
     def submit(self, rid, x, arrival_ns=0):
         x = np.ascontiguousarray(x, dtype=np.uint8)
@@ -529,6 +580,23 @@ class Server:
         if not ready.value:
             return None
         return out[: rows.value].copy(), arr.value, done.value
+
+    def finished(self, cap=4096):
+        """Ids of finished requests not yet taken."""
+        ids, n = (C.c_uint64 * cap)(), C.c_uint32()
+        check(lib().gbs_server_finished(self.s, ids, cap, C.byref(n)))
+        return list(ids[:n.value])
+
+    def layer_input(self, layer):
+        """(plane words [act_bits][capacity][words per row], rows) of the
+        bit-serial input the cohort at `layer` reads next step."""
+        m, rows = C.c_void_p(), C.c_uint32()
+        check(lib().gbs_server_layer_input(self.s, layer, C.byref(m), C.byref(rows)))
+        cap = self.stats()["row_capacity"]
+        wpr = (self.h[layer] + 63) // 64
+        out = np.zeros(self.act_bits * cap * wpr, np.uint64)
+        check(lib().gbs_planes_to_host(m, _ptr(out)))
+        return out.reshape(self.act_bits, cap, wpr), rows.value
 
     def stats(self):
         st = _ServerCounters()

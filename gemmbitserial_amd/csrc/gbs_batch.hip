@@ -41,6 +41,8 @@ struct gbs_batch {
   std::vector<gbs_problem> probs;          // as given
   std::vector<uint32_t> mfma_idx;          // persistent products, in run order
   std::vector<uint8_t> mfma_fmt;           // their copy formats
+  std::vector<gbs::MfmaPlan> mfma_plans;   // their plans
+  std::vector<int32_t> plan_of;            // per problem: index into mfma_plans, or -1
   std::vector<uint32_t> separate_idx;      // popcount products
   void *d_desc = nullptr;                  // BatchProb[], grouped by format
   void *d_zero = nullptr;                  // ZeroSpan[] of split products
@@ -125,18 +127,39 @@ __global__ __launch_bounds__(256, 2) void batch_kernel(const BatchProb *probs, u
 
 }  // namespace
 
-// Plan of one product inside a persistent launch: the tile shape with the
-// least padding for its rhs row count (the tuning table's shapes are
-// chosen to fill the chip with one product, which a shared queue does not
-// need), and deep products cut into pieces of at most 4 KiB (nominal) of K
-// so that no item is much longer than the rest -- or into up to
-// min_pieces pieces of at least 1 KiB, for products that would otherwise
-// dominate the queue.
+// Tile table of the persistent launch, separate from the standalone one
+// (whose shapes are chosen to fill the chip with one product, which a
+// shared queue does not need).  Per batch class (rhs row count bucket x
+// depth bucket) and copy format: a tile shape the persistent kernel
+// carries and the largest K piece of an item, in K units (kMfmaKS nominal
+// bytes).  Filled by gbs_batch_tune_load / _set (the tuner times whole
+// persistent launches of a mixed workload); shape -1 keeps the built-in
+// rule for that class.
+struct BatchTune {
+  int shape = -1, piece = 0;
+};
+BatchTune g_batch_tune[kMfmaFormats][kBatchClasses];
+
+int batch_class(uint64_t N, uint64_t K) {
+  const int cn = N <= 16 ? 0 : N <= 32 ? 1 : N <= 64 ? 2 : N <= 128 ? 3 : N <= 256 ? 4 : 5;
+  const int ck = K < 1024 ? 0 : K < 4096 ? 1 : 2;
+  return cn * kBatchClassK + ck;
+}
+
+// Plan of one product inside a persistent launch: the tuned tile shape and
+// K piece of its batch class, else the built-in rule -- the tile shape with
+// the least padding for its rhs row count and pieces of at most 4 KiB
+// (nominal) of K so that no item is much longer than the rest; products
+// that would otherwise dominate the queue are cut into up to min_pieces
+// pieces of at least 1 KiB.
 static MfmaPlan batch_plan(uint32_t M, uint32_t N, uint32_t kbytes, int fmt,
                            uint32_t min_pieces = 1) {
-  const int shape = batch_shape(N <= 16 ? 3 : N <= 32 ? 2 : 4, fmt);
+  const uint64_t K = (uint64_t)kbytes * (fmt == kFmtI8 ? 1 : 2);  // values
+  const BatchTune t = g_batch_tune[fmt][batch_class(N, K)];
+  const int shape = batch_shape(t.shape >= 0 ? t.shape : N <= 16 ? 3 : N <= 32 ? 2 : 4, fmt);
+  const uint32_t piece = t.shape >= 0 && t.piece > 0 ? (uint32_t)t.piece : 16;
   const uint32_t units = kbytes / kMfmaKS;
-  uint32_t want = (units + 15) / 16;
+  uint32_t want = (units + piece - 1) / piece;
   const uint32_t most = units / 4 ? units / 4 : 1;  // pieces of at least 1 KiB
   if (want < min_pieces) want = min_pieces < most ? min_pieces : most;
   return mfma_plan(M, N, kbytes, fmt, shape, (int)(want ? want : 1));
@@ -168,21 +191,26 @@ static void rebalance(BatchProb *d, int n, double total_bytes) {
 
 // The serving loop's per-step batch: at most kMaxViews products whose
 // descriptors travel in the kernel arguments (no copy ahead of the
-// launch); the caller zeroes the queue counter after the launch, with
-// work it launches next anyway; split products add their pieces
-// atomically into results the caller keeps zero between launches.
+// launch).  Whole-K tiles requantize their results into the next layer's
+// input, or store them as int32, in their epilogue (MfmaRequant); products
+// split along K add their pieces into zeroed accumulators, which
+// finish_split_kernel converts after the launch.  The launch zeroes the
+// queue counter of the next launch (the two alternate).
 constexpr int kMaxViews = 16;
 struct ViewBatch {
   BatchProb p[kMaxViews];
+  MfmaRequant q[kMaxViews];
   uint32_t n;
   uint64_t total;
 };
 
-__global__ __launch_bounds__(256, 2) void views_kernel(ViewBatch V, unsigned int *counters) {
+__global__ __launch_bounds__(256, 2) void views_kernel(ViewBatch V, unsigned int *counter,
+                                                       unsigned int *next) {
   __shared__ __attribute__((aligned(16))) int8_t lds[kMfmaLdsMax];
   __shared__ uint32_t s_item;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *next = 0;  // its launch follows this one
   for (;;) {
-    if (threadIdx.x == 0) s_item = atomicAdd(counters, 1u);
+    if (threadIdx.x == 0) s_item = atomicAdd(counter, 1u);
     __syncthreads();
     const uint64_t item = s_item;
     __syncthreads();
@@ -199,19 +227,62 @@ __global__ __launch_bounds__(256, 2) void views_kernel(ViewBatch V, unsigned int
     plan_decode(pl, item - V.p[lo].tile_begin, pt, qt, sp);
     const uint32_t kbeg = sp * pl.kchunk;
     const uint32_t kend = min(ops.kbytes, kbeg + pl.kchunk);
-    mfma_tile_by_shape<kFmtI8>(pl.shape, ops, pt, qt, kbeg, kend, pl.splits > 1, lds);
+    mfma_tile_by_shape<kFmtI8, true>(pl.shape, ops, pt, qt, kbeg, kend, pl.splits > 1, lds,
+                                     V.q[lo]);
     __syncthreads();
   }
 }
 
-// Plan and launch products given as int8 operand views.  *counter must be
-// zero at the launch; the caller zeroes it again after the launch (in
-// stream order).  The results of products the plan splits along K must be
-// zero before the launch.
-int launch_views(const MfmaOperands *views, int n, unsigned int *counter, hipStream_t s) {
+// Products of a step launch that were split along K, converted after it
+// (the launch boundary orders every piece's atomic add before these reads).
+struct FinishJob {
+  int32_t *acc;
+  uint32_t M, N, chunks;  // chunks: 64-column units of a row
+  MfmaRequant q;
+  uint64_t unit_begin;    // prefix of N * chunks over the jobs
+};
+struct FinishJobs {
+  FinishJob j[kMaxViews];
+  int n;
+};
+
+// One wave per (product, row, 64 columns), lane = column: the sum read and
+// zeroed for the next step, then requantized into codes and one plane word
+// per plane (a ballot), or stored as int32.
+__global__ __launch_bounds__(256) void finish_split_kernel(FinishJobs J, uint64_t units) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t u = (uint64_t)blockIdx.x * 4 + threadIdx.x / 64;
+  if (u >= units) return;
+  int k = 0;
+  while (k + 1 < J.n && J.j[k + 1].unit_begin <= u) k++;
+  const FinishJob &f = J.j[k];
+  const uint64_t local = u - f.unit_begin;
+  const uint64_t n = local / f.chunks, mc = (local - n * f.chunks) * 64, m = mc + lane;
+  int32_t v = 0;
+  if (m < f.M) {
+    v = f.acc[n * f.M + m];
+    f.acc[n * f.M + m] = 0;
+  }
+  if (f.q.vals) {
+    const uint32_t code = m < f.M ? requant_code(v, f.q) : 0u;
+    if (m < f.M) f.q.vals[n * f.q.ld + m] = (int8_t)code;
+    for (uint32_t b = 0; b < f.q.nbits; b++) {
+      const uint64_t w = __ballot((code >> b) & 1u);
+      if (lane == 0) f.q.planes[b * f.q.wpp + n * f.q.wpr + mc / 64] = w;
+    }
+  } else if (m < f.M) {
+    f.q.out[n * f.M + m] = v;
+  }
+}
+
+// Plan and launch products given as int8 operand views with their
+// requantization or output (see launch_views in gbs_mfma_tile.h).
+int launch_views(const MfmaOperands *views, const MfmaRequant *rq, int n, unsigned int *counter,
+                 unsigned int *next, hipStream_t s) {
   if (n > kMaxViews) return fail(GBS_EINVAL, "launch_views: %d products (at most %d)", n, kMaxViews);
   ViewBatch V{};
   BatchProb *h = V.p;
+  MfmaRequant hq[kMaxViews];
   int m = 0;
   double total = 0;
   for (int i = 0; i < n; i++) {
@@ -220,7 +291,9 @@ int launch_views(const MfmaOperands *views, int n, unsigned int *counter, hipStr
     BatchProb d{};
     d.ops = v;
     d.plan = batch_plan(v.M, v.N, v.kbytes, kFmtI8);
+    d.tile_begin = (uint64_t)i;  // the product's index until sorted
     total += item_bytes(d.plan) * (double)d.plan.tiles();
+    hq[i] = rq[i];
     h[m++] = d;
   }
   // a step of few GEMV-like products would otherwise run on a few CUs:
@@ -230,7 +303,25 @@ int launch_views(const MfmaOperands *views, int n, unsigned int *counter, hipStr
     return item_bytes(x.plan) > item_bytes(y.plan);
   });
   uint64_t tiles = 0;
+  FinishJobs F{};
+  uint64_t units = 0;
   for (int i = 0; i < m; i++) {
+    const MfmaRequant q = hq[h[i].tile_begin];
+    if (h[i].plan.splits > 1) {
+      FinishJob &f = F.j[F.n++];
+      f.acc = h[i].ops.res;
+      f.M = h[i].ops.M;
+      f.N = h[i].ops.N;
+      f.chunks = (f.M + 63) / 64;
+      f.q = q;
+      f.unit_begin = units;
+      units += (uint64_t)f.N * f.chunks;
+    } else if (!q.vals) {
+      // a whole-K product without requantization stores its int32 rows
+      // straight into the destination
+      h[i].ops.res = q.out;
+    }
+    V.q[i] = q;
     h[i].tile_begin = tiles;
     tiles += h[i].plan.tiles();
   }
@@ -238,7 +329,12 @@ int launch_views(const MfmaOperands *views, int n, unsigned int *counter, hipStr
   V.n = (uint32_t)m;
   V.total = tiles;
   const uint32_t grid = (uint32_t)(tiles < 512 ? tiles : 512);
-  hipLaunchKernelGGL(views_kernel, dim3(grid), dim3(256), 0, s, V, counter);
+  hipLaunchKernelGGL(views_kernel, dim3(grid), dim3(256), 0, s, V, counter, next);
+  GBS_HIP(hipGetLastError());
+  if (units) {
+    hipLaunchKernelGGL(finish_split_kernel, dim3((uint32_t)((units + 3) / 4)), dim3(256), 0, s,
+                       F, units);
+  }
   GBS_HIP(hipGetLastError());
   return GBS_OK;
 }
@@ -391,7 +487,10 @@ int gbs_batch_create(const gbs_problem *problems, uint64_t count, gbs_batch **ou
     desc.push_back(d);
     b->mfma_idx.push_back(it.idx);
     b->mfma_fmt.push_back((uint8_t)f);
+    b->mfma_plans.push_back(d.plan);
   }
+  b->plan_of.assign(count, -1);
+  for (size_t k = 0; k < b->mfma_idx.size(); k++) b->plan_of[b->mfma_idx[k]] = (int32_t)k;
   for (int f = 0; f < 3; f++) {
     b->item0[f] = b->total_tiles;
     b->total_tiles += b->tiles[f];
@@ -509,6 +608,99 @@ int gbs_batch_info(const gbs_batch *b, uint64_t *tiles, uint64_t *persistent,
   if (tiles) *tiles = b->total_tiles;
   if (persistent) *persistent = b->mfma_idx.size();
   if (separate) *separate = b->separate_idx.size();
+  return GBS_OK;
+}
+
+int gbs_batch_plan_of(const gbs_batch *b, uint64_t problem, int *fmt, int *shape,
+                      uint32_t *splits, uint32_t *piece_units) {
+  if (!b) return fail(GBS_EINVAL, "gbs_batch_plan_of: null batch");
+  if (problem >= b->probs.size())
+    return fail(GBS_ERANGE, "gbs_batch_plan_of: problem %llu of %zu",
+                (unsigned long long)problem, b->probs.size());
+  const int k = b->plan_of.empty() ? -1 : b->plan_of[problem];
+  const MfmaPlan *pl = k >= 0 ? &b->mfma_plans[k] : nullptr;
+  if (fmt) *fmt = pl ? pl->fmt : -1;
+  if (shape) *shape = pl ? pl->shape : -1;
+  if (splits) *splits = pl ? pl->splits : 0;
+  if (piece_units) *piece_units = pl ? pl->kchunk / kMfmaKS : 0;
+  return GBS_OK;
+}
+
+int gbs_batch_tune_classes(int *n_classes) {
+  if (!n_classes) return fail(GBS_EINVAL, "gbs_batch_tune_classes: null out");
+  *n_classes = kBatchClasses;
+  return GBS_OK;
+}
+
+int gbs_batch_tune_class_of(uint64_t N, uint64_t K, int *cls) {
+  if (!cls) return fail(GBS_EINVAL, "gbs_batch_tune_class_of: null out");
+  *cls = batch_class(N, K);
+  return GBS_OK;
+}
+
+int gbs_batch_tune_shapes(int fmt, int *shapes, int cap, int *n) {
+  if (!n || (cap > 0 && !shapes)) return fail(GBS_EINVAL, "gbs_batch_tune_shapes: null out");
+  if (fmt < 0 || fmt >= kMfmaFormats)
+    return fail(GBS_EINVAL, "gbs_batch_tune_shapes: format %d", fmt);
+  int k = 0;
+  for (int s = 0; s < kMfmaShapes; s++)
+    if (batch_shape(s, fmt) == s) {
+      if (k < cap) shapes[k] = s;
+      k++;
+    }
+  *n = k;
+  return GBS_OK;
+}
+
+int gbs_batch_tune_set(int fmt, int cls, int shape, int piece_units) {
+  if (fmt < 0 || fmt >= kMfmaFormats || cls < 0 || cls >= kBatchClasses)
+    return fail(GBS_EINVAL, "gbs_batch_tune_set: format %d class %d", fmt, cls);
+  if (shape >= 0 && (shape >= kMfmaShapes || batch_shape(shape, fmt) != shape))
+    return fail(GBS_EINVAL,
+                "gbs_batch_tune_set: shape %d is not carried by the persistent kernel "
+                "for format %d",
+                shape, fmt);
+  if (shape >= 0 && (piece_units < 1 || piece_units > 4096))
+    return fail(GBS_EINVAL, "gbs_batch_tune_set: K piece of %d units outside [1, 4096]",
+                piece_units);
+  g_batch_tune[fmt][cls].shape = shape < 0 ? -1 : shape;
+  g_batch_tune[fmt][cls].piece = shape < 0 ? 0 : piece_units;
+  return GBS_OK;
+}
+
+int gbs_batch_tune_get(int fmt, int cls, int *shape, int *piece_units) {
+  if (fmt < 0 || fmt >= kMfmaFormats || cls < 0 || cls >= kBatchClasses)
+    return fail(GBS_EINVAL, "gbs_batch_tune_get: format %d class %d", fmt, cls);
+  if (shape) *shape = g_batch_tune[fmt][cls].shape;
+  if (piece_units) *piece_units = g_batch_tune[fmt][cls].piece;
+  return GBS_OK;
+}
+
+int gbs_batch_tune_load(const char *path) {
+  if (!path) return fail(GBS_EINVAL, "gbs_batch_tune_load: null path");
+  FILE *f = fopen(path, "r");
+  if (!f) return fail(GBS_EINVAL, "gbs_batch_tune_load: cannot open %s", path);
+  BatchTune t[kMfmaFormats][kBatchClasses];
+  char line[512];
+  int n = 0, bad = 0;
+  while (fgets(line, sizeof(line), f)) {
+    int c, fmt, sh, pc;
+    if (line[0] == '#') continue;
+    if (sscanf(line, "%d %d %d %d", &c, &fmt, &sh, &pc) != 4) continue;
+    if (c < 0 || c >= kBatchClasses || fmt < 0 || fmt >= kMfmaFormats || sh < 0 ||
+        sh >= kMfmaShapes || batch_shape(sh, fmt) != sh || pc < 1 || pc > 4096) {
+      bad++;
+      continue;
+    }
+    t[fmt][c].shape = sh;
+    t[fmt][c].piece = pc;
+    n++;
+  }
+  fclose(f);
+  if (bad) return fail(GBS_EINVAL, "gbs_batch_tune_load: %d invalid entries in %s", bad, path);
+  if (n == 0) return fail(GBS_EINVAL, "gbs_batch_tune_load: no entries in %s", path);
+  for (int fmt = 0; fmt < kMfmaFormats; fmt++)
+    for (int c = 0; c < kBatchClasses; c++) g_batch_tune[fmt][c] = t[fmt][c];
   return GBS_OK;
 }
 

@@ -87,8 +87,6 @@ __device__ __forceinline__ MfmaAcc<FMT> mfma_op(const MfmaFrag<FMT> &a, const Mf
                                                             kF6Scale);
   }
 }
-__device__ __forceinline__ int32_t acc_value(int32_t x) { return x; }
-__device__ __forceinline__ int32_t acc_value(float x) { return (int32_t)x; }  // exact integer
 
 constexpr int kMfmaKS = 256;  // padding unit of K (nominal bytes): every stage size divides it
 
@@ -101,6 +99,80 @@ struct MfmaOperands {
   uint32_t M, N;         // lhs rows, rhs rows
   uint32_t kbytes;       // nominal bytes of K in the copies, a multiple of kMfmaKS (zero padded)
 };
+
+// Fused requantization of a product's results into the input of the next
+// layer (the serving loop): code = min(maxv, max(0, v >> shift)) written as
+// int8 into vals (row stride ld) and as bit-planes into planes
+// ([nbits][rows][wpr] words, wpp words per plane), so the int32 sums never
+// reach memory; with vals null the int32 results go to out (N x M,
+// row-major) instead.  Whole-K tiles convert in their epilogue; a product
+// split along K adds its pieces into MfmaOperands::res and is converted
+// after the launch (launch_views).
+struct MfmaRequant {
+  int8_t *vals;
+  uint64_t *planes;
+  int32_t *out;
+  uint64_t ld, wpr, wpp;
+  uint32_t shift, nbits, maxv;
+};
+
+__device__ __forceinline__ uint32_t requant_code(int32_t v, const MfmaRequant &q) {
+  const int32_t a = v >> q.shift;  // arithmetic shift
+  return a < 0 ? 0u : (a > (int32_t)q.maxv ? q.maxv : (uint32_t)a);
+}
+
+__device__ __forceinline__ int32_t acc_value(int32_t x) { return x; }
+__device__ __forceinline__ int32_t acc_value(float x) { return (int32_t)x; }  // exact integer
+
+// Results of one wave's FP x FQ fragments (lane holds rhs rows
+// nb + 16 i + 4 (lane / 16) + r, r < 4, of lhs column mb + 16 j + lane % 16):
+// int32 stores into res (atomic adds for a K piece), or with RQ and a
+// whole-K tile the requantized codes -- int8 bytes, and per plane one
+// 16-bit piece of a plane word per 16 columns, gathered with one ballot.
+template <int FP, int FQ, bool RQ, class AccT>
+__device__ __forceinline__ void store_fragments(const MfmaOperands &a, const AccT (&acc)[FP][FQ],
+                                                uint32_t nb, uint32_t mb, bool atomic,
+                                                const MfmaRequant &q) {
+  const int lane = threadIdx.x & 63, fr = lane & 15, g = lane >> 4;
+  if constexpr (RQ) {
+    if (!atomic && q.vals) {
+#pragma unroll
+      for (int i = 0; i < FP; i++)
+#pragma unroll
+        for (int j = 0; j < FQ; j++)
+#pragma unroll
+          for (int r = 0; r < 4; r++) {
+            const uint32_t n = nb + i * 16 + 4 * g + r;
+            const uint32_t m0 = mb + j * 16, m = m0 + fr;
+            const bool ok = n < a.N && m < a.M;
+            const uint32_t code = ok ? requant_code(acc_value(acc[i][j][r]), q) : 0u;
+            if (ok) q.vals[(uint64_t)n * q.ld + m] = (int8_t)code;
+            for (uint32_t b = 0; b < q.nbits; b++) {
+              const uint64_t w = __ballot((code >> b) & 1u);
+              if (fr == 0 && n < a.N && m0 < a.M)
+                reinterpret_cast<uint16_t *>(q.planes + b * q.wpp + (uint64_t)n * q.wpr)[m0 / 16] =
+                    (uint16_t)(w >> (16 * g));
+            }
+          }
+      return;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < FP; i++)
+#pragma unroll
+    for (int j = 0; j < FQ; j++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const uint32_t n = nb + i * 16 + 4 * g + r;
+        const uint32_t m = mb + j * 16 + fr;
+        if (n < a.N && m < a.M) {
+          const uint64_t idx = (uint64_t)n * a.M + m;
+          const int32_t v = acc_value(acc[i][j][r]);
+          if (atomic) atomicAdd(a.res + idx, v);
+          else a.res[idx] = v;
+        }
+      }
+}
 
 // Tile shapes the engine is compiled for: P x Q outputs, WP waves along P,
 // KSB bytes of K per LDS stage, KG groups of 4 waves per block (each group
@@ -152,10 +224,11 @@ constexpr size_t kLdsPerCU = 160 * 1024;
 // persistent kernel: the largest carried shape (64 x 64, three FP6 stages)
 constexpr size_t kMfmaLdsMax = (size_t)3 * 128 * 192;
 
-template <int P, int Q, int WP, int KSB, int DEPTH, int FMT = kFmtI8, bool TAILS_APART = true>
+template <int P, int Q, int WP, int KSB, int DEPTH, int FMT = kFmtI8, bool TAILS_APART = true,
+          bool RQ = false>
 __device__ __forceinline__ void mfma_tile(const MfmaOperands &a, uint32_t pt, uint32_t qt,
                                           uint32_t kbeg, uint32_t kend, bool atomic,
-                                          int8_t *lds) {
+                                          int8_t *lds, const MfmaRequant &q = MfmaRequant{}) {
   constexpr int WQ = 4 / WP;
   constexpr int PW = P / WP, QW = Q / WQ;       // wave tile
   constexpr int FP = PW / 16, FQ = QW / 16;     // fragments per wave
@@ -284,23 +357,7 @@ __device__ __forceinline__ void mfma_tile(const MfmaOperands &a, uint32_t pt, ui
     }
   }
 
-  // fragment output: lane holds rows 4*(lane/16)+r (rhs, n) of column
-  // lane%16 (lhs, m)
-#pragma unroll
-  for (int i = 0; i < FP; i++)
-#pragma unroll
-    for (int j = 0; j < FQ; j++)
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const uint32_t n = p0 + wp * PW + i * 16 + 4 * (lane >> 4) + r;
-        const uint32_t m = q0 + wq * QW + j * 16 + fr;
-        if (n < a.N && m < a.M) {
-          const uint64_t idx = (uint64_t)n * a.M + m;
-          const int32_t v = acc_value(acc[i][j][r]);
-          if (atomic) atomicAdd(a.res + idx, v);
-          else a.res[idx] = v;
-        }
-      }
+  store_fragments<FP, FQ, RQ>(a, acc, p0 + wp * PW, q0 + wq * QW, atomic, q);
 }
 
 // Block-wide barrier that waits for this wave's LDS operations only (not
@@ -352,10 +409,10 @@ __device__ __forceinline__ int dl_chunk(int row, int slot) {  // logical chunk o
   else return lds_chunk<KSB>(row, slot);
 }
 template <int P, int Q, int WP, int KSB, int NBUF, int FMT = kFmtI8, int KG = 1,
-          bool TAILS_APART = true>
+          bool TAILS_APART = true, bool RQ = false>
 __device__ __forceinline__ void mfma_tile_dl(const MfmaOperands &a, uint32_t pt, uint32_t qt,
                                              uint32_t kbeg, uint32_t kend, bool atomic,
-                                             int8_t *lds) {
+                                             int8_t *lds, const MfmaRequant &q = MfmaRequant{}) {
   constexpr int NW = 4 * KG;                       // waves per block
   constexpr int WQ = 4 / WP;
   constexpr int PW = P / WP, QW = Q / WQ;
@@ -514,21 +571,7 @@ __device__ __forceinline__ void mfma_tile_dl(const MfmaOperands &a, uint32_t pt,
             acc[i][j][r] += red[g * PART + ((i * FQ + j) * 4 + r) * 256 + at];
   }
 
-#pragma unroll
-  for (int i = 0; i < FP; i++)
-#pragma unroll
-    for (int j = 0; j < FQ; j++)
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const uint32_t n = p0 + wp * PW + i * 16 + 4 * (lane >> 4) + r;
-        const uint32_t m = q0 + wq * QW + j * 16 + fr;
-        if (n < a.N && m < a.M) {
-          const uint64_t idx = (uint64_t)n * a.M + m;
-          const int32_t v = acc_value(acc[i][j][r]);
-          if (atomic) atomicAdd(a.res + idx, v);
-          else a.res[idx] = v;
-        }
-      }
+  store_fragments<FP, FQ, RQ>(a, acc, p0 + wp * PW, q0 + wq * QW, atomic, q);
 }
 
 // Software-pipelined tile: K in 64-byte stages.  In the iteration that
@@ -537,10 +580,10 @@ __device__ __forceinline__ void mfma_tile_dl(const MfmaOperands &a, uint32_t pt,
 // loads of stage i+2, passes one LDS-only barrier, and reads stage i+1's
 // fragments -- those LDS reads and the global loads overlap the MFMAs of
 // stage i.
-template <int P, int Q, int WP, int FMT = kFmtI8>
+template <int P, int Q, int WP, int FMT = kFmtI8, bool RQ = false>
 __device__ __forceinline__ void mfma_tile_pp(const MfmaOperands &a, uint32_t pt, uint32_t qt,
                                              uint32_t kbeg, uint32_t kend, bool atomic,
-                                             int8_t *lds) {
+                                             int8_t *lds, const MfmaRequant &q = MfmaRequant{}) {
   static_assert(FMT != kFmtF6, "the pipelined tile takes 16-byte fragments only");
   constexpr int KSB = 64, LDR = KSB + 16;
   constexpr int WQ = 4 / WP;
@@ -632,21 +675,7 @@ __device__ __forceinline__ void mfma_tile_pp(const MfmaOperands &a, uint32_t pt,
   // the LDS is reused by the next tile of a persistent block
   lds_barrier();
 
-#pragma unroll
-  for (int i = 0; i < FP; i++)
-#pragma unroll
-    for (int j = 0; j < FQ; j++)
-#pragma unroll
-      for (int r = 0; r < 4; r++) {
-        const uint32_t n = p0 + wp * PW + i * 16 + 4 * (lane >> 4) + r;
-        const uint32_t m = q0 + wq * QW + j * 16 + fr;
-        if (n < a.N && m < a.M) {
-          const uint64_t idx = (uint64_t)n * a.M + m;
-          const int32_t v = acc_value(acc[i][j][r]);
-          if (atomic) atomicAdd(a.res + idx, v);
-          else a.res[idx] = v;
-        }
-      }
+  store_fragments<FP, FQ, RQ>(a, acc, p0 + wp * PW, q0 + wq * QW, atomic, q);
 }
 
 // Shape run for shape s in format fmt: FP6 has no pipelined tile (it takes
@@ -701,28 +730,32 @@ constexpr bool batch_shapes_fit_lds() {
 }
 static_assert(batch_shapes_fit_lds(), "a shape of the persistent kernel overflows its LDS");
 
-// Dispatch one tile by shape id (the persistent kernel's per-product switch).
-template <int FMT>
+// Dispatch one tile by shape id (the persistent kernel's per-product
+// switch); RQ: the serving loop's launch, whose whole-K tiles may be
+// requantized in the epilogue (MfmaRequant).
+template <int FMT, bool RQ = false>
 __device__ __forceinline__ void mfma_tile_by_shape(int shape, const MfmaOperands &a,
                                                    uint32_t pt, uint32_t qt, uint32_t kbeg,
-                                                   uint32_t kend, bool atomic, int8_t *lds) {
+                                                   uint32_t kend, bool atomic, int8_t *lds,
+                                                   const MfmaRequant &q = MfmaRequant{}) {
   if constexpr (FMT == kFmtF6) {
-    if (shape == 2) mfma_tile<32, 256, 1, 64, 2, FMT, false>(a, pt, qt, kbeg, kend, atomic, lds);
-    else mfma_tile_dl<64, 64, 2, 128, 3, FMT, 1, false>(a, pt, qt, kbeg, kend, atomic, lds);
+    if (shape == 2)
+      mfma_tile<32, 256, 1, 64, 2, FMT, false, RQ>(a, pt, qt, kbeg, kend, atomic, lds, q);
+    else mfma_tile_dl<64, 64, 2, 128, 3, FMT, 1, false, RQ>(a, pt, qt, kbeg, kend, atomic, lds, q);
   } else {
     switch (shape) {
       case 0:
         if constexpr (FMT == kFmtI8)
-          mfma_tile<128, 128, 2, 128, 2, FMT>(a, pt, qt, kbeg, kend, atomic, lds);
+          mfma_tile<128, 128, 2, 128, 2, FMT, true, RQ>(a, pt, qt, kbeg, kend, atomic, lds, q);
         break;
-      case 1: mfma_tile<64, 128, 2, 128, 2, FMT>(a, pt, qt, kbeg, kend, atomic, lds); break;
-      case 2: mfma_tile<32, 256, 1, 64, 2, FMT>(a, pt, qt, kbeg, kend, atomic, lds); break;
-      case 3: mfma_tile<16, 256, 1, 128, 2, FMT>(a, pt, qt, kbeg, kend, atomic, lds); break;
-      case 13: mfma_tile_dl<64, 64, 2, 128, 4, FMT>(a, pt, qt, kbeg, kend, atomic, lds); break;
-      case 5: mfma_tile<128, 64, 4, 128, 2, FMT>(a, pt, qt, kbeg, kend, atomic, lds); break;
-      case 6: mfma_tile<128, 128, 2, 64, 1, FMT>(a, pt, qt, kbeg, kend, atomic, lds); break;
-      case 8: mfma_tile_pp<128, 128, 2, FMT>(a, pt, qt, kbeg, kend, atomic, lds); break;
-      default: mfma_tile_pp<64, 128, 2, FMT>(a, pt, qt, kbeg, kend, atomic, lds); break;
+      case 1: mfma_tile<64, 128, 2, 128, 2, FMT, true, RQ>(a, pt, qt, kbeg, kend, atomic, lds, q); break;
+      case 2: mfma_tile<32, 256, 1, 64, 2, FMT, true, RQ>(a, pt, qt, kbeg, kend, atomic, lds, q); break;
+      case 3: mfma_tile<16, 256, 1, 128, 2, FMT, true, RQ>(a, pt, qt, kbeg, kend, atomic, lds, q); break;
+      case 13: mfma_tile_dl<64, 64, 2, 128, 4, FMT, 1, true, RQ>(a, pt, qt, kbeg, kend, atomic, lds, q); break;
+      case 5: mfma_tile<128, 64, 4, 128, 2, FMT, true, RQ>(a, pt, qt, kbeg, kend, atomic, lds, q); break;
+      case 6: mfma_tile<128, 128, 2, 64, 1, FMT, true, RQ>(a, pt, qt, kbeg, kend, atomic, lds, q); break;
+      case 8: mfma_tile_pp<128, 128, 2, FMT, RQ>(a, pt, qt, kbeg, kend, atomic, lds, q); break;
+      default: mfma_tile_pp<64, 128, 2, FMT, RQ>(a, pt, qt, kbeg, kend, atomic, lds, q); break;
     }
   }
 }
@@ -769,14 +802,24 @@ inline int shape_class(uint64_t M, uint64_t N, uint64_t K) {
   return (cm * kClassN + cn) * kClassK + ck;
 }
 
+// Batch class of a product inside a persistent launch, for the batch tile
+// table: rhs rows <= 16 / <= 32 / <= 64 / <= 128 / <= 256 / more, times
+// depth (values) < 1024 / < 4096 / more.
+constexpr int kBatchClassN = 6, kBatchClassK = 3, kBatchClasses = kBatchClassN * kBatchClassK;
+int batch_class(uint64_t N, uint64_t K);
+
 // Tile shape and K split chosen for a product of depth K (values) in
 // format fmt (table set by gbs_tune_load; else a size heuristic).
 // splits 0 = decide from the tile count.
 void mfma_config_for(uint64_t M, uint64_t N, uint64_t K, int fmt, int *shape, int *splits);
 
 // Persistent launch over at most 16 products given as int8 operand views
-// (gbs_batch.hip): *counter is zero at the launch and zeroed again by the
-// caller after it; results of products split along K must be zero before.
-int launch_views(const MfmaOperands *views, int n, unsigned int *counter, hipStream_t s);
+// (gbs_batch.hip), each with its requantization (rq[i].vals set) or int32
+// destination (rq[i].out): views[i].res are int32 accumulators that
+// products split along K add into, zero before the call and left zero (a
+// second, small launch converts those products).  *counter is zero at the
+// call; the launch zeroes *next, the counter of the following call.
+int launch_views(const MfmaOperands *views, const MfmaRequant *rq, int n, unsigned int *counter,
+                 unsigned int *next, hipStream_t s);
 
 }  // namespace gbs

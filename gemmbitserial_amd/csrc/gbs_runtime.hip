@@ -52,28 +52,36 @@ int hip_fail(hipError_t e, const char *what) {
               hipGetErrorString(e));
 }
 
-// The tuned tile table: $GBS_TUNE_FILE, else tune_table.txt beside the
-// library; the built-in heuristic when neither exists.
+// A file beside this library, or the path $env names.
+static std::string table_path(const char *env, const char *name) {
+  if (const char *e = getenv(env)) return e;
+  Dl_info info;
+  if (!dladdr((void *)&table_path, &info) || !info.dli_fname) return std::string();
+  std::string path = info.dli_fname;
+  const size_t slash = path.rfind('/');
+  return (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + "/" + name;
+}
+
+// The tuned tile tables: the standalone one ($GBS_TUNE_FILE, else
+// tune_table.txt beside the library) and the persistent launch's
+// ($GBS_BATCH_TUNE_FILE, else batch_tune_table.txt); the built-in rules
+// where a table does not exist.
 static void load_default_table() {
   static std::once_flag once;
   std::call_once(once, [] {
-    std::string path;
-    if (const char *env = getenv("GBS_TUNE_FILE")) {
-      path = env;
-    } else {
-      Dl_info info;
-      if (dladdr((void *)&load_default_table, &info) && info.dli_fname) {
-        path = info.dli_fname;
-        const size_t slash = path.rfind('/');
-        path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) +
-               "/tune_table.txt";
-      }
-    }
-    FILE *f = path.empty() ? nullptr : fopen(path.c_str(), "r");
-    if (!f) return;
-    fclose(f);
     const std::string saved = t_err;
-    if (gbs_tune_load(path.c_str()) != GBS_OK) t_err = saved;
+    const std::string path = table_path("GBS_TUNE_FILE", "tune_table.txt");
+    FILE *f = path.empty() ? nullptr : fopen(path.c_str(), "r");
+    if (f) {
+      fclose(f);
+      if (gbs_tune_load(path.c_str()) != GBS_OK) t_err = saved;
+    }
+    const std::string bpath = table_path("GBS_BATCH_TUNE_FILE", "batch_tune_table.txt");
+    f = bpath.empty() ? nullptr : fopen(bpath.c_str(), "r");
+    if (f) {
+      fclose(f);
+      if (gbs_batch_tune_load(bpath.c_str()) != GBS_OK) t_err = saved;
+    }
   });
 }
 

@@ -2,39 +2,29 @@
 // bit-serial layers with weights resident in HBM.
 //
 // A server holds L weight matrices W_l (h_{l+1} x h_l, bit-planes plus their
-// int8 copy) and, per layer, an activation pool: bit-planes and int8 values
-// of up to R rows of width h_l.  Requests (1..max_rows rows of act_bits-bit
-// unsigned activations) wait in a FIFO.  Every step:
+// int8 copy) and, per layer, two activation pools (step parity): bit-planes
+// and int8 values of up to R rows of width h_l.  Requests (1..max_rows rows
+// of act_bits-bit unsigned activations) wait in a FIFO.  Every step t:
 //
 //   1. admission: waiting requests join a new cohort while its rows fit the
 //      pool capacity R (set from the memory budget at creation) and the
 //      number of requests in flight stays under max_requests; the cohort's
-//      inputs are packed on the GPU into pool 0, read from pinned host
-//      memory by the packing kernel itself;
+//      inputs are packed on the GPU into pool 0 (parity t), read from pinned
+//      host memory by the packing kernel itself;
 //   2. one persistent launch computes, for every cohort in flight (cohort l
 //      sits at layer l), acc_l = pool_l * W_l^T -- all layers at once, so
-//      requests that arrived at different times share the device; its
-//      descriptors ride in the kernel arguments and the next launch zeroes
-//      its queue counter, so the launch is the step's only GEMM-side call;
-//   3. one launch requantizes acc_l -> min(2^act_bits - 1, max(0, acc >> s_l))
-//      and packs it (one __ballot per plane word, plus the int8 copy) into
-//      pool l+1, where cohort l sits next step, and writes the last layer's
-//      rows straight into pinned host memory; it leaves every accumulator
-//      it read zero, ready for the next step's K-split products to add into;
-//   4. those requests finish.
+//      requests that arrived at different times share the device -- and
+//      requantizes the result in the same launch: min(2^act_bits - 1,
+//      max(0, acc >> s_l)) as int8 codes and bit-planes into pool l+1 of
+//      parity t+1 (the pool cohort l+1 reads in this launch has parity t,
+//      so the two never meet), the last layer's int32 rows straight into
+//      pinned host memory.  Whole-K output tiles convert in their epilogue,
+//      from registers; products split along K add their pieces into zeroed
+//      accumulators, which a second, small launch converts and zeroes
+//      again.  Its descriptors ride in the kernel arguments and it zeroes
+//      the queue counter of the next launch (two alternate);
+//   3. those requests finish.
 //
-// A call of gbs_server_step enqueues one step and then completes the step
-// before it (waits for its event, hands its last-layer rows to their
-// requests): the host admits and plans step t+1 while the GPU still runs
-// step t.  The pinned staging buffers (inputs, outputs) are doubled for
-// that; device buffers need no doubling, the stream orders
-// their reuse.  A request therefore becomes ready one call after the step
-// that computed it.
-//
-// Rows of different requests never mix inside a GEMM row, and integer
-// arithmetic is exact, so a request's result is the same alone or in any
-// mix.  A server is driven by one host thread.  Weights may be re-imported
-// between steps (same shapes); each step multiplies by the current ones.
 #include <string.h>
 
 #include <chrono>
@@ -64,63 +54,26 @@ struct Req {
 
 constexpr int kMaxLayers = 16;
 
-// One activation job over rows [0, rows): mode 0 packs input codes into a
-// pool; mode 1 requantizes int32 accumulators into a pool; mode 2 copies
-// the last layer's accumulators out.  Modes 1 and 2 leave the accumulators
-// they read zero: the next step's split products add into them.
-struct ActJob {
-  const uint8_t *in;     // mode 0: input codes, rows x h (row-major)
-  int32_t *acc;          // modes 1, 2: int32 accumulators, rows x h
-  uint64_t *planes;      // modes 0, 1: pool planes [nbits][R][wpr]
-  int8_t *vals;          // modes 0, 1: pool int8 copy [.][ld]
-  uint64_t wpr, wpp, ld;
-  uint32_t rows, h, mode, shift;
-  uint64_t unit_begin;   // prefix of rows * wpr over the jobs
-  int32_t *out;          // mode 2: rows x h int32
-};
-
-struct ActJobs {
-  ActJob j[kMaxLayers];
-  int n;
-  uint32_t nbits, maxv;
-};
-
-// One wave per (job, row, plane word): lane = column of that word.
-// reset (optional): the GEMM launch's queue counter, zeroed here because
-// this launch follows that one in stream order.
-__global__ __launch_bounds__(256) void act_pack_kernel(ActJobs J, uint64_t units,
-                                                       unsigned int *reset) {
-  if (reset && blockIdx.x == 0 && threadIdx.x == 0) *reset = 0;
+// An admitted cohort's input codes (rows x h, row-major, pinned host
+// memory) into a pool: int8 copy and bit-planes.  One wave per (row, plane
+// word), lane = column of that word, one ballot per plane.
+__global__ __launch_bounds__(256) void input_pack_kernel(const uint8_t *in, uint64_t *planes,
+                                                         int8_t *vals, uint64_t wpr, uint64_t wpp,
+                                                         uint64_t ld, uint32_t h, uint64_t units,
+                                                         uint32_t nbits, uint32_t maxv) {
   const uint32_t lane = threadIdx.x & 63;
   const uint64_t u = (uint64_t)blockIdx.x * 4 + threadIdx.x / 64;
   if (u >= units) return;
-  int k = 0;
-  while (k + 1 < J.n && J.j[k + 1].unit_begin <= u) k++;
-  const ActJob &jb = J.j[k];
-  const uint64_t local = u - jb.unit_begin;
-  const uint64_t r = local / jb.wpr, w = local - r * jb.wpr;
+  const uint64_t r = u / wpr, w = u - r * wpr;
   const uint64_t c = w * 64 + lane;
   uint32_t code = 0;
-  if (jb.mode == 2) {
-    if (c < jb.h) {
-      jb.out[r * jb.h + c] = jb.acc[r * jb.h + c];
-      jb.acc[r * jb.h + c] = 0;
-    }
-    return;  // the whole wave: a job's words are one mode
+  if (c < h) {
+    code = in[r * h + c] & maxv;
+    vals[r * ld + c] = (int8_t)code;
   }
-  if (c < jb.h) {
-    if (jb.mode == 0) {
-      code = jb.in[r * jb.h + c] & J.maxv;
-    } else {
-      const int32_t a = jb.acc[r * jb.h + c] >> jb.shift;  // arithmetic shift
-      jb.acc[r * jb.h + c] = 0;
-      code = a < 0 ? 0u : (a > (int32_t)J.maxv ? J.maxv : (uint32_t)a);
-    }
-    jb.vals[r * jb.ld + c] = (int8_t)code;
-  }
-  for (uint32_t b = 0; b < J.nbits; b++) {
+  for (uint32_t b = 0; b < nbits; b++) {
     const uint64_t m = __ballot((code >> b) & 1u);
-    if (lane == 0) jb.planes[b * jb.wpp + r * jb.wpr + w] = m;
+    if (lane == 0) planes[b * wpp + r * wpr + w] = m;
   }
 }
 
@@ -130,9 +83,10 @@ struct gbs_server {
   uint32_t L = 0, act_bits = 0, R = 0, max_requests = 0, max_rows = 0;
   std::vector<const gbs_matrix *> W;
   std::vector<uint32_t> h, shift;
-  std::vector<gbs_matrix *> pool;
-  std::vector<int32_t *> acc;                  // zero between steps
-  unsigned int *d_counter = nullptr;           // the GEMM launch's queue counter
+  std::vector<gbs_matrix *> pool;              // pool[2 * l + parity]
+  std::vector<int32_t *> acc;                  // split products' sums, zero between steps
+  unsigned int *d_counter = nullptr;           // the GEMM launches' queue counters (two)
+  uint64_t launches = 0;
   // pinned host staging, two halves each, read and written by the kernels
   // in place (no copies): inputs of an admitted cohort, last-layer rows
   uint8_t *h_in = nullptr;
@@ -146,8 +100,8 @@ struct gbs_server {
   std::vector<std::vector<Req *>> cohort;  // cohort[l]: requests at layer l
   std::vector<uint32_t> cohort_rows;
   std::map<uint64_t, Req *> finished;
-  uint64_t steps = 0, done_requests = 0, done_rows = 0;
-  double ops_done = 0;
+  uint64_t steps = 0, done_requests = 0, done_rows = 0, layer_steps = 0;
+  double ops_done = 0, hbm_bytes = 0;
 };
 
 using namespace gbs;
@@ -223,10 +177,11 @@ int gbs_server_create(const gbs_matrix *const *weights, const uint32_t *shifts,
     s->h.push_back((uint32_t)w->nrows);
     s->shift.push_back(l + 1 < n_layers ? shifts[l] : 0);
   }
-  // bytes one row of capacity costs across every pool and buffer
-  uint64_t per_row = s->h[0] + 4ull * s->h[n_layers];  // input / output staging
+  // bytes one row of capacity costs across every pool and buffer: the
+  // staging, two pools per layer and the accumulators
+  uint64_t per_row = s->h[0] + 4ull * s->h[n_layers];
   for (uint32_t l = 0; l < n_layers; l++)
-    per_row += act_bits * align_up(s->h[l], 64) / 8 + align_up(s->h[l], kValsColAlign) +
+    per_row += 2 * (act_bits * align_up(s->h[l], 64) / 8 + align_up(s->h[l], kValsColAlign)) +
                4ull * s->h[l + 1];
   uint64_t budget = cfg->mem_budget;
   if (budget == 0) {
@@ -248,22 +203,25 @@ int gbs_server_create(const gbs_matrix *const *weights, const uint32_t *shifts,
   s->cohort_rows.assign(n_layers, 0);
   const uint64_t Ra = align_up(R, kValsRowAlign);
   for (uint32_t l = 0; l < n_layers && !rc; l++) {
-    gbs_matrix *p = nullptr;
-    rc = gbs_matrix_create(act_bits, R, s->h[l], 0, R, align_up(s->h[l], 64), &p);
-    if (rc) break;
-    s->pool.push_back(p);
-    const uint64_t ld = align_up(s->h[l], kValsColAlign);
-    hipError_t e = hipMalloc((void **)&p->d_vals, Ra * ld);
-    if (e == hipSuccess) e = hipMemset(p->d_vals, 0, Ra * ld);
-    if (e != hipSuccess) {
-      rc = hip_fail(e, "gbs_server_create: pool");
-      break;
+    for (int par = 0; par < 2 && !rc; par++) {
+      gbs_matrix *p = nullptr;
+      rc = gbs_matrix_create(act_bits, R, s->h[l], 0, R, align_up(s->h[l], 64), &p);
+      if (rc) break;
+      s->pool.push_back(p);
+      const uint64_t ld = align_up(s->h[l], kValsColAlign);
+      hipError_t e = hipMalloc((void **)&p->d_vals, Ra * ld);
+      if (e == hipSuccess) e = hipMemset(p->d_vals, 0, Ra * ld);
+      if (e != hipSuccess) {
+        rc = hip_fail(e, "gbs_server_create: pool");
+        break;
+      }
+      p->vals_rows = Ra;
+      p->vals_ld = ld;
+      p->vals_version = p->planes_version;  // kept current by the step launches
     }
-    p->vals_rows = Ra;
-    p->vals_ld = ld;
-    p->vals_version = p->planes_version;  // kept current by act_pack_kernel
+    if (rc) break;
     int32_t *a = nullptr;
-    e = hipMalloc((void **)&a, R * s->h[l + 1] * 4ull);
+    hipError_t e = hipMalloc((void **)&a, R * s->h[l + 1] * 4ull);
     if (e == hipSuccess) e = hipMemset(a, 0, R * s->h[l + 1] * 4ull);
     if (e != hipSuccess) {
       rc = hip_fail(e, "gbs_server_create: accumulators");
@@ -341,70 +299,66 @@ int gbs_server_step(gbs_server *s, uint32_t *finished_now) {
     s->cohort[0].push_back(r);
   }
   s->cohort_rows[0] = n0;
+  const int par = (int)(s->steps & 1);  // pools this step reads; it writes parity par ^ 1
   if (n0) {
-    ActJobs J{};
-    J.nbits = s->act_bits;
-    J.maxv = (1u << s->act_bits) - 1;
-    gbs_matrix *p = s->pool[0];
-    J.j[0] = ActJob{h_in, nullptr, p->d_planes, p->d_vals, p->wpr(), p->wpp(),
-                    p->vals_ld, n0, s->h[0], 0, 0, 0, nullptr};
-    J.n = 1;
+    gbs_matrix *p = s->pool[par];
     const uint64_t units = (uint64_t)n0 * p->wpr();
-    hipLaunchKernelGGL(act_pack_kernel, dim3((uint32_t)((units + 3) / 4)), dim3(256), 0, st,
-                       J, units, (unsigned int *)nullptr);
+    hipLaunchKernelGGL(input_pack_kernel, dim3((uint32_t)((units + 3) / 4)), dim3(256), 0, st,
+                       (const uint8_t *)h_in, p->d_planes, p->d_vals, p->wpr(), p->wpp(),
+                       p->vals_ld, s->h[0], units, s->act_bits, (1u << s->act_bits) - 1);
     GBS_HIP(hipGetLastError());
   }
-  // 2. every layer's GEMM in one persistent launch (weights re-imported
-  // since the last step get their int8 copies rebuilt first)
+  // 2. every layer's GEMM and requantization in one persistent launch
+  // (weights re-imported since the last step get their int8 copies rebuilt
+  // first)
   for (uint32_t l = 0; l < L; l++)
     if ((rc = ensure_vals(const_cast<gbs_matrix *>(s->W[l]), st, false))) return rc;
   MfmaOperands views[kMaxLayers];
+  MfmaRequant rq[kMaxLayers];
   int nv = 0;
   for (uint32_t l = 0; l < L; l++) {
     const uint32_t n = s->cohort_rows[l];
     if (!n) continue;
+    const gbs_matrix *in = s->pool[2 * l + par];
     MfmaOperands v{};
-    v.R = s->pool[l]->d_vals;
-    v.r_ld = s->pool[l]->vals_ld;
+    v.R = in->d_vals;
+    v.r_ld = in->vals_ld;
     v.L = s->W[l]->d_vals;
     v.l_ld = s->W[l]->vals_ld;
     v.res = s->acc[l];
     v.M = s->h[l + 1];
     v.N = n;
     v.kbytes = (uint32_t)align_up(s->h[l], kMfmaKS);
-    views[nv++] = v;
+    MfmaRequant q{};
+    if (l + 1 < L) {
+      gbs_matrix *o = s->pool[2 * (l + 1) + (par ^ 1)];
+      q.vals = o->d_vals;
+      q.planes = o->d_planes;
+      q.ld = o->vals_ld;
+      q.wpr = o->wpr();
+      q.wpp = o->wpp();
+      q.shift = s->shift[l];
+      q.nbits = s->act_bits;
+      q.maxv = (1u << s->act_bits) - 1;
+    } else {
+      q.out = h_out;
+    }
+    views[nv] = v;
+    rq[nv++] = q;
     s->ops_done += 2.0 * s->h[l + 1] * s->h[l] * n;
+    // algorithmic bytes: weight planes, input codes, output codes (int32 rows last)
+    s->hbm_bytes += (double)s->W[l]->nbits * s->h[l + 1] * s->h[l] / 8 +
+                    (double)s->act_bits * n * s->h[l] / 8 +
+                    (l + 1 < L ? (double)s->act_bits * n * s->h[l + 1] / 8
+                               : 4.0 * n * s->h[l + 1]);
+    s->layer_steps++;
   }
-  if (nv && (rc = launch_views(views, nv, s->d_counter, st))) return rc;
-  // 3. requantize + pack into the next layer's pool; the last layer's rows
-  // to the output buffer (both leave their accumulators zero)
-  ActJobs J{};
-  J.nbits = s->act_bits;
-  J.maxv = (1u << s->act_bits) - 1;
-  uint64_t units = 0;
-  for (uint32_t l = 0; l + 1 < L; l++) {
-    const uint32_t n = s->cohort_rows[l];
-    if (!n) continue;
-    gbs_matrix *p = s->pool[l + 1];
-    J.j[J.n++] = ActJob{nullptr, s->acc[l], p->d_planes, p->d_vals, p->wpr(), p->wpp(),
-                        p->vals_ld, n, s->h[l + 1], 1, s->shift[l], units, nullptr};
-    units += (uint64_t)n * p->wpr();
+  if (nv) {
+    unsigned int *cur_c = s->d_counter + (s->launches & 1), *next_c = s->d_counter + ((s->launches + 1) & 1);
+    if ((rc = launch_views(views, rq, nv, cur_c, next_c, st))) return rc;
+    s->launches++;
   }
-  const uint32_t nl = s->cohort_rows[L - 1];
-  if (nl) {
-    const uint64_t wpr = align_up(s->h[L], 64) / 64;
-    J.j[J.n++] = ActJob{nullptr, s->acc[L - 1], nullptr, nullptr, wpr, 0, 0, nl, s->h[L], 2,
-                        0, units, h_out};
-    units += (uint64_t)nl * wpr;
-  }
-  // (every cohort the GEMM launch served has a job here, so units > 0
-  // whenever that launch ran and this one zeroes its queue counter)
-  if (units || nv) {
-    hipLaunchKernelGGL(act_pack_kernel, dim3((uint32_t)std::max<uint64_t>(1, (units + 3) / 4)),
-                       dim3(256), 0, st, J, units, s->d_counter);
-    GBS_HIP(hipGetLastError());
-  }
-  // 4. the last layer's rows are in h_out once this step's event has
+  // 3. the last layer's rows are in h_out once this step's event has
   // passed (read in the next call)
   GBS_HIP(hipEventRecord(s->ev[cur], st));
   s->completing[cur].swap(s->cohort[L - 1]);
@@ -418,7 +372,7 @@ int gbs_server_step(gbs_server *s, uint32_t *finished_now) {
   s->cohort[0].clear();
   s->cohort_rows[0] = 0;
   s->steps++;
-  // 5. complete the previous step
+  // 4. complete the previous step
   uint32_t fin = 0;
   if (s->pending[prev]) {
     GBS_HIP(hipEventSynchronize(s->ev[prev]));
@@ -463,6 +417,31 @@ int gbs_server_take(gbs_server *s, uint64_t id, int32_t *out, uint64_t cap_elems
   return GBS_OK;
 }
 
+int gbs_server_finished(const gbs_server *s, uint64_t *ids, uint32_t cap, uint32_t *n) {
+  if (!s || !n || (cap && !ids)) return fail(GBS_EINVAL, "gbs_server_finished: null argument");
+  uint32_t k = 0;
+  for (const auto &kv : s->finished) {
+    if (k >= cap) break;
+    ids[k++] = kv.first;
+  }
+  *n = k;
+  return GBS_OK;
+}
+
+int gbs_server_layer_input(gbs_server *s, uint32_t layer, const gbs_matrix **out,
+                           uint32_t *rows) {
+  if (!s || !out || !rows) return fail(GBS_EINVAL, "gbs_server_layer_input: null argument");
+  if (layer >= s->L)
+    return fail(GBS_ERANGE, "gbs_server_layer_input: layer %u of %u", layer, s->L);
+  int rc = ensure_device();
+  if (rc) return rc;
+  GBS_HIP(hipStreamSynchronize(thread_stream()));
+  // the pools the next step reads have the parity of the next step
+  *out = s->pool[2 * layer + (s->steps & 1)];
+  *rows = s->cohort_rows[layer];
+  return GBS_OK;
+}
+
 int gbs_server_stats(const gbs_server *s, gbs_server_counters *st) {
   if (!s || !st) return fail(GBS_EINVAL, "gbs_server_stats: null argument");
   st->steps = s->steps;
@@ -476,6 +455,8 @@ int gbs_server_stats(const gbs_server *s, gbs_server_counters *st) {
   for (const auto &c : s->cohort) act += (uint32_t)c.size();
   for (const auto &c : s->completing) act += (uint32_t)c.size();
   st->in_flight = act;
+  st->layer_steps = s->layer_steps;
+  st->hbm_bytes = s->hbm_bytes;
   return GBS_OK;
 }
 

@@ -12,6 +12,22 @@ gbs_tune_load.  The times let AUTO pick FP6 over int8 copies only for the
 classes where FP6 measured faster.
 
     python -m gemmbitserial_amd.tune [--out PATH] [--reps 20]
+
+The persistent launch has a table of its own (a shared queue needs no grid
+filling, so the standalone table's shapes do not carry over): per batch
+class (rhs-row bucket x depth bucket, gbs_batch_tune_class_of) and copy
+format, the tile shape and the largest K piece of a work item.  --batch
+tunes it by coordinate descent on whole persistent launches of a mixed
+workload (512 uneven products, 1..4-bit operands, random signedness, the
+same generator as the batched benchmark with another seed): for every
+class present, heaviest first, each carried shape x K piece of 2, 4 or
+8 KiB (nominal) is planned into the batch and the launch timed (best of
+three runs of --reps launches); a candidate replaces the class's entry
+when the launch gets at least 0.5 % faster.  The result is checked bit
+for bit against one launch per product and written to
+batch_tune_table.txt, which gbs_init loads.
+
+    python -m gemmbitserial_amd.tune --batch [--out PATH] [--reps 10] [--passes 2]
 """
 import argparse
 import itertools
@@ -26,6 +42,8 @@ from gemmbitserial_amd import workloads as W
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_OUT = os.path.join(HERE, "tune_table.txt")
+DEFAULT_BATCH_OUT = os.path.join(HERE, "batch_tune_table.txt")
+BATCH_PIECES = (8, 16, 32)  # K units of 256 nominal bytes: 2, 4, 8 KiB
 # representative sizes of the class buckets (see shape_class in
 # csrc/gbs_mfma_tile.h): M <256 / <1024 / <4096 / <16384 / >=16384,
 # N <=16 / <=32 / <=64 / <=256 / <=2048 / >2048, K <1024 / <4096 / >=4096.
@@ -118,11 +136,140 @@ def write_table(shapes, splits, detail, path):
                         f"unsplit={d.get('us_unsplit_by_shape')}\n")
 
 
+def _plane_bytes(p):
+    return (p["M"] * p["K"] * p["lb"] + p["N"] * p["K"] * p["rb"]) // 8 + 4 * p["M"] * p["N"]
+
+
+def tune_batch(reps=10, passes=2, seed=W.SEED0 + 33, log=print):
+    """Coordinate descent over the batch tile table on one mixed workload;
+    returns (entries, detail): entries maps (fmt, cls) -> (shape, piece)."""
+    gbs.init(0)
+    stream = gbs.stream()
+    probs = W.batched_uneven(seed=seed)
+    mats = []
+    for p in probs:
+        mats.append((gbs.from_values(p["lhs"], p["lb"], p["ls"]),
+                     gbs.from_values(p["rhs"], p["rb"], p["rs"]),
+                     gbs.DeviceBuffer(4 * p["M"] * p["N"])))
+    problems = [(l, r, d.ptr) for l, r, d in mats]
+    n_cls = gbs.batch_tune_classes()
+    for f in range(len(FORMATS)):
+        for c in range(n_cls):
+            gbs.batch_tune_set(f, c, -1)
+
+    def measure():
+        b = gbs.Batch(problems)
+        for _ in range(3):
+            b.run(stream)
+        gbs.sync(stream)
+        best = float("inf")
+        for _ in range(3):
+            e0, e1 = gbs.Event(), gbs.Event()
+            e0.record(stream)
+            for _ in range(reps):
+                b.run(stream)
+            e1.record(stream)
+            gbs.sync(stream)
+            best = min(best, e0.elapsed_ms(e1) / reps)
+        plans = [b.plan_of(i) for i in range(len(problems))]
+        b.close()
+        return best * 1e3, plans
+
+    t_builtin, plans = measure()
+    log(f"batch: built-in rule {t_builtin:.1f} us")
+    weight = {}
+    for p, pl in zip(probs, plans):
+        if pl[0] >= 0:
+            key = (pl[0], gbs.batch_tune_class_of(p["N"], p["K"]))
+            weight[key] = weight.get(key, 0) + _plane_bytes(p)
+    order = sorted(weight, key=lambda k: -weight[k])
+    # start every present class from the shape and piece the built-in rule gave it
+    entries = {}
+    for p, pl in zip(probs, plans):
+        if pl[0] >= 0:
+            key = (pl[0], gbs.batch_tune_class_of(p["N"], p["K"]))
+            entries.setdefault(key, (pl[1], 16))
+    for (f, c), (sh, pc) in entries.items():
+        gbs.batch_tune_set(f, c, sh, pc)
+    cur, _ = measure()
+    for it in range(passes):
+        for key in order:
+            f, c = key
+            best, best_t = entries[key], cur
+            for sh in gbs.batch_tune_shapes(f):
+                for pc in BATCH_PIECES:
+                    if (sh, pc) == entries[key]:
+                        continue
+                    gbs.batch_tune_set(f, c, sh, pc)
+                    t, _ = measure()
+                    if t < best_t * 0.995:
+                        best, best_t = (sh, pc), t
+            entries[key] = best
+            gbs.batch_tune_set(f, c, *best)
+            cur = best_t
+            log(f"batch pass {it}: {FORMATS[f][0]} class {c:2d} ({weight[key] / 1e6:6.1f} MB) "
+                f"-> shape {SHAPES[best[0]]} piece {best[1]}: {cur:.1f} us")
+        cur, _ = measure()  # re-measure the table as a whole after each pass
+        log(f"batch pass {it}: whole table {cur:.1f} us (built-in {t_builtin:.1f} us)")
+    # bit-exact against one launch per product
+    b = gbs.Batch(problems)
+    b.run(stream)
+    gbs.sync(stream)
+    got = [d.to_host(np.int32, (p["N"], p["M"])) for (l, r, d), p in zip(mats, probs)]
+    for _, _, d in mats:
+        d.fill_zero()
+    b.run_per_call(stream)
+    gbs.sync(stream)
+    for g, (l, r, d), p in zip(got, mats, probs):
+        if not np.array_equal(g, d.to_host(np.int32, (p["N"], p["M"]))):
+            raise SystemExit("tuned batch differs from the per-product launches")
+    b.close()
+    counts = {}
+    for p, pl in zip(probs, plans):
+        if pl[0] >= 0:
+            key = (pl[0], gbs.batch_tune_class_of(p["N"], p["K"]))
+            counts[key] = counts.get(key, 0) + 1
+    detail = {"workload": f"batched_uneven(seed={seed:#x}), {len(probs)} products",
+              "builtin_us": round(t_builtin, 1), "tuned_us": round(cur, 1),
+              "products": counts, "mbytes": {k: round(v / 1e6, 1) for k, v in weight.items()}}
+    for x in mats:
+        for y in x:
+            y.close()
+    return entries, detail
+
+
+def write_batch_table(entries, detail, path):
+    with open(path, "w") as f:
+        f.write("# persistent-launch tile shape and K piece per batch class and copy format, "
+                "measured by gemmbitserial_amd.tune --batch\n")
+        f.write(f"# workload: {detail['workload']}; whole launch {detail['tuned_us']} us "
+                f"(built-in rule {detail['builtin_us']} us)\n")
+        f.write("# classes: rhs rows <=16/<=32/<=64/<=128/<=256/more x depth <1024/<4096/more "
+                "(class = 3 * rows bucket + depth bucket)\n")
+        f.write("# shapes: " + " ".join(f"{i}={s}" for i, s in enumerate(SHAPES)) + "\n")
+        f.write("# formats: " + " ".join(f"{i}={n}" for i, (n, _) in enumerate(FORMATS)) + "\n")
+        f.write("# class format shape piece_units  # products in the workload, plane MB\n")
+        for (fmt, cls), (sh, pc) in sorted(entries.items(), key=lambda kv: (kv[0][1], kv[0][0])):
+            f.write(f"{cls} {fmt} {sh} {pc}  # {detail['products'].get((fmt, cls), 0)} products, "
+                    f"{detail['mbytes'].get((fmt, cls), 0)} MB\n")
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--out", default=DEFAULT_OUT)
+    ap.add_argument("--out", default=None)
     ap.add_argument("--reps", type=int, default=10)
+    ap.add_argument("--batch", action="store_true", help="tune the persistent launch's table")
+    ap.add_argument("--passes", type=int, default=2)
     a = ap.parse_args()
+    if a.batch:
+        entries, detail = tune_batch(a.reps, a.passes,
+                                     log=lambda *x: print(*x, file=sys.stderr, flush=True))
+        out = a.out or DEFAULT_BATCH_OUT
+        write_batch_table(entries, detail, out)
+        print(json.dumps({"builtin_us": detail["builtin_us"], "tuned_us": detail["tuned_us"],
+                          "entries": len(entries), "out": out}))
+        return
+    a.out = a.out or DEFAULT_OUT
     shapes, splits, detail = tune(a.reps, log=lambda *x: print(*x, file=sys.stderr, flush=True))
     write_table(shapes, splits, detail, a.out)
     print(json.dumps({"shapes": shapes, "splits": splits, "out": a.out}))

@@ -323,6 +323,34 @@ int gbs_batch_run_graph(gbs_batch *b, void *stream);
 int gbs_batch_run_per_call(gbs_batch *b, void *stream);
 int gbs_batch_info(const gbs_batch *b, uint64_t *tiles, uint64_t *persistent,
                    uint64_t *separate);
+/* The plan a batch made for problem i: copy format (0 int8, 1 FP4, 2 FP6;
+ * -1 for a product run by the popcount engine), tile shape, K splits and K
+ * piece (units of 256 nominal bytes). */
+int gbs_batch_plan_of(const gbs_batch *b, uint64_t problem, int *fmt, int *shape,
+                      uint32_t *splits, uint32_t *piece_units);
+
+/* Batch tile table: the persistent launch's tile shape and largest K piece
+ * per batch class (rhs-row bucket x depth bucket) and copy format, measured
+ * on whole persistent launches of a mixed workload by
+ * gemmbitserial_amd/tune.py --batch and loaded at gbs_init from
+ * batch_tune_table.txt beside the library ($GBS_BATCH_TUNE_FILE
+ * overrides).  A class without an entry keeps the built-in rule (least
+ * padding for the rhs row count, pieces of at most 16 units).  Used by
+ * gbs_batch_create and the serving loop's step launches; plans already
+ * made keep theirs.  Process-wide: set it while no other thread plans.
+ * No reference counterpart (the reference sizes CPU cache blocks,
+ * gemmbitserial.hpp:343-377). */
+int gbs_batch_tune_classes(int *n_classes);
+int gbs_batch_tune_class_of(uint64_t N, uint64_t K, int *cls);
+/* The tile shapes the persistent kernel carries for a format. */
+int gbs_batch_tune_shapes(int fmt, int *shapes, int cap, int *n);
+/* shape -1: the built-in rule for that class; piece_units in [1, 4096]. */
+int gbs_batch_tune_set(int fmt, int cls, int shape, int piece_units);
+int gbs_batch_tune_get(int fmt, int cls, int *shape, int *piece_units);
+/* Text table, one "class format shape piece_units [microseconds]" line per
+ * entry, '#' comments; replaces the whole table (classes not listed revert
+ * to the built-in rule). */
+int gbs_batch_tune_load(const char *path);
 
 /* Tile autotuning of the MFMA engine -------------------------------------- */
 /* Products fall into shape classes (buckets of M, N and K); each class maps
@@ -353,9 +381,12 @@ int gbs_gemm_async_config(gbs_workspace *ws, const gbs_matrix *lhs,
  * stay resident in HBM in bit-serial form (plus their int8 copy).  Each
  * request is 1..max_rows rows of act_bits-bit unsigned activations; between
  * layers int32 results are requantized to min(2^act_bits - 1,
- * max(0, acc >> shift_l)) and packed into bit-planes on the GPU.  Every step
- * admits waiting requests as row capacity and request slots free up, and
- * runs the GEMMs of all requests in flight (each at its own layer) in one
+ * max(0, acc >> shift_l)) and packed into bit-planes (and int8 codes) on the
+ * GPU by the GEMM launch itself, in its epilogue (generalising
+ * importRegularAndQuantize, gemmbitserial.hpp:281-308, to a shift-and-clamp
+ * requantization on the device).  Every step admits waiting requests as row
+ * capacity and request slots free up, and runs the GEMMs and
+ * requantizations of all requests in flight (each at its own layer) in one
  * persistent launch.  No reference counterpart: the reference ships only
  * the single-GEMM call (its MNIST test, test/test.cpp:157-172, runs one
  * layer).  A server is driven by one host thread. */
@@ -372,6 +403,10 @@ typedef struct gbs_server_counters {
   uint32_t row_capacity;  /* rows one cohort may hold (from mem_budget) */
   uint64_t bytes_reserved;
   uint32_t waiting, in_flight;
+  uint64_t layer_steps;   /* layer GEMMs run, summed over steps */
+  double hbm_bytes;       /* algorithmic bytes of those GEMMs: weight planes,
+                             input and output activation planes (int32 rows
+                             out of the last layer) */
 } gbs_server_counters;
 
 /* Nanoseconds of the steady clock the server stamps requests with. */
@@ -399,6 +434,16 @@ int gbs_server_step(gbs_server *s, uint32_t *finished_now);
 int gbs_server_take(gbs_server *s, uint64_t id, int32_t *out, uint64_t cap_elems,
                     int *ready, uint32_t *rows, int64_t *arrival_ns, int64_t *done_ns);
 int gbs_server_stats(const gbs_server *s, gbs_server_counters *st);
+/* Ids of up to cap finished requests not yet taken (ascending), *n of them. */
+int gbs_server_finished(const gbs_server *s, uint64_t *ids, uint32_t cap, uint32_t *n);
+/* The bit-serial input of the cohort that sits at `layer` for the next
+ * step: an act_bits-plane matrix of row capacity x h_layer (planes in the
+ * layout above; the first *rows rows are the cohort's, its requests in
+ * admission order), valid until the next step.  Waits for the server's
+ * stream.  For layer > 0 these are the requantized outputs of layer - 1,
+ * written by the step launch's epilogue. */
+int gbs_server_layer_input(gbs_server *s, uint32_t layer, const gbs_matrix **out,
+                           uint32_t *rows);
 
 #ifdef __cplusplus
 }

@@ -100,3 +100,39 @@ def test_matrix_load_rejects_bad_files_before_touching_the_device(tmp_path):
         msg = L.gbs_last_error()
         assert b"not a packed-matrix file" in msg or b"inconsistent" in msg, (name, msg)
     assert L.gbs_matrix_load(b"/nonexistent/x.gbsp", C.byref(out)) == 1
+
+
+def test_batch_tune_table_validation_without_device():
+    """The persistent launch's tile table: entries are checked against the
+    shapes the persistent kernel carries per format, and the committed table
+    (written by the tuner) loads."""
+    import gemmbitserial_amd as g
+    n = g.batch_tune_classes()
+    assert n == 18
+    assert g.batch_tune_class_of(1, 256) == 0 and g.batch_tune_class_of(256, 8192) == 14
+    assert g.batch_tune_class_of(257, 1024) == 16
+    for fmt in range(3):
+        shapes = g.batch_tune_shapes(fmt)
+        assert shapes and 13 in shapes or 18 in shapes
+    assert g.batch_tune_shapes(2) == [2, 18]
+    L = g.lib()
+    saved = [[g.batch_tune_get(f, c) for c in range(n)] for f in range(3)]
+    try:
+        assert L.gbs_batch_tune_set(2, 0, 13, 16) == 1      # 64x64 dl4 is not an FP6 shape
+        assert b"not carried" in L.gbs_last_error()
+        assert L.gbs_batch_tune_set(1, 0, 13, 0) == 1       # piece out of range
+        assert L.gbs_batch_tune_set(1, n, 13, 16) == 1      # class out of range
+        g.batch_tune_set(1, 4, 2, 8)
+        assert g.batch_tune_get(1, 4) == (2, 8)
+        g.batch_tune_set(1, 4, -1)
+        assert g.batch_tune_get(1, 4) == (-1, 0)
+        table = os.path.join(ROOT, "gemmbitserial_amd", "batch_tune_table.txt")
+        if os.path.exists(table):
+            g.batch_tune_load(table)
+            got = [g.batch_tune_get(f, c) for f in range(3) for c in range(n)]
+            assert any(s >= 0 for s, _ in got)
+    finally:
+        for f in range(3):
+            for c in range(n):
+                s, p = saved[f][c]
+                g.batch_tune_set(f, c, s, p if s >= 0 else 16)

@@ -150,3 +150,44 @@ def test_batch_sees_reimported_operands(gbs, orc):
             for g, p in zip(results(items, probs), probs):
                 assert np.array_equal(g, exact(p)), (run.__name__, step, p["lb"], p["rb"])
     b.close()
+
+
+def test_batch_plans_follow_the_tile_table(gbs, orc):
+    """Products of a class with a table entry are planned with that entry's
+    tile shape (here forced to each carried shape in turn, with a fine and a
+    coarse K piece) and stay bit-exact."""
+    rng = np.random.default_rng(21)
+    probs = [dict(M=300, K=2560, N=12, lb=2, rb=1, ls=False, rs=False),   # FP4, N <= 16
+             dict(M=260, K=5120, N=100, lb=4, rb=3, ls=True, rs=False)]   # FP6, N <= 128
+    for p in probs:
+        p["lhs"] = W.values(rng, (p["M"], p["K"]), p["lb"], p["ls"])
+        p["rhs"] = W.values(rng, (p["N"], p["K"]), p["rb"], p["rs"])
+    items = build(gbs, probs)
+    want = [exact(p) for p in probs]
+    n = gbs.batch_tune_classes()
+    saved = [[gbs.batch_tune_get(f, c) for c in range(n)] for f in range(3)]
+    try:
+        for fmt, p, k in ((1, probs[0], 0), (2, probs[1], 1)):
+            cls = gbs.batch_tune_class_of(p["N"], p["K"])
+            for shape in gbs.batch_tune_shapes(fmt):
+                for piece in (4, 32):
+                    gbs.batch_tune_set(fmt, cls, shape, piece)
+                    zero(items)
+                    b = gbs.Batch([(l, r, d.ptr) for l, r, d in items])
+                    f, s, splits, pc = b.plan_of(k)
+                    assert (f, s) == (fmt, shape)
+                    units = p["K"] // 2 // 256  # nominal 256-byte units of FP4 / FP6 copies
+                    assert splits * pc >= units and (splits - 1) * pc < units
+                    b.run()
+                    gbs.sync()
+                    for got, w in zip(results(items, probs), want):
+                        assert np.array_equal(got, w), (fmt, shape, piece)
+                    b.close()
+    finally:
+        for f in range(3):
+            for c in range(n):
+                s, pc = saved[f][c]
+                gbs.batch_tune_set(f, c, s, pc if s >= 0 else 16)
+        for x in items:
+            for y in x:
+                y.close()

@@ -83,9 +83,10 @@ def test_serve_staggered_arrivals(gbs):
 def test_serve_admission_by_memory(gbs):
     h, Ws, shifts = small_mlp(5)
     # a budget that holds only ~10 rows per cohort: requests queue, not fail
-    # (per row: every pool's planes and int8 copy, every layer's int32
-    # accumulators, the input and the last layer's output staging)
-    per_row = sum(4 * ((x + 63) // 64 * 64) // 8 + (x + 255) // 256 * 256 for x in h[:-1]) \
+    # (per row: two pools per layer, each planes and int8 copy, every
+    # layer's int32 accumulators, the input and the last layer's output
+    # staging)
+    per_row = sum(2 * (4 * ((x + 63) // 64 * 64) // 8 + (x + 255) // 256 * 256) for x in h[:-1]) \
         + sum(4 * x for x in h[1:]) + h[0] + 4 * h[-1]
     srv, mats = make_server(gbs, h, Ws, shifts, max_requests=64, max_rows=8,
                             mem_budget=per_row * 10)
@@ -152,4 +153,76 @@ def test_serve_shallow_stacks(gbs, layers):
     got = drain(srv, [r["id"] for r in reqs])
     for r in reqs:
         assert np.array_equal(got[r["id"]][0], W.mlp_reference(r["x"], Ws, shifts)), r["id"]
+    srv.close()
+
+
+def _layer_codes(x, Ws, shifts, upto):
+    """Requantized activations entering layer `upto` (the exact forward pass)."""
+    a = x.astype(np.int64)
+    for l in range(upto):
+        acc = a @ Ws[l].astype(np.int64).T
+        a = np.clip(acc >> shifts[l], 0, 15)
+    return a
+
+
+@pytest.mark.parametrize("widths", [(150, 200, 90, 120), (3000, 1100, 4096, 1500)])
+def test_serve_fused_requant_writes_planes(gbs, orc, widths):
+    """The step launch requantizes each layer's sums into the next layer's
+    input: narrow layers (whole-K tiles, converted from registers in the
+    epilogue) and wide ones (split along K, converted after the pieces).  The bit-planes it leaves for every layer equal the oracle's
+    import of the exact requantized activations, and the final rows equal
+    the exact forward pass."""
+    from oracle import orc as O
+    rng = np.random.default_rng(sum(widths))
+    h = list(widths)
+    Ws = [W.values(rng, (h[l + 1], h[l]), 4, True) for l in range(len(h) - 1)]
+    shifts = [W.mlp_shift(h[l]) for l in range(len(h) - 2)]
+    srv, mats = make_server(gbs, h, Ws, shifts, max_requests=8, max_rows=40)
+    xs = [rng.integers(0, 16, (n, h[0])).astype(np.uint8) for n in (7, 33, 1)]
+    for i, x in enumerate(xs):
+        srv.submit(i, x)
+    stacked = np.concatenate(xs).astype(np.int64)
+    got = {}
+    for layer in range(1, len(h) - 1):
+        srv.step()  # the cohort moves to `layer`
+        planes, rows = srv.layer_input(layer)
+        assert rows == stacked.shape[0]
+        want = O.import_regular(_layer_codes(stacked, Ws, shifts, layer).astype(np.int8), 4,
+                                False, nrows_a=rows, ncols_a=(h[layer] + 63) // 64 * 64)
+        assert np.array_equal(planes[:, :rows, :].reshape(-1), want.planes.reshape(-1)), layer
+        for i in range(len(xs)):
+            t = srv.take(i)
+            if t is not None:
+                got[i] = t
+    got.update(drain(srv, [i for i in range(len(xs)) if i not in got]))
+    for i, x in enumerate(xs):
+        assert np.array_equal(got[i][0], W.mlp_reference(x, Ws, shifts)), i
+    st = srv.stats()
+    assert st["layer_steps"] == len(Ws)
+    assert st["hbm_bytes"] > 0
+    srv.close()
+
+
+def test_serve_finished_lists_ready_ids(gbs):
+    """gbs_server_finished names exactly the requests that take() would
+    return, and taking one removes it from the list."""
+    h, Ws, shifts = small_mlp(11, layers=2)
+    reqs = W.serving_requests(h[0], 6, 1000.0, seed=12, max_rows=5)
+    srv, mats = make_server(gbs, h, Ws, shifts, max_requests=6, max_rows=5)
+    for r in reqs:
+        srv.submit(r["id"], r["x"])
+    seen = {}
+    for _ in range(50):
+        srv.step()
+        for i in srv.finished():
+            assert i not in seen
+            t = srv.take(i)
+            assert t is not None
+            seen[i] = t
+        assert not srv.finished()
+        if len(seen) == len(reqs):
+            break
+    assert sorted(seen) == [r["id"] for r in reqs]
+    for r in reqs:
+        assert np.array_equal(seen[r["id"]][0], W.mlp_reference(r["x"], Ws, shifts))
     srv.close()
