@@ -233,6 +233,31 @@ def run_config(gbs, O, name, runs, engines, steps, warmup, stream, ws, cpu_secs)
                                    "bit_exact": True}
             log(f"[bench] {name} {enc}: {eng} {ms * 1e3:.1f} us, "
                 f"{2.0 * M * K * N / sec / 1e12:.1f} TOPS, bit-exact")
+        # fresh rhs planes before every call (a conv layer's new activations):
+        # AUTO, and the int8 copy engine that then rebuilds its copy each time
+        fresh = {}
+        for eng in ("auto", "mfma"):
+            def once(e=eng):
+                gbs.check(gbs.lib().gbs_matrix_mark_modified(r.h))
+                ws.gemm_async(l, r, d_res.ptr, e, stream)
+            try:
+                once()
+                gbs.sync(stream)
+                picked = gbs.select(l, r) if eng == "auto" else eng
+                e0, e1 = gbs.Event(), gbs.Event()
+                e0.record(stream)
+                for _ in range(steps):
+                    once()
+                e1.record(stream)
+                gbs.sync(stream)
+            except gbs.GbsError:
+                continue
+            if not np.array_equal(d_res.to_host(np.int32, (N, M)), want):
+                raise SystemExit(f"{name} {enc}: fresh-rhs {eng} differs from the exact product")
+            fresh[eng] = {"us": round(e0.elapsed_ms(e1) / steps * 1e3, 2), "engine": picked}
+        row["fresh_rhs"] = fresh
+        log(f"[bench] {name} {enc}: fresh rhs planes every call: " +
+            ", ".join(f"{k} ({v['engine']}) {v['us']} us" for k, v in fresh.items()))
         e2e, got = time_dropin(gbs, l, r, M, N, max(3, min(20, steps)))
         if want is not None and not np.array_equal(got, want):
             raise SystemExit(f"{name} {enc}: drop-in call result differs from the exact product")

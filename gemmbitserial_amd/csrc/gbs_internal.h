@@ -130,6 +130,8 @@ bool f6_supported(const Operand &l, const Operand &r, uint64_t K);    // FP6 cop
 // copy format AUTO uses (1 FP4, 2 FP6, 0 int8), -1 if none
 int best_format(const Operand &l, const Operand &r, uint64_t K);
 gbs_algo select_algo(const Operand &l, const Operand &r, uint64_t K);
+// AUTO for two matrices: select_algo, with the state of their value copies
+gbs_algo select_for(const gbs_matrix *l, const gbs_matrix *r);
 constexpr uint64_t kValsRowAlign = 256;  // rows of the int8 copy
 constexpr uint64_t kValsColAlign = 256;  // values per row of the int8 copy (bytes)
 constexpr uint64_t kPackedColAlign = 512;  // values per row of the FP4 / FP6 copies

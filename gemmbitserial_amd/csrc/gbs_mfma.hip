@@ -385,6 +385,35 @@ gbs_algo select_algo(const Operand &l, const Operand &r, uint64_t K) {
   return fmt == kFmtF4 ? GBS_ALGO_MFMA_FP4 : fmt == kFmtF6 ? GBS_ALGO_MFMA_FP6 : GBS_ALGO_MFMA;
 }
 
+static bool copies_current(const gbs_matrix *m, int fmt);
+
+// AUTO for two matrices in their present state: select_algo's choice,
+// except that an int8- or FP6-copy engine whose operands' copies would
+// first have to be rebuilt (planes imported or written since the copies
+// were made) gives way to the planes engine for large products that engine
+// holds exactly.  The rebuild is a pass over the planes that writes 2-8x
+// their bytes -- measured on the conv-as-GEMM product (256 x 2304 2-bit
+// signed weights, 50176 x 2304 3-bit unsigned activations): 48 us to
+// rebuild the activations' int8 copy against 52 us for the product on the
+// copies, while the planes engine takes 53 us from the planes alone.
+// FP4 copies stay: they cost a quarter of the int8 bytes to rebuild and run
+// much faster than the planes engine (the 4096 x 8192 x 512 1-bit product:
+// 15.5 against 39.5 us); so do small products, which the planes engine,
+// having no K split, runs on few blocks (256 x 2048 x 256: 15 against 5 us).
+gbs_algo select_for(const gbs_matrix *l, const gbs_matrix *r) {
+  const Operand lo = view(l), ro = view(r);
+  const uint64_t K = l->ncols;
+  const gbs_algo eng = select_algo(lo, ro, K);
+  const int fmt = eng == GBS_ALGO_MFMA_FP4   ? kFmtF4
+                  : eng == GBS_ALGO_MFMA_FP6 ? kFmtF6
+                  : eng == GBS_ALGO_MFMA     ? kFmtI8
+                                             : -1;
+  if (fmt < 0 || fmt == kFmtF4 || (copies_current(l, fmt) && copies_current(r, fmt))) return eng;
+  const double ops = 2.0 * (double)lo.nrows * (double)ro.nrows * (double)K;
+  if (ops >= 8e9 && planes_supported(lo, ro, K)) return GBS_ALGO_MFMA_PLANES;
+  return eng;
+}
+
 int ensure_vals(gbs_matrix *m, hipStream_t s, bool may_allocate) {
   if (m->vals_version == m->planes_version && m->d_vals) return GBS_OK;
   const uint64_t rows = ((m->nrows + kValsRowAlign - 1) / kValsRowAlign) * kValsRowAlign;

@@ -148,7 +148,7 @@ static int gemm_enqueue_into(gbs_workspace *ws, const gbs_matrix *l, const gbs_m
   Operand lo = view(l), ro = view(r);
   const uint64_t K = l->ncols;
   const uint64_t kwords = (K + 63) / 64;
-  gbs_algo eng = algo == GBS_ALGO_AUTO ? select_algo(lo, ro, K) : algo;
+  gbs_algo eng = algo == GBS_ALGO_AUTO ? select_for(l, r) : algo;
   if (eng == GBS_ALGO_MFMA_BITS) return launch_bits_gemm(lo, ro, K, d_res, s);
   if (eng == GBS_ALGO_MFMA_PLANES) return launch_planes_gemm(lo, ro, K, d_res, s);
   if (eng == GBS_ALGO_MFMA || eng == GBS_ALGO_MFMA_FP4 || eng == GBS_ALGO_MFMA_FP6) {
@@ -693,7 +693,7 @@ int gbs_gemm_async_noalloc(const gbs_matrix *lhs, const gbs_matrix *rhs, int32_t
     // engine whose value copies both operands already hold, else the
     // engines that read the planes themselves
     const Operand lo = view(lhs), ro = view(rhs);
-    eng = select_algo(lo, ro, lhs->ncols);
+    eng = select_for(lhs, rhs);
     const bool have =
         eng == GBS_ALGO_MFMA_FP4   ? (lhs->d_f4 && rhs->d_f4)
         : eng == GBS_ALGO_MFMA_FP6 ? (lhs->d_f6 && rhs->d_f6)
@@ -710,7 +710,7 @@ int gbs_gemm_select(const gbs_matrix *lhs, const gbs_matrix *rhs, gbs_algo *chos
   if (!rc) rc = check_matrix(rhs, "gbs_gemm_select");
   if (rc) return rc;
   if (!chosen) return fail(GBS_EINVAL, "gbs_gemm_select: null out");
-  *chosen = select_algo(view(lhs), view(rhs), lhs->ncols);
+  *chosen = select_for(lhs, rhs);
   return GBS_OK;
 }
 

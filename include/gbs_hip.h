@@ -277,7 +277,10 @@ int gbs_gemm_async_noalloc(const gbs_matrix *lhs, const gbs_matrix *rhs,
 int gbs_workspace_reserve(gbs_workspace *ws, const gbs_matrix *lhs,
                           const gbs_matrix *rhs);
 
-/* The engine GBS_ALGO_AUTO resolves to for these operands. */
+/* The engine GBS_ALGO_AUTO resolves to for these operands in their present
+ * state: a matrix-core engine on value copies where the copies are current
+ * (or the product is small), the planes engine for large products whose
+ * copies would first have to be rebuilt from freshly imported planes. */
 int gbs_gemm_select(const gbs_matrix *lhs, const gbs_matrix *rhs,
                     gbs_algo *chosen);
 

@@ -114,3 +114,31 @@ def test_planes_refuses_what_it_cannot_hold(gbs, orc):
         gbs.gemm(cod, big, "mfma_planes")
     for x in (l, r, big, cod):
         x.close()
+
+
+def test_auto_reads_planes_when_copies_are_stale(gbs, orc):
+    """AUTO on a large product whose value copies would have to be rebuilt
+    (freshly imported activations) takes the planes engine; once a copy
+    engine has made the copies it takes them; both bit-exact."""
+    from oracle import orc as O
+    rng = np.random.default_rng(77)
+    M, K, N = 256, 2304, 8192   # 2 * M * K * N = 9.7e9 ops
+    a = rng.integers(-2, 2, (M, K)).astype(np.int8)
+    b = rng.integers(0, 8, (N, K)).astype(np.int8)
+    want = O.wrap_i32(O.blas_gemm_exact(O.values_of(a, 2, True), O.values_of(b, 3, False)))
+    l, r = gbs.from_values(a, 2, True), gbs.from_values(b, 3, False)
+    assert gbs.select(l, r) == "mfma_planes"
+    assert np.array_equal(gbs.gemm(l, r, "auto"), want)
+    assert np.array_equal(gbs.gemm(l, r, "mfma"), want)       # builds the int8 copies
+    assert gbs.select(l, r) == "mfma"
+    assert np.array_equal(gbs.gemm(l, r, "auto"), want)
+    b2 = rng.integers(0, 8, (N, K)).astype(np.int8)
+    r.import_regular(b2)                                      # copies stale again
+    assert gbs.select(l, r) == "mfma_planes"
+    want2 = O.wrap_i32(O.blas_gemm_exact(O.values_of(a, 2, True), O.values_of(b2, 3, False)))
+    assert np.array_equal(gbs.gemm(l, r, "auto"), want2)
+    # small products keep the copy engines whatever the copies' state
+    ls, rs = gbs.from_values(a[:, :512], 2, True), gbs.from_values(b[:256, :512], 3, False)
+    assert gbs.select(ls, rs) != "mfma_planes"
+    for m in (l, r, ls, rs):
+        m.close()
