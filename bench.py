@@ -286,6 +286,43 @@ def run_config(gbs, O, name, runs, engines, steps, warmup, stream, ws, cpu_secs)
     return out
 
 
+def bench_host_import(gbs, O, run, log, cpu_secs):
+    """importRegular of the conv activations as the drop-in makes it: an
+    int8 matrix in host memory to device planes (copy over PCIe + pack),
+    host clock around the synchronous call; the reference's importRegular
+    (oracle/_ref, one core) on its first rows beside it."""
+    enc, a, b, lb, rb, ls, rs = run
+    rows, cols = b.shape
+    m = gbs.DeviceMatrix(rb, rows, cols, rs)
+    m.import_regular(b)
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        m.import_regular(b)
+    dt = (time.perf_counter() - t0) / reps
+    if not np.array_equal(m.planes(), O.import_regular(b, rb, rs).planes):
+        raise SystemExit("c2 host import: planes differ from the oracle's")
+    m.close()
+    out = {"workload": f"{rows} x {cols} int8 activations ({rb}-bit {'signed' if rs else 'unsigned'}) "
+                       "from pageable host memory", "ms": round(dt * 1e3, 3),
+           "host_bytes": int(b.nbytes), "gb_per_s": round(b.nbytes / dt / 1e9, 1)}
+    if cpu_secs > 0 and O.ref_available():
+        k = min(rows, 8192)
+        t0, n = time.perf_counter(), 0
+        while True:
+            O.ref_import_regular(b[:k], rb, rs)
+            n += 1
+            if time.perf_counter() - t0 > min(cpu_secs, 2.0):
+                break
+        per = (time.perf_counter() - t0) / n
+        out["cpu_reference"] = {"ms": round(per * rows / k * 1e3, 3), "cores": 1,
+                                "sample": f"first {k} of {rows} rows, time scaled by rows",
+                                "cpu": cpu_model()}
+    log(f"[bench] c2 import from host: {out['ms']} ms ({out['gb_per_s']} GB/s of int8)" +
+        (f", reference CPU {out['cpu_reference']['ms']} ms" if "cpu_reference" in out else ""))
+    return out
+
+
 def bench_im2col_pack(gbs, steps, warmup, log):
     """Conv activations of the c2 shape (16 x 256 x 56 x 56, 3 x 3, pad 1,
     3-bit unsigned) packed into bit-planes two ways, sources resident in HBM:
@@ -502,6 +539,27 @@ def bench_serving(gbs, O, rate, count, log, cpu_secs=0.0):
         sweep.append(row)
         log(f"[bench] c4: p50 {row['p50_ms']} ms, p99 {row['p99_ms']} ms, {row['tops']} TOPS, "
             f"{steps} steps")
+    # admission by memory: a second server whose budget holds 256 rows of
+    # every pool and buffer (the first one's bytes per row), all requests
+    # at once -- cohorts are cut by the row capacity, the rest wait
+    st1 = srv.stats()
+    per_row = st1["bytes_reserved"] // st1["row_capacity"]
+    tight = gbs.Server(mats, shifts, act_bits=4, max_requests=64, max_rows=64,
+                       mem_budget=per_row * 256)
+    log(f"[bench] c4: memory budget of {per_row * 256 / 1e6:.1f} MB "
+        f"({tight.stats()['row_capacity']} rows), {count} requests at once")
+    lat, res, wall, ops, hbm, steps = serve_run(gbs, tight, reqs, None, log)
+    for i in range(0, count, max(1, count // 16)):
+        if not np.array_equal(res[i][0], W.mlp_reference(reqs[i]["x"], Ws, shifts)):
+            raise SystemExit(f"c4: request {i} differs under the tight memory budget")
+    out["admission_by_memory"] = {
+        "mem_budget_bytes": int(per_row * 256), "row_capacity": tight.stats()["row_capacity"],
+        "p50_ms": round(float(np.percentile(lat, 50)), 3),
+        "p99_ms": round(float(np.percentile(lat, 99)), 3),
+        "tops": round(ops / wall / 1e12, 3), "steps": steps}
+    log(f"[bench] c4: tight budget: p50 {out['admission_by_memory']['p50_ms']} ms, "
+        f"{steps} steps")
+    tight.close()
     base, cap = sweep[0], sweep[-1]
     out.update({k: base[k] for k in ("p50_ms", "p99_ms", "tops", "steps", "wall_s")})
     out["bit_exact_sample"] = base["bit_exact_sample"]
@@ -598,6 +656,8 @@ def main():
                                             args.warmup, stream, ws, cpu_secs)}
         if name == "c2":
             results[name]["pack"] = bench_im2col_pack(gbs, min(args.steps, 20), 2, log)
+            results[name]["import_from_host"] = bench_host_import(gbs, O, runs[0], log,
+                                                                  cpu_secs)
     # headline: the binary layers through gemmBitSerial's own engine choice
     c1 = results["c1"]["runs"]
     step_ms = sum(r["engines"][r["auto"]]["us"] for r in c1) / 1e3

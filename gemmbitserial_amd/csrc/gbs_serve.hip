@@ -7,8 +7,9 @@
 // of act_bits-bit unsigned activations) wait in a FIFO.  Every step t:
 //
 //   1. admission: waiting requests join a new cohort while its rows fit the
-//      pool capacity R (set from the memory budget at creation) and the
-//      number of requests in flight stays under max_requests; the cohort's
+//      pool capacity R (set from the memory budget at creation), the
+//      number of requests in flight stays under max_requests and the
+//      cohort holds fewer than max_requests / L of them; the cohort's
 //      inputs are packed on the GPU into pool 0 (parity t), read from pinned
 //      host memory by the packing kernel itself;
 //   2. one persistent launch computes, for every cohort in flight (cohort l
@@ -285,13 +286,19 @@ int gbs_server_step(gbs_server *s, uint32_t *finished_now) {
   int32_t *h_out = s->h_out + (size_t)cur * s->out_half;
   // Between steps cohort[l] holds the requests whose next GEMM is layer l
   // (their activations sit in pool l); cohort[0] is empty.
-  // 1. admission into layer 0
+  // 1. admission into layer 0: a cohort takes at most max_requests / L
+  // requests, so that under load every layer holds one and the layers'
+  // GEMMs share each launch (a cohort that took every slot would leave the
+  // other layers idle until it finished)
   size_t in_flight = 0;
   for (uint32_t l = 1; l < L; l++) in_flight += s->cohort[l].size();
+  const size_t per_cohort = std::max<size_t>(1, s->max_requests / L);
   uint32_t n0 = 0;
   while (!s->waiting.empty()) {
     Req *r = s->waiting.front();
-    if (n0 + r->rows > s->R || in_flight + s->cohort[0].size() + 1 > s->max_requests) break;
+    if (n0 + r->rows > s->R || in_flight + s->cohort[0].size() + 1 > s->max_requests ||
+        s->cohort[0].size() >= per_cohort)
+      break;
     s->waiting.pop_front();
     r->off = n0;
     std::copy(r->x.begin(), r->x.end(), h_in + (size_t)n0 * s->h[0]);

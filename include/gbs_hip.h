@@ -395,7 +395,8 @@ int gbs_gemm_async_config(gbs_workspace *ws, const gbs_matrix *lhs,
  * layer).  A server is driven by one host thread. */
 typedef struct gbs_server gbs_server;
 typedef struct gbs_server_config {
-  uint32_t max_requests; /* requests in flight at once */
+  uint32_t max_requests; /* requests in flight at once; one step admits at
+                            most max(1, max_requests / n_layers) of them */
   uint32_t max_rows;     /* rows per request */
   uint64_t mem_budget;   /* bytes for activation pools and staging; 0 = a
                             quarter of the free HBM at creation */

@@ -136,3 +136,32 @@ def test_batch_tune_table_validation_without_device():
             for c in range(n):
                 s, p = saved[f][c]
                 g.batch_tune_set(f, c, s, p if s >= 0 else 16)
+
+
+def test_batch_tuner_table_round_trip(tmp_path):
+    """The table the batch tuner writes is the one the library loads: every
+    entry comes back, classes not written revert to the built-in rule."""
+    import gemmbitserial_amd as g
+    from gemmbitserial_amd import tune
+    n = g.batch_tune_classes()
+    saved = [[g.batch_tune_get(f, c) for c in range(n)] for f in range(3)]
+    entries = {(1, 0): (3, 16), (2, 7): (18, 32), (1, 14): (13, 8), (0, 4): (10, 16)}
+    detail = {"workload": "test", "builtin_us": 1.0, "tuned_us": 0.9,
+              "products": {k: 1 for k in entries}, "mbytes": {k: 0.1 for k in entries}}
+    path = tmp_path / "batch_tune_table.txt"
+    tune.write_batch_table(entries, detail, str(path))
+    try:
+        g.batch_tune_load(str(path))
+        for f in range(3):
+            for c in range(n):
+                assert g.batch_tune_get(f, c) == entries.get((f, c), (-1, 0)), (f, c)
+        bad = tmp_path / "bad.txt"
+        bad.write_text("0 2 13 16\n")        # 64x64 dl4 is not an FP6 shape
+        L = g.lib()
+        assert L.gbs_batch_tune_load(str(bad).encode()) == 1
+        assert b"invalid" in L.gbs_last_error()
+    finally:
+        for f in range(3):
+            for c in range(n):
+                s, p = saved[f][c]
+                g.batch_tune_set(f, c, s, p if s >= 0 else 16)

@@ -177,7 +177,7 @@ def test_serve_fused_requant_writes_planes(gbs, orc, widths):
     h = list(widths)
     Ws = [W.values(rng, (h[l + 1], h[l]), 4, True) for l in range(len(h) - 1)]
     shifts = [W.mlp_shift(h[l]) for l in range(len(h) - 2)]
-    srv, mats = make_server(gbs, h, Ws, shifts, max_requests=8, max_rows=40)
+    srv, mats = make_server(gbs, h, Ws, shifts, max_requests=24, max_rows=40)
     xs = [rng.integers(0, 16, (n, h[0])).astype(np.uint8) for n in (7, 33, 1)]
     for i, x in enumerate(xs):
         srv.submit(i, x)
@@ -225,4 +225,27 @@ def test_serve_finished_lists_ready_ids(gbs):
     assert sorted(seen) == [r["id"] for r in reqs]
     for r in reqs:
         assert np.array_equal(seen[r["id"]][0], W.mlp_reference(r["x"], Ws, shifts))
+    srv.close()
+
+
+def test_serve_cohorts_share_the_layers(gbs):
+    """Under load a step admits at most max_requests / layers requests, so
+    every layer holds a cohort: with 12 requests waiting and 4 layers x 2,
+    the first cohort has 2 of them and the next step admits the next 2."""
+    h, Ws, shifts = small_mlp(21, layers=4)
+    reqs = W.serving_requests(h[0], 12, 1000.0, seed=22, max_rows=6)
+    srv, mats = make_server(gbs, h, Ws, shifts, max_requests=8, max_rows=6)
+    for r in reqs:
+        srv.submit(r["id"], r["x"])
+    srv.step()
+    _, rows1 = srv.layer_input(1)
+    assert rows1 == sum(r["x"].shape[0] for r in reqs[:2])
+    srv.step()
+    _, rows1 = srv.layer_input(1)
+    _, rows2 = srv.layer_input(2)
+    assert rows1 == sum(r["x"].shape[0] for r in reqs[2:4])
+    assert rows2 == sum(r["x"].shape[0] for r in reqs[:2])
+    got = drain(srv, [r["id"] for r in reqs])
+    for r in reqs:
+        assert np.array_equal(got[r["id"]][0], W.mlp_reference(r["x"], Ws, shifts)), r["id"]
     srv.close()
