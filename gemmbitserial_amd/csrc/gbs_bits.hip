@@ -16,9 +16,12 @@
 // nibble) -> k assignment, so any bijection works.  Lane l of a fragment
 // holds row l % 16 and lane group g = l / 16; per 1024-bit super-step a
 // lane loads 32 bytes of its row, the 32-bit words 8g .. 8g + 7 of the
-// step, and instruction c (0..7) of the step takes word 8g + c: bit
-// 4j + d of it becomes nibble j of register d.  Instruction c therefore
-// covers words c, 8 + c, 16 + c, 24 + c: every bit of the step once.
+// step, in two halves of four words.  bits_dl_kernel: instruction c of a
+// half takes one word, bit 4j + d of it becoming nibble j of register d.
+// bits_kernel: instruction D (0..3) of a half takes bit position D of all
+// four words, bit 4j + D of word r becoming nibble j of register r, with
+// per-instruction block scales for the position's weight (step_reg).
+// Either way the instructions of a super-step cover every bit of it once.
 //
 // Padding: bits past K are zero in a plane row, and words past the row's
 // data are read as zero.  A zero bit of an unsigned operand is 0, so it
@@ -73,6 +76,46 @@ __device__ __forceinline__ v8i bits_frag(uint32_t x) {
   }
   f[4] = f[5] = f[6] = f[7] = 0;
   return f;
+}
+
+// The register-form kernel's fragments take another bijection: k-step D
+// (0..3) of a half super-step takes bit position D of all four of the
+// lane's 32-bit words, register r holding bit 4j + D of word r in nibble j.
+// Bits already at nibble positions 0, 1 and 2 of an unsigned operand are
+// FP4 codes as they stand (0.5, 1 and 2: one AND each); position 3 (the
+// sign bit) is moved to position 2 (value 2).  Each instruction's block
+// scales cancel the step's weight: the unsigned side runs under 2^1, 2^0,
+// 2^-1, 2^-1 for D = 0..3, a bipolar side (magnitude 0.5, sign where the
+// bit is 0) under 2^1 throughout, so every product of two set bits is
+// exactly 1 (unsigned x unsigned: 0.5 * 0.5 * 4, 1 * 1 * 1, 2 * 2 / 4) and
+// of two bipolar values +-1.  The unsigned side costs 4 + 4 + 4 + 8 VALU
+// operations per fragment over the four steps instead of 4 x 7.
+template <bool BIP, int D>
+__device__ __forceinline__ uint32_t step_reg(uint32_t x) {
+  if constexpr (BIP) {
+    if constexpr (D == 3) return (~x & 0x88888888u) | 0x11111111u;
+    else return (~(x << (3 - D)) & 0x88888888u) | 0x11111111u;
+  } else {
+    if constexpr (D == 3) return (x >> 1) & 0x44444444u;
+    else return x & (0x11111111u << D);
+  }
+}
+
+template <bool BIP, int D>
+__device__ __forceinline__ v8i step_frag(const uint4 &w) {
+  v8i f;
+  f[0] = (int)step_reg<BIP, D>(w.x);
+  f[1] = (int)step_reg<BIP, D>(w.y);
+  f[2] = (int)step_reg<BIP, D>(w.z);
+  f[3] = (int)step_reg<BIP, D>(w.w);
+  f[4] = f[5] = f[6] = f[7] = 0;
+  return f;
+}
+
+// E8M0 block scale of a side in step D (see step_reg)
+template <bool BIP, int D>
+constexpr int step_scale() {
+  return BIP ? 128 : (D == 0 ? 128 : D == 1 ? 127 : 126);
 }
 
 // Two 64-bit words of a row at q: one 16-byte load when the row pitch is
@@ -138,18 +181,26 @@ __global__ __launch_bounds__(256, 2) void bits_kernel(BitsProb p) {
 #pragma unroll
     for (int i = 0; i < 4; i++) xb[i][h] = bits_pair_tail(la[i] + off, w, p.kw);
   };
-  auto step = [&](int h, int c) {
+  auto step_d = [&](int h, auto Dc) {
+    constexpr int D = decltype(Dc)::value;
     v8i fa[4], fb[4];
 #pragma unroll
-    for (int i = 0; i < 4; i++) fa[i] = bits_frag<BR>(word_of(xa[i][h], c));
+    for (int i = 0; i < 4; i++) fa[i] = step_frag<BR, D>(xa[i][h]);
 #pragma unroll
-    for (int j = 0; j < 4; j++) fb[j] = bits_frag<BL>(word_of(xb[j][h], c));
+    for (int j = 0; j < 4; j++) fb[j] = step_frag<BL, D>(xb[j][h]);
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
       for (int j = 0; j < 4; j++)
         acc[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
-            fa[i], fb[j], acc[i][j], 4, 4, 0, kF4Scale, 0, kF4Scale);
+            fa[i], fb[j], acc[i][j], 4, 4, 0, step_scale<BR, D>(), 0, step_scale<BL, D>());
+  };
+  // the four k-steps of half h
+  auto half = [&](int h) {
+    step_d(h, std::integral_constant<int, 0>());
+    step_d(h, std::integral_constant<int, 1>());
+    step_d(h, std::integral_constant<int, 2>());
+    step_d(h, std::integral_constant<int, 3>());
   };
 
   // full super-steps (all 16 words hold data): wave w takes w, w + 4, ...,
@@ -162,21 +213,17 @@ __global__ __launch_bounds__(256, 2) void bits_kernel(BitsProb p) {
   }
   for (; s < nfull; s += kBitsWaves) {
     const bool more = s + kBitsWaves < nfull;
-#pragma unroll
-    for (int c = 0; c < 4; c++) step(0, c);
+    half(0);
     if (more) load_half(s + kBitsWaves, 0);
-#pragma unroll
-    for (int c = 0; c < 4; c++) step(1, c);
+    half(1);
     if (more) load_half(s + kBitsWaves, 1);
   }
   // the partly filled last super-step, if any, by the wave whose turn it is
   if (nfull < p.nss && wv == nfull % kBitsWaves) {
     load_tail(nfull, 0);
     load_tail(nfull, 1);
-#pragma unroll
-    for (int c = 0; c < 4; c++) step(0, c);
-#pragma unroll
-    for (int c = 0; c < 4; c++) step(1, c);
+    half(0);
+    half(1);
   }
 
   // the four waves' partial sums (exact integers in fp32: |sum| <= 1024 *
