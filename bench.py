@@ -474,11 +474,15 @@ def bench_batched(gbs, O, steps, warmup, stream, log, name="c3", cpu_secs=0.0):
     return out
 
 
-def serve_run(gbs, srv, reqs, speedup, log):
+def serve_run(gbs, srv, reqs, speedup, log, keep=()):
     """Feed requests at their Poisson arrival times divided by speedup (None:
     all at once, the server's capacity) and step until every one finished.
-    Returns (latencies ms, results, wall s, ops, hbm bytes, steps)."""
+    Rows are taken into one reused buffer and kept for the ids in keep.
+    Returns (latencies ms, {id: (rows or None, arrival, done)}, wall s, ops,
+    hbm bytes, steps)."""
     count = len(reqs)
+    keep = set(keep)
+    buf = np.empty((64, srv.h[-1]), np.int32)
     st0 = srv.stats()
     t0 = gbs.clock_ns() + 1_000_000
     arrivals = [t0 + (int(r["t"] / speedup * 1e9) if speedup else 0) for r in reqs]
@@ -493,9 +497,9 @@ def serve_run(gbs, srv, reqs, speedup, log):
             continue  # idle until the next arrival
         if srv.step():
             for i in srv.finished():
-                t = srv.take(i)
-                res[i] = t
-                lat.append((t[2] - t[1]) / 1e6)
+                y, arr, done = srv.take(i, out=buf)
+                res[i] = (y.copy() if i in keep else None, arr, done)
+                lat.append((done - arr) / 1e6)
                 pending.discard(i)
     t_end = max(t[2] for t in res.values())
     st = srv.stats()
@@ -526,8 +530,9 @@ def bench_serving(gbs, O, rate, count, log, cpu_secs=0.0):
     for r_ in (rate, 4 * rate, 16 * rate, None):
         log(f"[bench] c4: serving {count} requests at "
             f"{'once (capacity)' if r_ is None else f'{r_:.0f}/s'}, hidden sizes {h}")
-        lat, res, wall, ops, hbm, steps = serve_run(gbs, srv, reqs, r_ and r_ / rate, log)
         sample = list(range(0, count, max(1, count // 16)))
+        lat, res, wall, ops, hbm, steps = serve_run(gbs, srv, reqs, r_ and r_ / rate, log,
+                                                    keep=sample)
         for i in sample:
             if not np.array_equal(res[i][0], W.mlp_reference(reqs[i]["x"], Ws, shifts)):
                 raise SystemExit(f"c4: request {i} differs from the exact forward pass")
@@ -548,8 +553,9 @@ def bench_serving(gbs, O, rate, count, log, cpu_secs=0.0):
                        mem_budget=per_row * 256)
     log(f"[bench] c4: memory budget of {per_row * 256 / 1e6:.1f} MB "
         f"({tight.stats()['row_capacity']} rows), {count} requests at once")
-    lat, res, wall, ops, hbm, steps = serve_run(gbs, tight, reqs, None, log)
-    for i in range(0, count, max(1, count // 16)):
+    sample = list(range(0, count, max(1, count // 16)))
+    lat, res, wall, ops, hbm, steps = serve_run(gbs, tight, reqs, None, log, keep=sample)
+    for i in sample:
         if not np.array_equal(res[i][0], W.mlp_reference(reqs[i]["x"], Ws, shifts)):
             raise SystemExit(f"c4: request {i} differs under the tight memory budget")
     out["admission_by_memory"] = {

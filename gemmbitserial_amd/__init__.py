@@ -571,17 +571,21 @@ class Server:
         check(lib().gbs_server_step(self.s, C.byref(n)))
         return n.value
 
-    def take(self, rid, max_rows=64):
-        out = np.zeros((max_rows, self.h[-1]), np.int32)
+    def take(self, rid, max_rows=64, out=None):
+        """(rows, arrival ns, done ns) of a finished request, or None.  The
+        rows are written into out (max_rows x h_L int32) when given, else a
+        new array; the returned rows are a view of it."""
+        if out is None:
+            out = np.empty((max_rows, self.h[-1]), np.int32)
         ready, rows = C.c_int(), C.c_uint32()
         arr, done = C.c_int64(), C.c_int64()
         check(lib().gbs_server_take(self.s, rid, _ptr(out), out.size, C.byref(ready),
                                     C.byref(rows), C.byref(arr), C.byref(done)))
         if not ready.value:
             return None
-        return out[: rows.value].copy(), arr.value, done.value
+        return out[: rows.value], arr.value, done.value
 
-    def finished(self, cap=4096):
+    def finished(self, cap=1024):
         """Ids of finished requests not yet taken."""
         ids, n = (C.c_uint64 * cap)(), C.c_uint32()
         check(lib().gbs_server_finished(self.s, ids, cap, C.byref(n)))

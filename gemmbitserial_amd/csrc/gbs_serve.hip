@@ -50,7 +50,11 @@ struct Req {
   uint32_t off;  // first row in its cohort's pool
   int64_t arrival, done;
   std::vector<uint8_t> x;
+  // a finished request's rows: in the pinned output half `half` (from
+  // `rows_at`) until that half is about to be reused, then in y
   std::vector<int32_t> y;
+  const int32_t *rows_at = nullptr;
+  int half = -1;
 };
 
 constexpr int kMaxLayers = 16;
@@ -95,6 +99,7 @@ struct gbs_server {
   size_t in_half = 0, out_half = 0;  // bytes / elements per half
   hipEvent_t ev[2] = {nullptr, nullptr};        // end of the step using each half
   std::vector<Req *> completing[2];             // requests whose rows land in h_out half
+  std::vector<Req *> resident[2];               // finished, untaken, rows still in that half
   bool pending[2] = {false, false};
   uint64_t bytes_reserved = 0;
   std::deque<Req *> waiting;
@@ -284,6 +289,14 @@ int gbs_server_step(gbs_server *s, uint32_t *finished_now) {
   const int cur = (int)(s->steps & 1), prev = cur ^ 1;
   uint8_t *h_in = s->h_in + (size_t)cur * s->in_half;
   int32_t *h_out = s->h_out + (size_t)cur * s->out_half;
+  // this step's launch writes output half cur: results not yet taken from
+  // it move to their own storage first
+  for (Req *r : s->resident[cur]) {
+    r->y.assign(r->rows_at, r->rows_at + (size_t)r->rows * s->h[L]);
+    r->rows_at = nullptr;
+    r->half = -1;
+  }
+  s->resident[cur].clear();
   // Between steps cohort[l] holds the requests whose next GEMM is layer l
   // (their activations sit in pool l); cohort[0] is empty.
   // 1. admission into layer 0: a cohort takes at most max_requests / L
@@ -386,7 +399,9 @@ int gbs_server_step(gbs_server *s, uint32_t *finished_now) {
     const int64_t t = now_ns();
     const int32_t *out = s->h_out + (size_t)prev * s->out_half;
     for (Req *r : s->completing[prev]) {
-      r->y.assign(out + (size_t)r->off * s->h[L], out + (size_t)(r->off + r->rows) * s->h[L]);
+      r->rows_at = out + (size_t)r->off * s->h[L];
+      r->half = prev;
+      s->resident[prev].push_back(r);
       r->done = t;
       s->finished[r->id] = r;
       s->done_requests++;
@@ -409,11 +424,17 @@ int gbs_server_take(gbs_server *s, uint64_t id, int32_t *out, uint64_t cap_elems
     return GBS_OK;
   }
   Req *r = it->second;
+  const size_t n = (size_t)r->rows * s->h[s->L];
   if (out) {
-    if (cap_elems < r->y.size())
+    if (cap_elems < n)
       return fail(GBS_ERANGE, "gbs_server_take: output holds %llu of %zu values",
-                  (unsigned long long)cap_elems, r->y.size());
-    std::copy(r->y.begin(), r->y.end(), out);
+                  (unsigned long long)cap_elems, n);
+    if (r->rows_at) std::copy(r->rows_at, r->rows_at + n, out);
+    else std::copy(r->y.begin(), r->y.end(), out);
+  }
+  if (r->half >= 0) {
+    std::vector<Req *> &res = s->resident[r->half];
+    res.erase(std::find(res.begin(), res.end(), r));
   }
   *ready = 1;
   if (rows) *rows = r->rows;

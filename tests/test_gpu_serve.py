@@ -249,3 +249,29 @@ def test_serve_cohorts_share_the_layers(gbs):
     for r in reqs:
         assert np.array_equal(got[r["id"]][0], W.mlp_reference(r["x"], Ws, shifts)), r["id"]
     srv.close()
+
+
+def test_serve_results_survive_late_take(gbs):
+    """A finished request's rows stay valid however many steps pass before
+    take(): rows still in the pinned output staging when it is reused are
+    moved out first.  Half the requests are taken at once, half only after
+    the server has run dry for a while."""
+    h, Ws, shifts = small_mlp(31, layers=2)
+    reqs = W.serving_requests(h[0], 10, 1000.0, seed=32, max_rows=9)
+    srv, mats = make_server(gbs, h, Ws, shifts, max_requests=4, max_rows=9)
+    for r in reqs:
+        srv.submit(r["id"], r["x"])
+    got = {}
+    for _ in range(40):
+        srv.step()
+        for i in srv.finished():
+            if i % 2 == 0:
+                got[i] = srv.take(i)
+    for _ in range(5):
+        srv.step()
+    for i in srv.finished():
+        got[i] = srv.take(i)
+    assert sorted(got) == [r["id"] for r in reqs]
+    for r in reqs:
+        assert np.array_equal(got[r["id"]][0], W.mlp_reference(r["x"], Ws, shifts)), r["id"]
+    srv.close()
