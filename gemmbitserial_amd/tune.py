@@ -28,6 +28,17 @@ for bit against one launch per product and written to
 batch_tune_table.txt, which gbs_init loads.
 
     python -m gemmbitserial_amd.tune --batch [--out PATH] [--reps 10] [--passes 2]
+
+The serving loop's step launch plans its int8 products from the same
+table (format 0).  --serving tunes those entries on the serving workload
+itself (the 8-layer 4-bit MLP, Poisson requests of 1..64 rows, another
+seed than the benchmark's): for every int8 class its products fall in,
+each carried shape x K piece of 1/2, 1, 2 or 4 KiB is planned and the
+median latency at 2000 and 8000 requests/s measured; a candidate replaces
+the entry when that mean drops by at least 2 %.  The int8 lines of the
+table are rewritten, the other formats' lines kept.
+
+    python -m gemmbitserial_amd.tune --serving [--out PATH] [--passes 1]
 """
 import argparse
 import itertools
@@ -238,6 +249,95 @@ def tune_batch(reps=10, passes=2, seed=W.SEED0 + 33, log=print):
     return entries, detail
 
 
+def _serve_p50(srv, reqs, speedup, h_last):
+    """Median latency (ms) of reqs fed at their arrival times / speedup."""
+    buf = np.empty((64, h_last), np.int32)
+    t0 = gbs.clock_ns() + 1_000_000
+    arr = [t0 + int(r["t"] / speedup * 1e9) for r in reqs]
+    nxt, lat, pending = 0, [], set()
+    while len(lat) < len(reqs):
+        now = gbs.clock_ns()
+        while nxt < len(reqs) and arr[nxt] <= now:
+            srv.submit(reqs[nxt]["id"], reqs[nxt]["x"], arr[nxt])
+            pending.add(reqs[nxt]["id"])
+            nxt += 1
+        if pending and srv.step():
+            for i in srv.finished():
+                _, a, d = srv.take(i, out=buf)
+                lat.append((d - a) / 1e6)
+                pending.discard(i)
+    return float(np.median(lat))
+
+
+def tune_serving(passes=1, seed=W.SEED0 + 66, log=print):
+    """Coordinate descent over the int8 batch-table entries the serving
+    step launch uses; returns (entries, detail)."""
+    gbs.init(0)
+    h, Ws, shifts = W.serving_mlp()
+    mats = [gbs.from_values(w, 4, True) for w in Ws]
+    reqs = W.serving_requests(h[0], 300, 2000.0, seed=seed)
+    n_cls = gbs.batch_tune_classes()
+    for c in range(n_cls):
+        gbs.batch_tune_set(0, c, -1)
+
+    def measure():
+        srv = gbs.Server(mats, shifts, act_bits=4, max_requests=64, max_rows=64)
+        _serve_p50(srv, reqs[:16], 1.0, h[-1])   # warm-up
+        t = [_serve_p50(srv, reqs, sp, h[-1]) for sp in (1.0, 4.0)]
+        srv.close()
+        return sum(t) / len(t)
+
+    # classes of the serving products: a cohort of 1..64 rows per layer at
+    # these loads (rhs rows <= 64), depth h_l
+    classes = sorted({gbs.batch_tune_class_of(n, k) for n in (8, 24, 48) for k in h[:-1]})
+    base = measure()
+    log(f"serving: built-in rule {base * 1e3:.1f} us median latency")
+    cur, entries = base, {}
+    for it in range(passes):
+        for c in classes:
+            best, best_t = entries.get(c), cur
+            for sh in gbs.batch_tune_shapes(0):
+                for pc in (2, 4, 8, 16):
+                    gbs.batch_tune_set(0, c, sh, pc)
+                    t = measure()
+                    if t < best_t * 0.98:
+                        best, best_t = (sh, pc), t
+            if best is None:
+                gbs.batch_tune_set(0, c, -1)
+            else:
+                gbs.batch_tune_set(0, c, *best)
+                entries[c] = best
+            cur = best_t
+            log(f"serving pass {it}: int8 class {c:2d} -> "
+                f"{SHAPES[best[0]] + ' piece ' + str(best[1]) if best else 'built-in'}: "
+                f"{cur * 1e3:.1f} us")
+    final = measure()
+    for m in mats:
+        m.close()
+    return entries, {"builtin_ms": round(base, 4), "tuned_ms": round(final, 4),
+                     "workload": f"serving_mlp(), 300 requests (seed {seed:#x}) at 2000 "
+                                 "and 8000 requests/s, mean of the two median latencies"}
+
+
+def write_serving_entries(entries, detail, path):
+    """Rewrite the int8 (format 0) lines of a batch table, keeping the rest."""
+    keep = []
+    if os.path.exists(path):
+        for ln in open(path):
+            body = ln.split("#", 1)[0].split()
+            if ln.startswith("# serving") or (len(body) >= 2 and body[1] == "0"):
+                continue
+            keep.append(ln.rstrip("\n"))
+    with open(path, "w") as f:
+        for ln in keep:
+            f.write(ln + "\n")
+        f.write(f"# serving (int8) entries, measured by gemmbitserial_amd.tune --serving on "
+                f"{detail['workload']}: {detail['tuned_ms']} ms (built-in rule "
+                f"{detail['builtin_ms']} ms)\n")
+        for c, (sh, pc) in sorted(entries.items()):
+            f.write(f"{c} 0 {sh} {pc}  # serving step launch\n")
+
+
 def write_batch_table(entries, detail, path):
     with open(path, "w") as f:
         f.write("# persistent-launch tile shape and K piece per batch class and copy format, "
@@ -259,8 +359,18 @@ def main():
     ap.add_argument("--out", default=None)
     ap.add_argument("--reps", type=int, default=10)
     ap.add_argument("--batch", action="store_true", help="tune the persistent launch's table")
+    ap.add_argument("--serving", action="store_true",
+                    help="tune the int8 entries the serving step launch uses")
     ap.add_argument("--passes", type=int, default=2)
     a = ap.parse_args()
+    if a.serving:
+        entries, detail = tune_serving(a.passes if a.passes else 1,
+                                       log=lambda *x: print(*x, file=sys.stderr, flush=True))
+        out = a.out or DEFAULT_BATCH_OUT
+        write_serving_entries(entries, detail, out)
+        print(json.dumps({"builtin_ms": detail["builtin_ms"], "tuned_ms": detail["tuned_ms"],
+                          "entries": {str(k): v for k, v in entries.items()}, "out": out}))
+        return
     if a.batch:
         entries, detail = tune_batch(a.reps, a.passes,
                                      log=lambda *x: print(*x, file=sys.stderr, flush=True))

@@ -165,3 +165,28 @@ def test_batch_tuner_table_round_trip(tmp_path):
             for c in range(n):
                 s, p = saved[f][c]
                 g.batch_tune_set(f, c, s, p if s >= 0 else 16)
+
+
+def test_serving_tuner_rewrites_only_int8_entries(tmp_path):
+    """tune --serving rewrites the int8 lines of the batch table and keeps
+    every other format's line; the result loads."""
+    import gemmbitserial_amd as g
+    from gemmbitserial_amd import tune
+    path = tmp_path / "t.txt"
+    path.write_text("# header\n3 1 3 16  # fp4\n4 2 18 16  # fp6\n1 0 2 16  # old int8\n"
+                    "# serving (int8) entries, old\n2 0 3 8  # old\n")
+    tune.write_serving_entries({1: (13, 4), 2: (13, 2)}, {"workload": "w", "tuned_ms": 0.1,
+                                                          "builtin_ms": 0.2}, str(path))
+    lines = [ln.split("#")[0].split() for ln in path.read_text().splitlines()]
+    rows = [tuple(int(x) for x in ln) for ln in lines if ln]
+    assert sorted(rows) == sorted([(3, 1, 3, 16), (4, 2, 18, 16), (1, 0, 13, 4), (2, 0, 13, 2)])
+    n = g.batch_tune_classes()
+    saved = [[g.batch_tune_get(f, c) for c in range(n)] for f in range(3)]
+    try:
+        g.batch_tune_load(str(path))
+        assert g.batch_tune_get(0, 1) == (13, 4) and g.batch_tune_get(1, 3) == (3, 16)
+    finally:
+        for f in range(3):
+            for c in range(n):
+                s, p = saved[f][c]
+                g.batch_tune_set(f, c, s, p if s >= 0 else 16)
