@@ -310,7 +310,10 @@ def test_bits_engine_every_1bit_encoding(gbs, orc, ls, rs):
     rng = np.random.default_rng(1300 + 2 * ls + rs)
     shapes = [(1, 1, 1), (3, 6, 9), (17, 63, 1), (64, 64, 64), (65, 65, 63),
               (130, 1000, 67), (64, 1024, 200), (200, 1025, 64), (129, 4095, 130),
-              (257, 4097, 70), (100, 8192, 33), (70, 20000, 150)]
+              (257, 4097, 70), (100, 8192, 33), (70, 20000, 150),
+              # tile counts divisible by 8 (the XCD-grouped tile order) and
+              # three super-steps (the register form's first depths)
+              (256, 2500, 128), (512, 3072, 64), (130, 2049, 520), (1024, 2112, 512)]
     for (M, K, N) in shapes:
         say("bits", ls, rs, M, K, N)
         lhs = rand_values(rng, (M, K), 1, ls)
