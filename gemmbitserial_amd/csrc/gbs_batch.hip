@@ -4,18 +4,21 @@
 // Plan (gbs_batch_create): every product that the matrix-core engine holds
 // exactly becomes a run of work items in the narrowest copy format both
 // operands fit (FP4, else FP6, else int8: inside one launch an item's time
-// follows the bytes of K it streams) -- output tiles of a shape picked by
-// the rhs row count (16 x 256 for N <= 16, 32 x 256 for N <= 32, else
-// 64 x 64: the least padding, and no grid to fill), times K splits (deep
-// products are cut into pieces of at most 4 KiB of nominal K, whose
-// partial sums are added with integer atomics into results zeroed first;
-// measured: 2 KiB pieces lose 6 % on the 512-product workload to pipeline
-// fill per item, 8 KiB pieces leave too few items for GEMV-heavy lists).  Products are ordered by
-// the cost of one item, largest first, so the long items start early and
-// the short ones fill the tail.  A device array holds one descriptor per product and
-// the prefix sum of tile counts.  Products the MFMA engine cannot hold
-// exactly (unsigned 8-bit codes, very deep K) go to the popcount engine as
-// separate launches after the persistent kernel.
+// follows the bytes of K it streams) -- output tiles times K pieces, whose
+// partial sums are added with integer atomics into results zeroed first.
+// Tile shape and largest K piece come from the batch tile table per batch
+// class (rhs-row bucket x depth bucket x format; batch_plan), measured on
+// whole persistent launches by the tuner; without an entry the built-in
+// rule: the tile with the least padding for the rhs row count (16 x 256
+// for N <= 16, 32 x 256 for N <= 32, else 64 x 64) and pieces of at most
+// 4 KiB of nominal K (measured: 2 KiB pieces lose 6 % on the 512-product
+// workload to pipeline fill per item, 8 KiB pieces leave too few items for
+// GEMV-heavy lists).  Products are ordered by the cost of one item, largest
+// first, so the long items start early and the short ones fill the tail.
+// A device array holds one descriptor per product and the prefix sum of
+// tile counts.  Products the MFMA engine cannot hold exactly (unsigned
+// 8-bit codes, very deep K) go to the popcount engine as separate launches
+// after the persistent kernel.
 //
 // Products are grouped by copy format (int8, FP4, FP6); each group has its
 // own descriptor range, tile prefix sums and queue counter, and its own
