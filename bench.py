@@ -578,7 +578,10 @@ def bench_serving(gbs, O, rate, count, log, cpu_secs=0.0):
     # layer) over the wall time of the all-at-once run
     out["roofline"] = {"bound": "hbm", "achieved": cap["hbm_gbs"], "peak": HBM_PEAK / 1e9,
                        "unit": "GB/s", "frac": round(cap["hbm_gbs"] * 1e9 / HBM_PEAK, 4),
-                       "at": "capacity (all requests at once), wall clock"}
+                       "at": "capacity (all requests at once), wall clock",
+                       "mfma_int8_frac": round(cap["tops"] * 1e12 / MFMA_I8_PEAK, 4),
+                       "note": "the step launches run int8 matrix-core GEMMs; at capacity "
+                               "the loop is paced by host work per step, not by either roof"}
     if cpu_secs > 0 and O.ref_available():
         # the reference forward pass of sample requests: per layer one
         # gemmBitSerial (weights 4-bit signed x activations 4-bit unsigned),
