@@ -17,9 +17,12 @@
 //               a p-plane operand costs ceil(p / 2) instructions instead of
 //               p (one binary product per plane) or a copy of its values.
 //   code side   the operand whose whole values fit an FP4 code (bipolar,
-//               <= 2 bits unsigned, <= 3 bits signed: |v| <= 4).  The block
-//               turns its plane words into FP4 codes once per half
-//               super-step, into LDS, and every wave reads them from there.
+//               <= 2 bits unsigned, <= 3 bits signed: |v| <= 4) -- or, when
+//               neither operand does, the one with fewer rows taken as two
+//               2-bit chunks like the plane side (3-4 bits unsigned, 4 bits
+//               signed; CP x 2 instructions per k-step).  The block turns
+//               its plane words into FP4 codes once per half super-step,
+//               into LDS, and every wave reads them from there.
 // Both use the block scale of v_mfma_scale_f32_16x16x128_f8f6f4 for the
 // chunk weight: FP4 code c (c = 0..4 -> 0, 0.5, 1, 1.5, 2) under scale
 // 2^(1 + p) is the integer c * 2^p, so chunk c of the plane side (planes
@@ -54,6 +57,10 @@ namespace {
 
 // code-side encodings: FP4 codes of whole values
 enum : int { kCodeU1 = 0, kCodeBip = 1, kCodeU2 = 2, kCodeS2 = 3, kCodeS3 = 4 };
+// code-side encodings of two chunks (values that do not fit one FP4 code):
+// chunk 0 = planes 0, 1 (unsigned [0, 3]), chunk 1 = plane 2 (U3: [0, 1]),
+// planes 2, 3 (U4: [0, 3]) or planes 2, 3 signed (S4: [-2, 1])
+enum : int { kCodeU3 = 5, kCodeU4 = 6, kCodeS4 = 7 };
 
 struct PlanesProb {
   const uint64_t *P, *V;   // plane 0 of the plane side / code side
@@ -106,6 +113,15 @@ __device__ __forceinline__ uint32_t chunk_reg(uint32_t lo, uint32_t hi) {
   }
 }
 
+// Register D of the top chunk of a two-chunk code-side value, unsigned
+// (sg = 0: the chunk_reg KIND 2 codes) or signed (sg = ~0: KIND 3) from one
+// formula, so the kernel carries no branch or select between the two.
+template <int D>
+__device__ __forceinline__ uint32_t top_chunk_reg(uint32_t lo, uint32_t hi, uint32_t sg) {
+  const uint32_t m0 = nib<D>(lo), h = nib<D>(hi), s = h & sg;
+  return m0 | (((h & ~sg) | (s & ~m0)) << 1) | (s << 3);
+}
+
 template <int KIND>
 __device__ __forceinline__ v8i chunk_frag(uint32_t lo, uint32_t hi) {
   v8i f;
@@ -146,8 +162,10 @@ __host__ __device__ inline CodeMasks code_masks(int enc) {
   }
 }
 
-// PSIGNED with PP == 1: bipolar.  NW waves per block.
-template <bool PLANE_RHS, int PP, bool PSIGNED, int NW, int WPG, int VG>
+// PSIGNED with PP == 1: bipolar.  NW waves per block.  CC: chunks of the
+// code side (1: whole values as FP4 codes; 2: two 2-bit chunks, each an
+// FP4 code under its own block scale, CP x 2 instructions per k-step).
+template <bool PLANE_RHS, int PP, bool PSIGNED, int NW, int WPG, int VG, int CC = 1>
 __global__ __launch_bounds__(NW * 64, 8 / NW) void planes_kernel(PlanesProb p) {
   constexpr int NT = NW * 64;                  // threads
   constexpr int CP = (PP + 1) / 2;             // chunks of the plane side
@@ -157,7 +175,7 @@ __global__ __launch_bounds__(NW * 64, 8 / NW) void planes_kernel(PlanesProb p) {
   constexpr bool ALL_TASKS = TASKS * NT == BV * 4;
   constexpr int BUF = 4 * BV * 64;             // bytes of one LDS buffer: 4 k-steps x BV rows
   constexpr bool BIP = PSIGNED && PP == 1;
-  __shared__ __attribute__((aligned(16))) uint8_t lds[2 * BUF];
+  __shared__ __attribute__((aligned(16))) uint8_t lds[2 * CC * BUF];  // [half][chunk]
 
   const uint32_t T = gridDim.x, b = blockIdx.x;
   // consecutive logical tiles (the code-side tiles of one plane-side tile)
@@ -201,8 +219,13 @@ __global__ __launch_bounds__(NW * 64, 8 / NW) void planes_kernel(PlanesProb p) {
     const uint32_t r = min(v0 + (uint32_t)(16 * vv + vf), p.NV - 1);
     vrow[i] = r * (uint32_t)p.v_wpr * 8u;
   }
-  const int npv = p.v_enc == kCodeS3 ? 3 : (p.v_enc == kCodeU2 || p.v_enc == kCodeS2) ? 2 : 1;
+  const int npv = p.v_enc == kCodeU4 || p.v_enc == kCodeS4 ? 4
+                  : p.v_enc == kCodeS3 || p.v_enc == kCodeU3 ? 3
+                  : (p.v_enc == kCodeU2 || p.v_enc == kCodeS2) ? 2 : 1;
   const CodeMasks cm = code_masks(p.v_enc);
+  // two-chunk code sides: plane 3 is absent for U3, the sign plane for S4
+  const uint32_t hi_mask = p.v_enc == kCodeU3 ? 0u : ~0u;
+  const uint32_t sg_mask = p.v_enc == kCodeS4 ? ~0u : 0u;
 
   v4f acc[WPG][VG];
 #pragma unroll
@@ -211,7 +234,8 @@ __global__ __launch_bounds__(NW * 64, 8 / NW) void planes_kernel(PlanesProb p) {
     for (int j = 0; j < VG; j++) acc[i][j] = v4f{0.f, 0.f, 0.f, 0.f};
 
   uint4 xp[WPG][PP][2];     // plane words: [row group][plane][half]
-  uint4 xv[TASKS][3];       // code-side plane words of the next half
+  constexpr int NPV = CC == 2 ? 4 : 3;  // code-side planes loaded
+  uint4 xv[TASKS][NPV];     // code-side plane words of the next half
 
   // K is walked in stages of four k-steps (half super-steps).  In a full
   // super-step lane group g covers dwords 8g .. 8g + 7 and k-step 4H + c
@@ -257,7 +281,7 @@ __global__ __launch_bounds__(NW * 64, 8 / NW) void planes_kernel(PlanesProb p) {
 #pragma unroll
     for (int u = 0; u < TASKS; u++)
 #pragma unroll
-      for (int q = 0; q < 3; q++)
+      for (int q = 0; q < NPV; q++)
         xv[u][q] = ld128(rv, vrow[u] + lane_off(j, vgrp[u]),
                          (uint32_t)min(q, npv - 1) * (uint32_t)p.v_wpp * 8u + so);
   };
@@ -272,15 +296,28 @@ __global__ __launch_bounds__(NW * 64, 8 / NW) void planes_kernel(PlanesProb p) {
 #pragma unroll
     for (int u = 0; u < TASKS; u++) {
       if (!vtask[u]) continue;  // uniform: whole waves
-      {
-        const uint32_t valid =
-            (!tail || 16u * B + 4u * (uint32_t)vgrp[u] + c < tail_dw) ? ~0u : 0u;
+      const uint32_t valid =
+          (!tail || 16u * B + 4u * (uint32_t)vgrp[u] + c < tail_dw) ? ~0u : 0u;
+      if constexpr (CC == 1) {
         const uint32_t b0 = (dword_of(xv[u][0], c) | cm.or0) & valid;
         const uint32_t b1 = ((dword_of(xv[u][1], c) & cm.and1) ^ cm.xor1) & valid;
         const uint32_t sg = ((dword_of(xv[u][2], c) & cm.and2) ^ cm.xor2) & valid;
         const uint4 codes = make_uint4(code_reg<0>(b0, b1, sg), code_reg<1>(b0, b1, sg),
                                        code_reg<2>(b0, b1, sg), code_reg<3>(b0, b1, sg));
         *reinterpret_cast<uint4 *>(lds + B * BUF + (c * VG * 64 + vunit[u]) * 16) = codes;
+      } else {
+        // chunk 0: planes 0, 1; chunk 1: plane 2 (U3), planes 2, 3 (U4), or
+        // planes 2, 3 signed (S4); the same chunk codes as the plane side
+        const uint32_t w0 = dword_of(xv[u][0], c) & valid, w1 = dword_of(xv[u][1], c) & valid;
+        const uint32_t w2 = dword_of(xv[u][2], c) & valid;
+        const uint32_t w3 = dword_of(xv[u][3], c) & valid & hi_mask;
+        const uint4 c0 = make_uint4(chunk_reg<0, 2>(w0, w1), chunk_reg<1, 2>(w0, w1),
+                                    chunk_reg<2, 2>(w0, w1), chunk_reg<3, 2>(w0, w1));
+        const uint4 c1 = make_uint4(top_chunk_reg<0>(w2, w3, sg_mask), top_chunk_reg<1>(w2, w3, sg_mask),
+                                    top_chunk_reg<2>(w2, w3, sg_mask), top_chunk_reg<3>(w2, w3, sg_mask));
+        const int at = (c * VG * 64 + vunit[u]) * 16;
+        *reinterpret_cast<uint4 *>(lds + (B * 2 + 0) * BUF + at) = c0;
+        *reinterpret_cast<uint4 *>(lds + (B * 2 + 1) * BUF + at) = c1;
       }
     }
   };
@@ -310,22 +347,26 @@ __global__ __launch_bounds__(NW * 64, 8 / NW) void planes_kernel(PlanesProb p) {
       }
     }
 #pragma unroll
-    for (int v = 0; v < VG; v++) {
-      const uint4 cw = *reinterpret_cast<const uint4 *>(
-          lds + H * BUF + (((c * VG + v) * 4 + g) * 16 + fr) * 16);
-      const v8i cf = {(int)cw.x, (int)cw.y, (int)cw.z, (int)cw.w, 0, 0, 0, 0};
+    for (int cc = 0; cc < CC; cc++) {
 #pragma unroll
-      for (int rg = 0; rg < WPG; rg++)
+      for (int v = 0; v < VG; v++) {
+        const uint4 cw = *reinterpret_cast<const uint4 *>(
+            lds + (H * CC + cc) * BUF + (((c * VG + v) * 4 + g) * 16 + fr) * 16);
+        const v8i cf = {(int)cw.x, (int)cw.y, (int)cw.z, (int)cw.w, 0, 0, 0, 0};
+        const int cs = kF4Scale + 2 * cc;  // code chunk cc: 2^(1 + 2 cc)
 #pragma unroll
-        for (int ch = 0; ch < CP; ch++) {
-          const int ps = kF4Scale + 2 * ch;  // 2^(1 + 2 ch)
-          if constexpr (PLANE_RHS)
-            acc[rg][v] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
-                pf[rg][ch], cf, acc[rg][v], 4, 4, 0, ps, 0, kF4Scale);
-          else
-            acc[rg][v] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
-                cf, pf[rg][ch], acc[rg][v], 4, 4, 0, kF4Scale, 0, ps);
-        }
+        for (int rg = 0; rg < WPG; rg++)
+#pragma unroll
+          for (int ch = 0; ch < CP; ch++) {
+            const int ps = kF4Scale + 2 * ch;  // 2^(1 + 2 ch)
+            if constexpr (PLANE_RHS)
+              acc[rg][v] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
+                  pf[rg][ch], cf, acc[rg][v], 4, 4, 0, ps, 0, cs);
+            else
+              acc[rg][v] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
+                  cf, pf[rg][ch], acc[rg][v], 4, 4, 0, cs, 0, ps);
+          }
+      }
     }
   };
 
@@ -398,11 +439,17 @@ __global__ __launch_bounds__(NW * 64, 8 / NW) void planes_kernel(PlanesProb p) {
 }
 
 // Tile configurations: waves per block, plane-side row groups per wave,
-// code-side row groups per block.
+// code-side row groups per block.  Two-chunk code sides take tile 1 only
+// (its LDS holds both chunks; the 8-wave forms spill registers).
 struct PlanesTile { int NW, WPG, VG; };
 constexpr int kPlanesTiles = 5;
 constexpr PlanesTile kPlanesTile[kPlanesTiles] = {
     {4, 2, 8}, {4, 2, 4}, {4, 2, 2}, {8, 2, 8}, {8, 2, 16}};
+
+template <bool PR, int PP, bool PS>
+void run_pp2(const PlanesProb &p, uint32_t grid, hipStream_t s) {
+  hipLaunchKernelGGL((planes_kernel<PR, PP, PS, 4, 2, 4, 2>), dim3(grid), dim3(256), 0, s, p);
+}
 
 template <bool PR, int PP, bool PS>
 void run_pp(int tile, const PlanesProb &p, uint32_t grid, hipStream_t s) {
@@ -426,7 +473,21 @@ void run_pp(int tile, const PlanesProb &p, uint32_t grid, hipStream_t s) {
 }
 
 template <bool PR>
-void run_planes(int pp, bool ps, int tile, const PlanesProb &p, uint32_t grid, hipStream_t s) {
+void run_planes(int pp, bool ps, int cc, int tile, const PlanesProb &p, uint32_t grid,
+                hipStream_t s) {
+  if (cc == 2) {
+    switch (pp * 2 + (ps ? 1 : 0)) {
+      case 2: run_pp2<PR, 1, false>(p, grid, s); break;
+      case 3: run_pp2<PR, 1, true>(p, grid, s); break;
+      case 4: run_pp2<PR, 2, false>(p, grid, s); break;
+      case 5: run_pp2<PR, 2, true>(p, grid, s); break;
+      case 6: run_pp2<PR, 3, false>(p, grid, s); break;
+      case 7: run_pp2<PR, 3, true>(p, grid, s); break;
+      case 8: run_pp2<PR, 4, false>(p, grid, s); break;
+      default: run_pp2<PR, 4, true>(p, grid, s); break;
+    }
+    return;
+  }
   switch (pp * 2 + (ps ? 1 : 0)) {
     case 2: run_pp<PR, 1, false>(tile, p, grid, s); break;
     case 3: run_pp<PR, 1, true>(tile, p, grid, s); break;
@@ -446,8 +507,20 @@ int code_enc(const Operand &o) {
   return o.nbits == 2 ? kCodeS2 : o.nbits == 3 ? kCodeS3 : -1;
 }
 
+// Two-chunk encoding of a code-side operand that does not fit one FP4
+// code, -1 when it has more than 4 planes (or is bipolar / fits one code)
+int code_enc2(const Operand &o) {
+  if (o.bipolar) return -1;
+  if (!o.issigned) return o.nbits == 3 ? kCodeU3 : o.nbits == 4 ? kCodeU4 : -1;
+  return o.nbits == 4 ? kCodeS4 : -1;
+}
+
+// Largest magnitude a code-side value's codes add up to (for two chunks,
+// the sum of |chunk maxima| x chunk weight: every partial sum is bounded by
+// K times it times the plane side's)
 double code_max(int enc) {
-  return enc == kCodeU2 ? 3 : enc == kCodeS2 ? 2 : enc == kCodeS3 ? 4 : 1;
+  return enc == kCodeU2 ? 3 : enc == kCodeS2 ? 2 : enc == kCodeS3 ? 4
+         : enc == kCodeU3 ? 7 : enc == kCodeU4 ? 15 : enc == kCodeS4 ? 11 : 1;
 }
 
 // Largest sum of |chunk maxima| * chunk weight of a plane-side operand:
@@ -463,7 +536,7 @@ double plane_max(const Operand &o) {
 struct PlanesPlan {
   bool ok;
   bool plane_rhs;      // rhs is the plane side
-  int enc, pp, tile;
+  int enc, pp, cc, tile;
   double bound;        // K * maxP * maxV
 };
 
@@ -471,9 +544,16 @@ PlanesPlan planes_plan(const Operand &l, const Operand &r, uint64_t K) {
   PlanesPlan pl{};
   pl.ok = false;
   if (l.bipolar && r.bipolar) return pl;  // the 1-bit engine's product
-  const int el = code_enc(l), er = code_enc(r);
+  int el = code_enc(l), er = code_enc(r);
   // code side: the operand that fits FP4 codes; of two that do, the one
-  // with fewer rows (the plane side is the streamed one)
+  // with fewer rows (the plane side is the streamed one); else an operand
+  // of 3-4 bits as two chunks, again the one with fewer rows
+  pl.cc = 1;
+  if (el < 0 && er < 0) {
+    el = code_enc2(l);
+    er = code_enc2(r);
+    pl.cc = 2;
+  }
   bool plane_rhs;
   if (el >= 0 && er >= 0) plane_rhs = r.nrows >= l.nrows;
   else if (el >= 0) plane_rhs = true;
@@ -493,6 +573,7 @@ PlanesPlan planes_plan(const Operand &l, const Operand &r, uint64_t K) {
   const uint64_t nv = cs.nrows;
   pl.tile = nv > 64 ? (ps.nrows >= 32768 ? 3 : 0) : nv > 32 ? 1 : 2;
   if (const char *e = getenv("GBS_PLANES_TILE")) pl.tile = atoi(e) % kPlanesTiles;
+  if (pl.cc == 2) pl.tile = 1;
   pl.ok = pl.bound < 16777216.0;
   return pl;
 }
@@ -543,8 +624,8 @@ int launch_planes_gemm(const Operand &l, const Operand &r, uint64_t K, int32_t *
   const uint64_t tiles = (uint64_t)p.vtiles * ((p.NP + bp - 1) / bp);
   if (tiles > 0x7fffffffull) return fail(GBS_ERANGE, "gemm: too many tiles");
   const bool psigned = ps.issigned;  // with one plane: bipolar
-  if (pl.plane_rhs) run_planes<true>(pl.pp, psigned, pl.tile, p, (uint32_t)tiles, s);
-  else run_planes<false>(pl.pp, psigned, pl.tile, p, (uint32_t)tiles, s);
+  if (pl.plane_rhs) run_planes<true>(pl.pp, psigned, pl.cc, pl.tile, p, (uint32_t)tiles, s);
+  else run_planes<false>(pl.pp, psigned, pl.cc, pl.tile, p, (uint32_t)tiles, s);
   GBS_HIP(hipGetLastError());
   return GBS_OK;
 }

@@ -78,10 +78,10 @@ typedef enum gbs_algo {
                             (no value copies), v_mfma_scale_f32_*_f8f6f4 */
   GBS_ALGO_MFMA_PLANES = 6 /* the bit-planes themselves (no value copies):
                             one operand of <= 4 planes taken two planes at a
-                            time as FP4 codes of 2-bit chunks, the other
-                            (bipolar, <= 2 bits unsigned, <= 3 bits signed)
-                            as FP4 codes of its whole values;
-                            v_mfma_scale_f32_*_f8f6f4 */
+                            time as FP4 codes of 2-bit chunks, the other as
+                            FP4 codes of its whole values (bipolar, <= 2 bits
+                            unsigned, <= 3 bits signed) or, of 3-4 bits, of
+                            two 2-bit chunks; v_mfma_scale_f32_*_f8f6f4 */
 } gbs_algo;
 
 typedef struct gbs_matrix gbs_matrix;       /* device-resident bit-serial matrix */

@@ -5,9 +5,11 @@ compared bit for bit with the oracle's gemm (oracle/gbs_oracle.c, pinned to
 the compiled reference in tests/test_oracle.py) on the oracle's own packing
 of the same values: every plane-side width 1..4 (unsigned, signed,
 bipolar) against every code-side encoding that fits an FP4 code (1-bit,
-bipolar, 2-bit unsigned, 2- and 3-bit signed), in both orientations, at
-ragged shapes, depths with partly filled super-steps and odd row pitches,
-each of the compiled tile shapes, and sums at 99 % of the 2^24 bound."""
+bipolar, 2-bit unsigned, 2- and 3-bit signed) and every one taken as two
+2-bit chunks (3- and 4-bit unsigned, 4-bit signed), in both orientations,
+at ragged shapes, depths with partly filled super-steps and odd row
+pitches, each of the compiled tile shapes, and sums at 99 % of the 2^24
+bound."""
 import itertools
 
 import numpy as np
@@ -20,6 +22,7 @@ pytestmark = pytest.mark.gpu
 
 PLANE_ENC = [(b, s) for b in (1, 2, 3, 4) for s in (False, True)]   # (1, True) = bipolar
 CODE_ENC = [(1, False), (1, True), (2, False), (2, True), (3, True)]
+CODE_ENC2 = [(3, False), (4, False), (4, True)]   # two 2-bit chunks
 
 
 def _check(gbs, a, b, lb, ls, rb, rs, ncols_a=None):
@@ -101,10 +104,60 @@ def test_planes_sums_near_the_bound(gbs, orc):
     assert abs(int(want.astype(np.int64).min())) > 0.97 * (1 << 24)
 
 
+@pytest.mark.parametrize("plane,code", list(itertools.product(PLANE_ENC, CODE_ENC2)))
+@pytest.mark.parametrize("code_is_lhs", [True, False])
+def test_planes_two_chunk_code_side(gbs, orc, plane, code, code_is_lhs):
+    """Operands neither of which fits one FP4 code: the one with fewer rows
+    is taken as two 2-bit chunks (4 instructions per k-step against a
+    two-chunk plane side), the other streamed as planes."""
+    pb, ps = plane
+    cb, cs = code
+    if pb <= 2 or (pb == 3 and ps):
+        # the plane operand fits one FP4 code: the engine makes it the code side
+        pb, ps = (4, ps) if pb <= 2 else (4, True)
+    rng = np.random.default_rng(pb * 100 + ps * 10 + cb * 3 + cs + 2000 * code_is_lhs)
+    for NP, NC, K in [(300, 70, 2304), (129, 17, 1000), (200, 64, 64), (96, 80, 3000)]:
+        P = rand_values(rng, (NP, K), pb, ps)
+        Cv = rand_values(rng, (NC, K), cb, cs)
+        if code_is_lhs:
+            got, want = _check(gbs, Cv, P, cb, cs, pb, ps)
+        else:
+            got, want = _check(gbs, P, Cv, pb, ps, cb, cs)
+        assert np.array_equal(got, want), (plane, code, code_is_lhs, NP, NC, K)
+
+
+@pytest.mark.parametrize("K", [1, 65, 1024, 1025, 2047, 4160])
+def test_planes_two_chunk_depths(gbs, orc, K):
+    """4-bit signed weights x 4-bit unsigned activations (the conv product's
+    4 x 4 form) across depth boundaries and an odd row pitch."""
+    rng = np.random.default_rng(K + 17)
+    a = rand_values(rng, (70, K), 4, True)
+    b = rand_values(rng, (333, K), 4, False)
+    got, want = _check(gbs, a, b, 4, True, 4, False)
+    assert np.array_equal(got, want), K
+    if K == 1025:
+        got, want = _check(gbs, a, b, 4, True, 4, False, ncols_a=1088)
+        assert np.array_equal(got, want)
+
+
+def test_planes_two_chunk_sums_near_the_bound(gbs, orc):
+    """Code side 4-bit unsigned (chunk maxima 3 + 4 x 3 = 15) against plane
+    side 4-bit unsigned all 15: |sum| = K * 225 at 99 % of 2^24."""
+    K = 73600
+    assert K * 15 * 15 < (1 << 24) and K * 15 * 15 > 0.98 * (1 << 24)
+    a = np.full((20, K), 15, np.int8)
+    a[:, ::991] = 13
+    b = np.full((40, K), 15, np.int8)
+    b[5, ::7] = 14
+    got, want = _check(gbs, a, b, 4, False, 4, False)
+    assert np.array_equal(got, want)
+    assert int(want.astype(np.int64).max()) > 0.97 * (1 << 24)
+
+
 def test_planes_refuses_what_it_cannot_hold(gbs, orc):
     rng = np.random.default_rng(2)
-    l = gbs.from_values(rand_values(rng, (32, 64), 4, True), 4, True)    # no FP4 code side
-    r = gbs.from_values(rand_values(rng, (32, 64), 3, False), 3, False)
+    l = gbs.from_values(rand_values(rng, (32, 64), 5, True), 5, True)    # neither side fits
+    r = gbs.from_values(rand_values(rng, (32, 64), 5, False), 5, False)
     with pytest.raises(gbs.GbsError) as e:
         gbs.gemm(l, r, "mfma_planes")
     assert e.value.code == 5
