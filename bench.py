@@ -430,19 +430,72 @@ def bench_batched(gbs, O, steps, warmup, stream, log, name="c3", cpu_secs=0.0):
     sec = out["persistent"]["ms"] * 1e-3
     peak_of = {0: MFMA_I8_PEAK, 1: MFMA_F4_PEAK, 2: MFMA_F4_PEAK}
     plans = [b.plan_of(i) for i in range(len(probs))]
-    mfma_floor = sum(2.0 * p["M"] * p["K"] * p["N"] / peak_of[pl[0]]
-                     for p, pl in zip(probs, plans) if pl[0] >= 0)
+
+    def floor_s(p, grp):
+        ops1 = 2.0 * p["M"] * p["K"] * p["N"]
+        if grp == gbs.BATCH_PLANES:
+            # one FP4 instruction per 2-bit chunk pair: ceil(lb / 2) plane-side
+            # chunks x one or two code-side chunks
+            cc = 2 if (p["rb"] >= 3 and not p["rs"]) or p["rb"] == 4 else 1
+            return ops1 * ((p["lb"] + 1) // 2) * cc / MFMA_F4_PEAK
+        return ops1 / peak_of[grp]
+    mfma_floor = sum(floor_s(p, pl[0]) for p, pl in zip(probs, plans) if pl[0] >= 0)
     out["roofline"] = {"bound": "hbm", "achieved": round(alg / sec / 1e9, 1),
                        "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                        "frac": round(alg / sec / HBM_PEAK, 4), "algorithmic_bytes": alg,
                        "hbm_floor_us": round(alg / HBM_PEAK * 1e6, 1),
                        "mfma_floor_us": round(mfma_floor * 1e6, 1),
                        "traffic": measured_traffic(cname)}
-    tabled = sum(1 for p, pl in zip(probs, plans) if pl[0] >= 0 and
-                 gbs.batch_tune_get(pl[0], gbs.batch_tune_class_of(p["N"], p["K"]))[0] == pl[1])
-    out["tiles_from_table"] = f"{tabled} of {sum(1 for pl in plans if pl[0] >= 0)} products"
+    copy_plans = [(p, pl) for p, pl in zip(probs, plans) if pl[0] in (0, 1, 2)]
+    tabled = sum(1 for p, pl in copy_plans
+                 if gbs.batch_tune_get(pl[0], gbs.batch_tune_class_of(p["N"], p["K"]))[0] == pl[1])
+    out["tiles_from_table"] = f"{tabled} of {len(copy_plans)} copy-format products"
     out["formats"] = {f: sum(1 for pl in plans if pl[0] == i)
-                      for i, f in enumerate(("int8", "fp4", "fp6"))}
+                      for i, f in enumerate(("int8", "fp4", "fp6", "planes"))}
+    # the planes group (products read from their bit-planes) against value
+    # copies, with static operands and with every rhs re-imported before
+    # each run (new activations: copies are rebuilt first, the planes group
+    # reads the new planes as they are)
+    mode = gbs.batch_get_planes()
+    alt = {}
+    try:
+        for label, m in (("copies", gbs.PLANES_NONE), ("planes_all", gbs.PLANES_ALL)):
+            gbs.batch_set_planes(m)
+            bm = gbs.Batch([(l, r, d.ptr) for l, r, d in items])
+
+            def alone_timed(n, fresh, bm=bm):
+                # each run timed alone, from just before the call (marking
+                # 512 matrices through ctypes takes longer than a run)
+                tot = 0.0
+                for i in range(warmup + n):
+                    for _, r, _ in (items if fresh else ()):
+                        gbs.check(gbs.lib().gbs_matrix_mark_modified(r.h))
+                    e0, e1 = gbs.Event(), gbs.Event()
+                    e0.record(stream)
+                    bm.run(stream)
+                    e1.record(stream)
+                    gbs.sync(stream)
+                    if i >= warmup:
+                        tot += e0.elapsed_ms(e1)
+                return tot / n
+            row = {}
+            for kind in ("static", "static_alone", "fresh_rhs_alone"):
+                log(f"[bench] {cname}: {label} {kind} x {steps}")
+                ms = (timed(bm.run, steps) if kind == "static" else
+                      alone_timed(steps, kind == "fresh_rhs_alone"))
+                (l, r, d), p = items[-1], probs[-1]
+                if not np.array_equal(d.to_host(np.int32, (p["N"], p["M"])),
+                                      O.wrap_i32(O.blas_gemm_exact(
+                                          O.values_of(p["lhs"], p["lb"], p["ls"]),
+                                          O.values_of(p["rhs"], p["rb"], p["rs"])))):
+                    raise SystemExit(f"{cname} {label} {kind}: result differs")
+                row[kind + "_us"] = round(ms * 1e3, 1)
+                log(f"[bench] {cname}: {label} {kind} {ms * 1e3:.1f} us")
+            alt[label] = row
+            bm.close()
+    finally:
+        gbs.batch_set_planes(mode)
+    out["planes_group"] = alt
     if cpu_secs > 0 and O.ref_available():
         # the reference loop of benchmark.cpp:213-220 (every product once per
         # pass), timed on every product of the list or, for the 512-product

@@ -42,6 +42,7 @@ EXPORTED = [
     "gbs_batch_run_per_call", "gbs_batch_info", "gbs_batch_plan_of",
     "gbs_batch_tune_classes", "gbs_batch_tune_class_of", "gbs_batch_tune_shapes",
     "gbs_batch_tune_set", "gbs_batch_tune_get", "gbs_batch_tune_load",
+    "gbs_batch_set_planes", "gbs_batch_get_planes",
     "gbs_tune_classes", "gbs_tune_class_of", "gbs_tune_set", "gbs_tune_load",
     "gbs_clock_ns", "gbs_server_create", "gbs_server_destroy", "gbs_server_submit",
     "gbs_server_step", "gbs_server_take", "gbs_server_stats", "gbs_server_layer_input",
@@ -93,6 +94,7 @@ def lib():
         "gbs_batch_tune_classes": [_p], "gbs_batch_tune_class_of": [_u64, _u64, _p],
         "gbs_batch_tune_shapes": [_i, _p, _i, _p], "gbs_batch_tune_set": [_i, _i, _i, _i],
         "gbs_batch_tune_get": [_i, _i, _p, _p], "gbs_batch_tune_load": [C.c_char_p],
+        "gbs_batch_set_planes": [_i], "gbs_batch_get_planes": [_p],
         "gbs_tune_classes": [_p, _p], "gbs_tune_class_of": [_u64, _u64, _u64, _p],
         "gbs_tune_set": [_p, _p, _i], "gbs_tune_load": [C.c_char_p],
         "gbs_gemm_async_config": [_p, _p, _p, _p, _i, _i, _i, _p],
@@ -447,8 +449,10 @@ class Batch:
         return {"tiles": t.value, "persistent": a.value, "separate": b.value}
 
     def plan_of(self, i):
-        """(format, shape, splits, K piece units) the batch planned for
-        problem i; format -1 for a popcount product."""
+        """(group, shape, splits, K piece units) the batch planned for
+        problem i: group 0/1/2 = int8/FP4/FP6 copies, BATCH_PLANES = read
+        from the planes (shape: its tile form; piece: super-steps of 1024
+        values), -1 for a popcount product."""
         f, s = C.c_int(), C.c_int()
         sp, pc = C.c_uint32(), C.c_uint32()
         check(lib().gbs_batch_plan_of(self.h, i, C.byref(f), C.byref(s), C.byref(sp),
@@ -465,6 +469,26 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+BATCH_PLANES = 3
+
+
+PLANES_NONE, PLANES_SKINNY, PLANES_ALL = 0, 1, 2
+
+
+def batch_set_planes(mode):
+    """Which products batches planned from now on read from their bit-planes
+    (the planes group): PLANES_NONE (the default: every product from value
+    copies), PLANES_SKINNY (products of at most 16 rhs rows that do not fit
+    FP4 codes on both sides) or PLANES_ALL (every product it holds)."""
+    check(lib().gbs_batch_set_planes(int(mode)))
+
+
+def batch_get_planes():
+    e = C.c_int()
+    check(lib().gbs_batch_get_planes(C.byref(e)))
+    return e.value
 
 
 def batch_tune_classes():

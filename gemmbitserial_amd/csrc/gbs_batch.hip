@@ -23,7 +23,14 @@
 // Products are grouped by copy format (int8, FP4, FP6); each group has its
 // own descriptor range, tile prefix sums and queue counter, and its own
 // persistent kernel (one kernel carrying every format's tile code would
-// spill registers).
+// spill registers).  On request (gbs_batch_set_planes) products of <= 4-bit
+// operands form a fourth group instead, read straight from their
+// bit-planes by the planes engine's persistent kernel (gbs_planes.hip): no
+// value copies, at more matrix instructions and code expansion per tile.
+//
+// Copies of operands re-imported since the plan are rebuilt at the start
+// of a run, every stale (matrix, format) pair once and all of them in one
+// launch over a job table.
 //
 // Run: one small kernel zeroes the queue counters (and the results of
 // split products, which their pieces add to atomically), then per format group a
@@ -43,21 +50,30 @@ struct gbs_batch {
   int device = 0;
   std::vector<gbs_problem> probs;          // as given
   std::vector<uint32_t> mfma_idx;          // persistent products, in run order
-  std::vector<uint8_t> mfma_fmt;           // their copy formats
-  std::vector<gbs::MfmaPlan> mfma_plans;   // their plans
+  std::vector<uint8_t> mfma_fmt;           // their groups: copy format, or planes (3)
+  std::vector<gbs::MfmaPlan> mfma_plans;   // their plans (planes: variant, splits, jper)
   std::vector<int32_t> plan_of;            // per problem: index into mfma_plans, or -1
   std::vector<uint32_t> separate_idx;      // popcount products
   void *d_desc = nullptr;                  // BatchProb[], grouped by format
+  void *d_pdesc = nullptr;                 // descriptors of the planes group
   void *d_zero = nullptr;                  // ZeroSpan[] of split products
   uint32_t n_zero = 0;
-  unsigned int *d_counter = nullptr;       // one queue counter per format
+  unsigned int *d_counter = nullptr;       // one queue counter per group
   uint32_t *d_item_prod = nullptr;         // product of every tile, per group
   uint64_t total_tiles = 0;
-  uint32_t first[3] = {0, 0, 0}, count[3] = {0, 0, 0};  // descriptor range per format
-  uint64_t tiles[3] = {0, 0, 0};
-  uint64_t item0[3] = {0, 0, 0};            // group's first entry in d_item_prod
-  uint32_t grid[3] = {0, 0, 0};
+  // per group (int8, FP4, FP6 copies; planes): descriptor range, tiles,
+  // first entry in d_item_prod, grid
+  uint32_t first[4] = {0, 0, 0, 0}, count[4] = {0, 0, 0, 0};
+  uint64_t tiles[4] = {0, 0, 0, 0};
+  uint64_t item0[4] = {0, 0, 0, 0};
+  uint32_t grid[4] = {0, 0, 0, 0};
   gbs_workspace *ws = nullptr;             // for separate products
+  // copy refresh in one launch: job table (pinned host, device), capacity
+  // in jobs, and the event after the last refresh launch (its table may be
+  // rewritten once that has passed)
+  gbs::ExpandJob *h_jobs = nullptr, *d_jobs = nullptr;
+  uint32_t jobs_cap = 0;
+  hipEvent_t jobs_done = nullptr;
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
   hipStream_t graph_stream = nullptr;
@@ -77,8 +93,10 @@ struct ZeroSpan {
   uint64_t n;
 };
 
+constexpr int kGroupPlanes = kMfmaFormats, kGroups = kMfmaFormats + 1;
+
 __global__ void reset_counter_kernel(unsigned int *c) {
-  if (threadIdx.x < 3) c[threadIdx.x] = 0;
+  if (threadIdx.x < kGroups) c[threadIdx.x] = 0;
 }
 
 // Zero the results of split products (their partial sums are added
@@ -86,7 +104,7 @@ __global__ void reset_counter_kernel(unsigned int *c) {
 // queue counters, which saves the launch of reset_counter_kernel.
 __global__ __launch_bounds__(256) void zero_spans_kernel(const ZeroSpan *spans,
                                                          unsigned int *counters) {
-  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 3) counters[threadIdx.x] = 0;
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < kGroups) counters[threadIdx.x] = 0;
   const ZeroSpan z = spans[blockIdx.y];
   for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < z.n;
        i += (uint64_t)gridDim.x * 256)
@@ -142,6 +160,22 @@ struct BatchTune {
   int shape = -1, piece = 0;
 };
 BatchTune g_batch_tune[kMfmaFormats][kBatchClasses];
+// Which products go to the planes group (gbs_batch_set_planes): 0 none
+// (the default), 1 those it holds that have at most kPlanesRhsRows rhs rows
+// and do not fit FP4 codes on both sides, 2 every product it holds.
+// Measured on the 512-product uneven workload: per class alone in a
+// launch, beyond FP4 with <= 16 rhs rows the planes group beats FP6 copies
+// (83 against 97 us: half the operand bytes), with 17-32 rows they are
+// even, and wider or FP4-format products run 1.3-2x faster from copies,
+// whose expansion happens once, before the launch, where the planes group
+// expands the plane side once per code-side tile and issues 2-4x the
+// matrix instructions.  In the whole batch the group's own launch (and the
+// tail it adds) outweighs that class's gain: copies 301-307 us, mode 1
+// 322-329, mode 2 365-370; with every rhs re-imported before each run,
+// copies (rebuilt in one launch) 344 us, mode 2 377 us.  The group needs
+// no value copies at all (787 MB less HBM on that workload).
+constexpr uint64_t kPlanesRhsRows = 16;
+int g_batch_planes = 0;
 
 int batch_class(uint64_t N, uint64_t K) {
   const int cn = N <= 16 ? 0 : N <= 32 ? 1 : N <= 64 ? 2 : N <= 128 ? 3 : N <= 256 ? 4 : 5;
@@ -344,15 +378,74 @@ int launch_views(const MfmaOperands *views, const MfmaRequant *rq, int n, unsign
 
 
 // Operands re-imported since the plan get their copies rebuilt before a
-// run, on the run's stream.  The plan stays valid: a copy buffer, once
-// allocated, keeps its address and layout for the matrix's lifetime.
+// run, on the run's stream: every stale (matrix, format) pair once, all in
+// one launch when there are more than two (one launch per copy costs
+// ~4 us of launch and host work: 512 fresh activations took 1.9 ms that
+// way).  The plan stays valid: a copy buffer, once allocated, keeps its
+// address and layout for the matrix's lifetime.
 static int refresh_copies(gbs_batch *b, hipStream_t s) {
+  // each stale pair once: a pair is marked current as it is listed (and
+  // put back if the refresh fails before its launch)
+  struct Stale { gbs_matrix *m; int f; uint64_t version; };
+  std::vector<Stale> stale;
+  auto version_of = [](gbs_matrix *m, int f) -> uint64_t & {
+    return f == kFmtF4 ? m->f4_version : f == kFmtF6 ? m->f6_version : m->vals_version;
+  };
   for (size_t k = 0; k < b->mfma_idx.size(); k++) {
+    const int f = b->mfma_fmt[k];
+    if (f == kGroupPlanes) continue;  // reads the planes themselves
     const gbs_problem &p = b->probs[b->mfma_idx[k]];
-    int rc = ensure_copy(const_cast<gbs_matrix *>(p.lhs), b->mfma_fmt[k], s, false);
-    if (!rc) rc = ensure_copy(const_cast<gbs_matrix *>(p.rhs), b->mfma_fmt[k], s, false);
-    if (rc) return rc;
+    for (gbs_matrix *m : {const_cast<gbs_matrix *>(p.lhs), const_cast<gbs_matrix *>(p.rhs)})
+      if (!copy_current(m, f)) {
+        stale.push_back({m, f, version_of(m, f)});
+        mark_copy_current(m, f);
+      }
   }
+  if (stale.empty()) return GBS_OK;
+  auto undo = [&](size_t from) {
+    for (size_t i = from; i < stale.size(); i++) version_of(stale[i].m, stale[i].f) = stale[i].version;
+  };
+  if (stale.size() <= 2 || stale.size() > b->jobs_cap) {
+    for (size_t i = 0; i < stale.size(); i++) {
+      version_of(stale[i].m, stale[i].f) = stale[i].version;
+      const int rc = ensure_copy(stale[i].m, stale[i].f, s, false);
+      if (rc) {
+        undo(i + 1);
+        return rc;
+      }
+    }
+    return GBS_OK;
+  }
+  // the previous refresh launch has read its table
+  hipError_t e = hipEventSynchronize(b->jobs_done);
+  if (e != hipSuccess) {
+    undo(0);
+    return hip_fail(e, "gbs_batch_run: copy refresh");
+  }
+  uint64_t blocks = 0;
+  for (size_t i = 0; i < stale.size(); i++) {
+    int rc = copy_job(stale[i].m, stale[i].f, false, stale[i].version == 0, &b->h_jobs[i]);
+    if (!rc && blocks + expand_job_blocks(b->h_jobs[i]) >= 0x7fffffffull)
+      rc = fail(GBS_ERANGE, "gbs_batch_run: copy refresh too large");
+    if (rc) {
+      undo(0);
+      return rc;
+    }
+    b->h_jobs[i].block_begin = (uint32_t)blocks;
+    blocks += expand_job_blocks(b->h_jobs[i]);
+  }
+  e = hipMemcpyAsync(b->d_jobs, b->h_jobs, stale.size() * sizeof(ExpandJob),
+                     hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) {
+    undo(0);
+    return hip_fail(e, "gbs_batch_run: copy refresh");
+  }
+  const int rc = launch_expand_jobs(b->d_jobs, (uint32_t)stale.size(), (uint32_t)blocks, s);
+  if (rc) {
+    undo(0);
+    return rc;
+  }
+  GBS_HIP(hipEventRecord(b->jobs_done, s));
   return GBS_OK;
 }
 
@@ -365,6 +458,13 @@ static int batch_launch(gbs_batch *b, hipStream_t s) {
       hipLaunchKernelGGL(reset_counter_kernel, dim3(1), dim3(64), 0, s, b->d_counter);
     GBS_HIP(hipGetLastError());
     const BatchProb *desc = (const BatchProb *)b->d_desc;
+    if (b->tiles[kGroupPlanes]) {
+      const int rc = launch_planes_batch(b->d_pdesc, (uint32_t)b->tiles[kGroupPlanes],
+                                         b->d_counter + kGroupPlanes,
+                                         b->d_item_prod + b->item0[kGroupPlanes],
+                                         b->grid[kGroupPlanes], s);
+      if (rc) return rc;
+    }
     if (b->tiles[kFmtI8])
       hipLaunchKernelGGL((batch_kernel<kFmtI8>), dim3(b->grid[kFmtI8]), dim3(256), 0, s,
                          desc + b->first[kFmtI8], b->count[kFmtI8], b->tiles[kFmtI8],
@@ -406,7 +506,14 @@ int gbs_batch_create(const gbs_problem *problems, uint64_t count, gbs_batch **ou
     delete b;
     return rc;
   }
-  struct Item { uint32_t idx; double item_cost; MfmaPlan plan; MfmaOperands ops; };
+  struct Item {
+    uint32_t idx;
+    int grp;  // copy format, or kGroupPlanes
+    double item_cost;
+    MfmaPlan plan;
+    MfmaOperands ops;
+    PlanesBatchPlan pplan;
+  };
   std::vector<Item> items;
   for (uint64_t i = 0; i < count; i++) {
     const gbs_problem &p = problems[i];
@@ -433,11 +540,20 @@ int gbs_batch_create(const gbs_problem *problems, uint64_t count, gbs_batch **ou
       return rc;
     }
     const Operand lo = view(p.lhs), ro = view(p.rhs);
-    const int fmt = f4_supported(lo, ro, p.lhs->ncols)   ? kFmtF4
-                    : f6_supported(lo, ro, p.lhs->ncols) ? kFmtF6
-                    : mfma_supported(lo, ro, p.lhs->ncols) ? kFmtI8
-                                                           : -1;
-    if (fmt >= 0 && p.lhs->ncols > 0) {
+    const uint64_t K = p.lhs->ncols;
+    if (g_batch_planes == 2 ||
+        (g_batch_planes == 1 && ro.nrows <= kPlanesRhsRows && !f4_supported(lo, ro, K))) {
+      const PlanesBatchPlan pp = planes_batch_plan(lo, ro, K);
+      if (pp.ok) {
+        items.push_back({(uint32_t)i, kGroupPlanes, pp.item_bytes, MfmaPlan{}, MfmaOperands{}, pp});
+        continue;
+      }
+    }
+    const int fmt = f4_supported(lo, ro, K)     ? kFmtF4
+                    : f6_supported(lo, ro, K)   ? kFmtF6
+                    : mfma_supported(lo, ro, K) ? kFmtI8
+                                                : -1;
+    if (fmt >= 0 && K > 0) {
       rc = ensure_copy(const_cast<gbs_matrix *>(p.lhs), fmt, s, true);
       if (!rc) rc = ensure_copy(const_cast<gbs_matrix *>(p.rhs), fmt, s, true);
       if (rc) {
@@ -447,43 +563,84 @@ int gbs_batch_create(const gbs_problem *problems, uint64_t count, gbs_batch **ou
       const MfmaOperands ops = mfma_operands(p.lhs, p.rhs, fmt, p.d_res);
       const MfmaPlan pl = batch_plan(ops.M, ops.N, ops.kbytes, fmt);
       const MfmaTileShape sh = kMfmaShape[pl.shape];
-      items.push_back({(uint32_t)i, (double)sh.P * sh.Q * (double)pl.kchunk, pl, ops});
+      items.push_back({(uint32_t)i, fmt, (double)sh.P * sh.Q * (double)pl.kchunk, pl, ops,
+                       PlanesBatchPlan{}});
     } else {
       b->separate_idx.push_back((uint32_t)i);
     }
   }
-  // re-split products whose items would dominate the queue (rebalance)
+  // re-split products whose items would dominate the queue (rebalance; for
+  // the planes group the same rule on its items' plane bytes)
   {
     double total = 0;
-    std::vector<BatchProb> tmp(items.size());
-    for (size_t i = 0; i < items.size(); i++) {
-      tmp[i].ops = items[i].ops;
-      tmp[i].plan = items[i].plan;
-      total += item_bytes(items[i].plan) * (double)items[i].plan.tiles();
-    }
-    rebalance(tmp.data(), (int)tmp.size(), total);
-    for (size_t i = 0; i < items.size(); i++) {
-      const MfmaTileShape sh = kMfmaShape[tmp[i].plan.shape];
-      items[i].plan = tmp[i].plan;
-      items[i].item_cost = (double)sh.P * sh.Q * (double)tmp[i].plan.kchunk;
+    std::vector<BatchProb> tmp;
+    for (const Item &it : items)
+      total += it.grp == kGroupPlanes ? it.pplan.item_bytes * (double)it.pplan.tiles()
+                                      : item_bytes(it.plan) * (double)it.plan.tiles();
+    for (Item &it : items) {
+      if (it.grp == kGroupPlanes) {
+        const double target = total / 1024;
+        if (target > 0 && it.pplan.item_bytes > target) {
+          const gbs_problem &p = problems[it.idx];
+          const double per_tile = it.pplan.item_bytes * it.pplan.splits;
+          it.pplan = planes_batch_plan(view(p.lhs), view(p.rhs), p.lhs->ncols,
+                                       (uint32_t)std::min(per_tile / target + 1, 1e6));
+          it.item_cost = it.pplan.item_bytes;
+        }
+        continue;
+      }
+      BatchProb t{};
+      t.ops = it.ops;
+      t.plan = it.plan;
+      rebalance(&t, 1, total);
+      const MfmaTileShape sh = kMfmaShape[t.plan.shape];
+      it.plan = t.plan;
+      it.item_cost = (double)sh.P * sh.Q * (double)t.plan.kchunk;
     }
   }
-  // grouped by format; inside a group the longest items first: they start
-  // early and the short ones fill the tail
-  std::stable_sort(items.begin(), items.end(), [](const Item &x, const Item &y) {
-    return x.plan.fmt != y.plan.fmt ? x.plan.fmt < y.plan.fmt : x.item_cost > y.item_cost;
+  // grouped (the planes group first, then the copy formats); inside a group
+  // the longest items first: they start early and the short ones fill the
+  // tail
+  auto order = [](int g) { return g == kGroupPlanes ? -1 : g; };
+  std::stable_sort(items.begin(), items.end(), [&](const Item &x, const Item &y) {
+    return x.grp != y.grp ? order(x.grp) < order(y.grp) : x.item_cost > y.item_cost;
   });
   std::vector<BatchProb> desc;
+  std::vector<uint8_t> pdesc;
+  const size_t psz = planes_batch_desc_size();
   std::vector<ZeroSpan> zero;
   const uint64_t slots = 256ull * 2;
+  // per group: (descriptor index within the group, its first tile, tiles)
+  struct Span { uint32_t k; uint64_t begin, n; };
+  std::vector<Span> spans[kGroups];
   for (const Item &it : items) {
     const gbs_problem &p = problems[it.idx];
-    const int f = it.plan.fmt;
+    const int f = it.grp;
+    if (f == kGroupPlanes) {
+      const PlanesBatchPlan &pp = it.pplan;
+      spans[f].push_back({b->count[f], b->tiles[f], pp.tiles()});
+      pdesc.resize(pdesc.size() + psz);
+      planes_batch_desc(view(p.lhs), view(p.rhs), p.lhs->ncols, p.d_res, pp, b->tiles[f],
+                        pdesc.data() + pdesc.size() - psz);
+      b->tiles[f] += pp.tiles();
+      b->count[f]++;
+      if (pp.splits > 1) zero.push_back(ZeroSpan{p.d_res, p.lhs->nrows * p.rhs->nrows});
+      MfmaPlan rec{};
+      rec.fmt = kGroupPlanes;
+      rec.shape = (int)pp.variant;
+      rec.splits = pp.splits;
+      rec.kchunk = pp.jper;
+      b->mfma_idx.push_back(it.idx);
+      b->mfma_fmt.push_back((uint8_t)f);
+      b->mfma_plans.push_back(rec);
+      continue;
+    }
     if (b->count[f] == 0) b->first[f] = (uint32_t)desc.size();
     BatchProb d{};
     d.ops = it.ops;
     d.plan = it.plan;
     d.tile_begin = b->tiles[f];
+    spans[f].push_back({b->count[f], b->tiles[f], d.plan.tiles()});
     b->tiles[f] += d.plan.tiles();
     b->count[f]++;
     if (d.plan.splits > 1) zero.push_back(ZeroSpan{p.d_res, (uint64_t)d.ops.M * d.ops.N});
@@ -494,7 +651,7 @@ int gbs_batch_create(const gbs_problem *problems, uint64_t count, gbs_batch **ou
   }
   b->plan_of.assign(count, -1);
   for (size_t k = 0; k < b->mfma_idx.size(); k++) b->plan_of[b->mfma_idx[k]] = (int32_t)k;
-  for (int f = 0; f < 3; f++) {
+  for (int f = 0; f < kGroups; f++) {
     b->item0[f] = b->total_tiles;
     b->total_tiles += b->tiles[f];
     b->grid[f] = (uint32_t)(b->tiles[f] < slots ? b->tiles[f] : slots);
@@ -503,16 +660,24 @@ int gbs_batch_create(const gbs_problem *problems, uint64_t count, gbs_batch **ou
     gbs_batch_destroy(b);
     return fail(GBS_ERANGE, "gbs_batch_create: more than 2^32 tiles");
   }
-  // product (index within its format group) of every tile
+  // product (descriptor index within its group) of every tile
   std::vector<uint32_t> item_prod(b->total_tiles);
-  for (int f = 0; f < 3; f++)
-    for (uint32_t k = 0; k < b->count[f]; k++) {
-      const BatchProb &d = desc[b->first[f] + k];
-      const uint64_t at = b->item0[f] + d.tile_begin;
-      std::fill(item_prod.begin() + at, item_prod.begin() + at + d.plan.tiles(), k);
+  for (int f = 0; f < kGroups; f++)
+    for (const Span &sp : spans[f]) {
+      const uint64_t at = b->item0[f] + sp.begin;
+      std::fill(item_prod.begin() + at, item_prod.begin() + at + sp.n, sp.k);
     }
-  if (!desc.empty()) {
-    hipError_t e = hipMalloc(&b->d_desc, desc.size() * sizeof(BatchProb));
+  if (!pdesc.empty()) {
+    hipError_t e = hipMalloc(&b->d_pdesc, pdesc.size());
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(b->d_pdesc, pdesc.data(), pdesc.size(), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) {
+      gbs_batch_destroy(b);
+      return hip_fail(e, "gbs_batch_create");
+    }
+  }
+  if (!desc.empty() || !pdesc.empty()) {
+    hipError_t e = desc.empty() ? hipSuccess : hipMalloc(&b->d_desc, desc.size() * sizeof(BatchProb));
     if (e == hipSuccess) e = hipMalloc((void **)&b->d_counter, 256);
     if (e == hipSuccess) e = hipMalloc((void **)&b->d_item_prod, item_prod.size() * 4);
     if (e == hipSuccess)
@@ -525,7 +690,7 @@ int gbs_batch_create(const gbs_problem *problems, uint64_t count, gbs_batch **ou
                            hipMemcpyHostToDevice, s);
       b->n_zero = (uint32_t)zero.size();
     }
-    if (e == hipSuccess)
+    if (e == hipSuccess && !desc.empty())
       e = hipMemcpyAsync(b->d_desc, desc.data(), desc.size() * sizeof(BatchProb),
                          hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -538,6 +703,21 @@ int gbs_batch_create(const gbs_problem *problems, uint64_t count, gbs_batch **ou
     if (e != hipSuccess) {
       gbs_batch_destroy(b);
       return hip_fail(e, "gbs_batch_create");
+    }
+  }
+  // copy refresh table: at most two copies per copy-format product
+  {
+    uint32_t n_copy = 0;
+    for (uint8_t f : b->mfma_fmt) n_copy += f != kGroupPlanes;
+    if (n_copy) {
+      b->jobs_cap = 2 * n_copy;
+      hipError_t e = hipHostMalloc((void **)&b->h_jobs, b->jobs_cap * sizeof(ExpandJob));
+      if (e == hipSuccess) e = hipMalloc((void **)&b->d_jobs, b->jobs_cap * sizeof(ExpandJob));
+      if (e == hipSuccess) e = hipEventCreateWithFlags(&b->jobs_done, hipEventDisableTiming);
+      if (e != hipSuccess) {
+        gbs_batch_destroy(b);
+        return hip_fail(e, "gbs_batch_create");
+      }
     }
   }
   b->device = 0;
@@ -553,6 +733,10 @@ int gbs_batch_destroy(gbs_batch *b) {
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
   if (b->d_desc) (void)hipFree(b->d_desc);
+  if (b->d_pdesc) (void)hipFree(b->d_pdesc);
+  if (b->h_jobs) (void)hipHostFree(b->h_jobs);
+  if (b->d_jobs) (void)hipFree(b->d_jobs);
+  if (b->jobs_done) (void)hipEventDestroy(b->jobs_done);
   if (b->d_counter) (void)hipFree(b->d_counter);
   if (b->d_item_prod) (void)hipFree(b->d_item_prod);
   if (b->d_zero) (void)hipFree(b->d_zero);
@@ -625,7 +809,20 @@ int gbs_batch_plan_of(const gbs_batch *b, uint64_t problem, int *fmt, int *shape
   if (fmt) *fmt = pl ? pl->fmt : -1;
   if (shape) *shape = pl ? pl->shape : -1;
   if (splits) *splits = pl ? pl->splits : 0;
-  if (piece_units) *piece_units = pl ? pl->kchunk / kMfmaKS : 0;
+  if (piece_units)
+    *piece_units = !pl ? 0 : pl->fmt == kGroupPlanes ? pl->kchunk : pl->kchunk / kMfmaKS;
+  return GBS_OK;
+}
+
+int gbs_batch_set_planes(int mode) {
+  if (mode < 0 || mode > 2) return fail(GBS_EINVAL, "gbs_batch_set_planes: mode %d", mode);
+  g_batch_planes = mode;
+  return GBS_OK;
+}
+
+int gbs_batch_get_planes(int *mode) {
+  if (!mode) return fail(GBS_EINVAL, "gbs_batch_get_planes: null out");
+  *mode = g_batch_planes;
   return GBS_OK;
 }
 

@@ -107,6 +107,28 @@ int launch_bits_gemm(const Operand &l, const Operand &r, uint64_t K, int32_t *d_
 bool planes_supported(const Operand &l, const Operand &r, uint64_t K);
 int launch_planes_gemm(const Operand &l, const Operand &r, uint64_t K, int32_t *d_res,
                        hipStream_t s);
+// The same engine as one persistent launch over many products (the batch's
+// planes group): lhs = plane side (<= 4 planes), rhs = code side (one FP4
+// code, or two chunks of <= 4 planes), not both bipolar.  A plan gives the
+// product's work items (code-side tiles x plane-side tiles x K pieces of
+// jper super-steps of 1024 values); planes_batch_desc writes its device
+// descriptor (planes_batch_desc_size() bytes) with the index of its first
+// item; launch_planes_batch runs `total` items over a grid of persistent
+// blocks (item_prod: the descriptor of every item; results of products of
+// more than one K piece must be zero before it, the pieces add into them).
+struct PlanesBatchPlan {
+  bool ok;
+  uint32_t vtiles, ptiles, splits, jper, variant;
+  double item_bytes;  // plane bytes one item loads
+  uint64_t tiles() const { return (uint64_t)vtiles * ptiles * splits; }
+};
+PlanesBatchPlan planes_batch_plan(const Operand &l, const Operand &r, uint64_t K,
+                                  uint32_t min_pieces = 1);
+size_t planes_batch_desc_size();
+void planes_batch_desc(const Operand &l, const Operand &r, uint64_t K, int32_t *d_res,
+                       const PlanesBatchPlan &pl, uint64_t tile_begin, void *out);
+int launch_planes_batch(const void *d_desc, uint32_t total, unsigned int *counter,
+                        const uint32_t *item_prod, uint32_t grid, hipStream_t s);
 // Matrix-core engine (gbs_mfma.hip).  ensure_vals / ensure_f4 / ensure_f6
 // make the int8 / FP4 / FP6 copy current (allocating it on first use when
 // may_allocate); ensure_copy picks by format (0 int8, 1 FP4, 2 FP6).
@@ -114,6 +136,31 @@ int ensure_vals(gbs_matrix *m, hipStream_t s, bool may_allocate);
 int ensure_f4(gbs_matrix *m, hipStream_t s, bool may_allocate);
 int ensure_f6(gbs_matrix *m, hipStream_t s, bool may_allocate);
 int ensure_copy(gbs_matrix *m, int fmt, hipStream_t s, bool may_allocate);
+// Copies rebuilt many at a time (the batch's refresh): copy_current tells
+// whether m's copy in fmt matches its planes; copy_job describes the
+// rebuild (allocating the copy on first use when may_allocate; `whole`
+// also writes the padding rows, which a copy built before already holds
+// as zeros); jobs laid
+// out with block_begin = the prefix of expand_job_blocks run in one
+// launch_expand_jobs over d_jobs (device memory), after which the caller
+// calls mark_copy_current on each.
+struct ExpandJob {
+  const uint64_t *planes;
+  uint64_t nrows, ncols, wpr, W;  // the matrix: rows, K, row pitch, words per plane
+  uint32_t nbits;
+  bool issigned;
+  uint64_t rows_a, ld;            // the copy: rows (padded), bytes per row
+  uint64_t rows_do;               // rows written: rows_a on the first build (the
+                                  // padding rows' zeros), the matrix's rows after
+  void *out;
+  int fmt;
+  uint32_t block_begin;           // first block of the job in a many-job launch
+};
+bool copy_current(const gbs_matrix *m, int fmt);
+int copy_job(gbs_matrix *m, int fmt, bool may_allocate, bool whole, ExpandJob *j);
+void mark_copy_current(gbs_matrix *m, int fmt);
+uint32_t expand_job_blocks(const ExpandJob &j);
+int launch_expand_jobs(const ExpandJob *d_jobs, uint32_t n, uint32_t blocks, hipStream_t s);
 int launch_mfma_gemm(const gbs_matrix *l, const gbs_matrix *r, int32_t *d_res,
                      hipStream_t s, int fmt, int forced_shape = -1, bool allow_split = true,
                      int forced_splits = 0);

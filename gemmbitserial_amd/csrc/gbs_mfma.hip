@@ -61,12 +61,13 @@ void mfma_kernel(MfmaOperands a, MfmaPlan pl) {
 // puts bit i of x at bit 8i, with no two partial products meeting); then
 // bipolar codes 0/1 become 0xFF/0x01 and signed codes are sign-extended
 // bytewise.  Zero outside the real matrix, also for bipolar operands.
-__global__ __launch_bounds__(256) void expand_kernel(
-    const uint64_t *planes, uint64_t nrows, uint64_t ncols, uint64_t wpr, uint64_t W,
-    uint32_t nbits, bool issigned, uint64_t rows_a, uint64_t ld, int8_t *out) {
+__device__ __forceinline__ void expand_i8_word(const ExpandJob &j, uint64_t i) {
+  const uint64_t *planes = j.planes, nrows = j.nrows, ncols = j.ncols, wpr = j.wpr, W = j.W;
+  const uint32_t nbits = j.nbits;
+  const bool issigned = j.issigned;
+  const uint64_t ld = j.ld;
+  int8_t *out = (int8_t *)j.out;
   const uint64_t wpl = ld / 64;
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= rows_a * wpl) return;
   const uint64_t r = i / wpl, w = i - r * wpl;
   uint32_t q[16];
 #pragma unroll
@@ -111,12 +112,13 @@ __global__ __launch_bounds__(256) void expand_kernel(
 // bytes; value k of the word in nibble k, low nibble first).  Code of an
 // integer v (|v| <= 4) under scale 2^1: sign bit 8 | |v|.  Zero outside the
 // real matrix, also for bipolar operands.
-__global__ __launch_bounds__(256) void expand_f4_kernel(
-    const uint64_t *planes, uint64_t nrows, uint64_t ncols, uint64_t wpr, uint64_t W,
-    uint32_t nbits, bool issigned, uint64_t rows_a, uint64_t ld, uint8_t *out) {
+__device__ __forceinline__ void expand_f4_word(const ExpandJob &j, uint64_t i) {
+  const uint64_t *planes = j.planes, nrows = j.nrows, ncols = j.ncols, wpr = j.wpr, W = j.W;
+  const uint32_t nbits = j.nbits;
+  const bool issigned = j.issigned;
+  const uint64_t ld = j.ld;
+  uint8_t *out = (uint8_t *)j.out;
   const uint64_t wpl = ld / 32;
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= rows_a * wpl) return;
   const uint64_t r = i / wpl, w = i - r * wpl;
   uint32_t q[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   if (r < nrows && w * 64 < ncols) {
@@ -168,12 +170,13 @@ __global__ __launch_bounds__(256) void expand_f4_kernel(
 // go to offset 16g of the k-step, bytes 16..23 to offset 64 + 8g.  Code of
 // an integer v (|v| <= 16) under scale 2^3: sign bit 32 | |v|.  Zero
 // outside the real matrix, also for bipolar operands.
-__global__ __launch_bounds__(256) void expand_f6_kernel(
-    const uint64_t *planes, uint64_t nrows, uint64_t ncols, uint64_t wpr, uint64_t W,
-    uint32_t nbits, bool issigned, uint64_t rows_a, uint64_t ld, uint8_t *out) {
+__device__ __forceinline__ void expand_f6_word(const ExpandJob &j, uint64_t i) {
+  const uint64_t *planes = j.planes, nrows = j.nrows, ncols = j.ncols, wpr = j.wpr, W = j.W;
+  const uint32_t nbits = j.nbits;
+  const bool issigned = j.issigned;
+  const uint64_t ld = j.ld;
+  uint8_t *out = (uint8_t *)j.out;
   const uint64_t wpl = ld / 48;
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= rows_a * wpl) return;
   const uint64_t r = i / wpl, w = i - r * wpl;
   uint32_t d[2][6] = {{0, 0, 0, 0, 0, 0}, {0, 0, 0, 0, 0, 0}};
   if (r < nrows && w * 64 < ncols) {
@@ -226,6 +229,38 @@ __global__ __launch_bounds__(256) void expand_f6_kernel(
     *reinterpret_cast<uint4 *>(base + 16 * g) = make_uint4(d[gi][0], d[gi][1], d[gi][2], d[gi][3]);
     *reinterpret_cast<uint2 *>(base + 64 + 8 * g) = make_uint2(d[gi][4], d[gi][5]);
   }
+}
+
+// Threads (64-value words of a copy row) of a job: every word of its first
+// rows_do rows (a rebuild leaves the padding rows' zeros in place).
+__host__ __device__ inline uint64_t expand_units(const ExpandJob &j) {
+  return j.rows_do * (j.ld / (j.fmt == kFmtF4 ? 32 : j.fmt == kFmtF6 ? 48 : 64));
+}
+
+template <int FMT>
+__global__ __launch_bounds__(256) void expand_kernel(ExpandJob j) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= expand_units(j)) return;
+  if constexpr (FMT == kFmtF4) expand_f4_word(j, i);
+  else if constexpr (FMT == kFmtF6) expand_f6_word(j, i);
+  else expand_i8_word(j, i);
+}
+
+// Many copies in one launch: job k owns blocks from jobs[k].block_begin
+// (each job starts on a block of its own, so a block's job is uniform).
+__global__ __launch_bounds__(256) void expand_jobs_kernel(const ExpandJob *jobs, uint32_t n) {
+  uint32_t lo = 0, hi = n - 1;  // largest k with block_begin <= blockIdx.x
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) / 2;
+    if (jobs[mid].block_begin <= blockIdx.x) lo = mid;
+    else hi = mid - 1;
+  }
+  const ExpandJob &j = jobs[lo];
+  const uint64_t i = (uint64_t)(blockIdx.x - j.block_begin) * blockDim.x + threadIdx.x;
+  if (i >= expand_units(j)) return;
+  if (j.fmt == kFmtF4) expand_f4_word(j, i);
+  else if (j.fmt == kFmtF6) expand_f6_word(j, i);
+  else expand_i8_word(j, i);
 }
 
 uint64_t max_abs(const Operand &o) {
@@ -385,7 +420,6 @@ gbs_algo select_algo(const Operand &l, const Operand &r, uint64_t K) {
   return fmt == kFmtF4 ? GBS_ALGO_MFMA_FP4 : fmt == kFmtF6 ? GBS_ALGO_MFMA_FP6 : GBS_ALGO_MFMA;
 }
 
-static bool copies_current(const gbs_matrix *m, int fmt);
 
 // AUTO for two matrices in their present state: select_algo's choice,
 // except that an int8- or FP6-copy engine whose operands' copies would
@@ -408,80 +442,101 @@ gbs_algo select_for(const gbs_matrix *l, const gbs_matrix *r) {
                   : eng == GBS_ALGO_MFMA_FP6 ? kFmtF6
                   : eng == GBS_ALGO_MFMA     ? kFmtI8
                                              : -1;
-  if (fmt < 0 || fmt == kFmtF4 || (copies_current(l, fmt) && copies_current(r, fmt))) return eng;
+  if (fmt < 0 || fmt == kFmtF4 || (copy_current(l, fmt) && copy_current(r, fmt))) return eng;
   const double ops = 2.0 * (double)lo.nrows * (double)ro.nrows * (double)K;
   if (ops >= 8e9 && planes_supported(lo, ro, K)) return GBS_ALGO_MFMA_PLANES;
   return eng;
 }
 
-int ensure_vals(gbs_matrix *m, hipStream_t s, bool may_allocate) {
-  if (m->vals_version == m->planes_version && m->d_vals) return GBS_OK;
+// Geometry of m's copy in format fmt: rows padded to kValsRowAlign; a row
+// holds the padded K in int8 bytes, FP4 nibbles or FP6 fields.
+static void copy_geometry(const gbs_matrix *m, int fmt, uint64_t &rows_a, uint64_t &ld_a) {
   const uint64_t rows = ((m->nrows + kValsRowAlign - 1) / kValsRowAlign) * kValsRowAlign;
-  const uint64_t ld = ((m->ncols + kValsColAlign - 1) / kValsColAlign) * kValsColAlign;
-  const uint64_t rows_a = rows ? rows : kValsRowAlign, ld_a = ld ? ld : kValsColAlign;
-  if (!m->d_vals) {
-    if (!may_allocate)
-      return fail(GBS_ERANGE, "mfma: operand has no int8 copy yet; call gbs_workspace_reserve");
-    GBS_HIP(hipMalloc((void **)&m->d_vals, rows_a * ld_a));
-    m->vals_rows = rows_a;
-    m->vals_ld = ld_a;
+  uint64_t ld, ld0;
+  if (fmt == kFmtF4) {
+    ld = ((m->ncols + kPackedColAlign - 1) / kPackedColAlign) * kPackedColAlign / 2;
+    ld0 = kPackedColAlign / 2;
+  } else if (fmt == kFmtF6) {
+    ld = ((m->ncols + kPackedColAlign - 1) / kPackedColAlign) * kPackedColAlign / 4 * 3;
+    ld0 = kPackedColAlign / 4 * 3;
+  } else {
+    ld = ((m->ncols + kValsColAlign - 1) / kValsColAlign) * kValsColAlign;
+    ld0 = kValsColAlign;
   }
-  const uint64_t n = m->vals_rows * (m->vals_ld / 64);
-  hipLaunchKernelGGL(expand_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s,
-                     m->d_planes, m->nrows, m->ncols, m->wpr(), m->wpp(),
-                     (uint32_t)m->nbits, m->issigned, m->vals_rows, m->vals_ld, m->d_vals);
-  GBS_HIP(hipGetLastError());
-  m->vals_version = m->planes_version;
-  return GBS_OK;
+  rows_a = rows ? rows : kValsRowAlign;
+  ld_a = ld ? ld : ld0;
 }
 
-int ensure_f4(gbs_matrix *m, hipStream_t s, bool may_allocate) {
-  if (m->f4_version == m->planes_version && m->d_f4) return GBS_OK;
-  if (!m->d_f4) {
-    const uint64_t rows = ((m->nrows + kValsRowAlign - 1) / kValsRowAlign) * kValsRowAlign;
-    const uint64_t ld = ((m->ncols + kPackedColAlign - 1) / kPackedColAlign) * kPackedColAlign / 2;
-    const uint64_t rows_a = rows ? rows : kValsRowAlign, ld_a = ld ? ld : kPackedColAlign / 2;
-    if (!may_allocate)
-      return fail(GBS_ERANGE, "mfma: operand has no FP4 copy yet; call gbs_workspace_reserve");
-    GBS_HIP(hipMalloc((void **)&m->d_f4, rows_a * ld_a));
-    m->f4_rows = rows_a;
-    m->f4_ld = ld_a;
-  }
-  const uint64_t n = m->f4_rows * (m->f4_ld / 32);
-  hipLaunchKernelGGL(expand_f4_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s,
-                     m->d_planes, m->nrows, m->ncols, m->wpr(), m->wpp(), (uint32_t)m->nbits,
-                     m->issigned, m->f4_rows, m->f4_ld, (uint8_t *)m->d_f4);
-  GBS_HIP(hipGetLastError());
-  m->f4_version = m->planes_version;
-  return GBS_OK;
+bool copy_current(const gbs_matrix *m, int fmt) {
+  if (fmt == kFmtF4) return m->d_f4 && m->f4_version == m->planes_version;
+  if (fmt == kFmtF6) return m->d_f6 && m->f6_version == m->planes_version;
+  return m->d_vals && m->vals_version == m->planes_version;
 }
 
-int ensure_f6(gbs_matrix *m, hipStream_t s, bool may_allocate) {
-  if (m->f6_version == m->planes_version && m->d_f6) return GBS_OK;
-  if (!m->d_f6) {
-    const uint64_t rows = ((m->nrows + kValsRowAlign - 1) / kValsRowAlign) * kValsRowAlign;
-    const uint64_t ld =
-        ((m->ncols + kPackedColAlign - 1) / kPackedColAlign) * kPackedColAlign / 4 * 3;
-    const uint64_t rows_a = rows ? rows : kValsRowAlign, ld_a = ld ? ld : kPackedColAlign / 4 * 3;
+void mark_copy_current(gbs_matrix *m, int fmt) {
+  (fmt == kFmtF4 ? m->f4_version : fmt == kFmtF6 ? m->f6_version : m->vals_version) =
+      m->planes_version;
+}
+
+int copy_job(gbs_matrix *m, int fmt, bool may_allocate, bool whole, ExpandJob *j) {
+  static const char *const names[] = {"int8", "FP4", "FP6"};
+  int8_t **buf = fmt == kFmtF4 ? &m->d_f4 : fmt == kFmtF6 ? &m->d_f6 : &m->d_vals;
+  uint64_t &rows = fmt == kFmtF4 ? m->f4_rows : fmt == kFmtF6 ? m->f6_rows : m->vals_rows;
+  uint64_t &ld = fmt == kFmtF4 ? m->f4_ld : fmt == kFmtF6 ? m->f6_ld : m->vals_ld;
+  if (!*buf) {
     if (!may_allocate)
-      return fail(GBS_ERANGE, "mfma: operand has no FP6 copy yet; call gbs_workspace_reserve");
-    GBS_HIP(hipMalloc((void **)&m->d_f6, rows_a * ld_a));
-    m->f6_rows = rows_a;
-    m->f6_ld = ld_a;
+      return fail(GBS_ERANGE, "mfma: operand has no %s copy yet; call gbs_workspace_reserve",
+                  names[fmt]);
+    uint64_t rows_a, ld_a;
+    copy_geometry(m, fmt, rows_a, ld_a);
+    GBS_HIP(hipMalloc((void **)buf, rows_a * ld_a));
+    rows = rows_a;
+    ld = ld_a;
+    whole = true;
   }
-  const uint64_t n = m->f6_rows * (m->f6_ld / 48);
-  hipLaunchKernelGGL(expand_f6_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s,
-                     m->d_planes, m->nrows, m->ncols, m->wpr(), m->wpp(), (uint32_t)m->nbits,
-                     m->issigned, m->f6_rows, m->f6_ld, (uint8_t *)m->d_f6);
-  GBS_HIP(hipGetLastError());
-  m->f6_version = m->planes_version;
+  *j = ExpandJob{m->d_planes, m->nrows, m->ncols, m->wpr(), m->wpp(), (uint32_t)m->nbits,
+                 m->issigned, rows, ld, whole ? rows : m->nrows, *buf, fmt, 0};
   return GBS_OK;
 }
 
 int ensure_copy(gbs_matrix *m, int fmt, hipStream_t s, bool may_allocate) {
-  return fmt == kFmtF4   ? ensure_f4(m, s, may_allocate)
-         : fmt == kFmtF6 ? ensure_f6(m, s, may_allocate)
-                         : ensure_vals(m, s, may_allocate);
+  if (copy_current(m, fmt)) return GBS_OK;
+  const uint64_t v = fmt == kFmtF4 ? m->f4_version : fmt == kFmtF6 ? m->f6_version : m->vals_version;
+  ExpandJob j;
+  const int rc = copy_job(m, fmt, may_allocate, v == 0, &j);
+  if (rc) return rc;
+  const uint32_t blocks = (uint32_t)((expand_units(j) + 255) / 256);
+  if (!blocks) {
+    mark_copy_current(m, fmt);
+    return GBS_OK;
+  }
+  if (fmt == kFmtF4) hipLaunchKernelGGL(expand_kernel<kFmtF4>, dim3(blocks), dim3(256), 0, s, j);
+  else if (fmt == kFmtF6) hipLaunchKernelGGL(expand_kernel<kFmtF6>, dim3(blocks), dim3(256), 0, s, j);
+  else hipLaunchKernelGGL(expand_kernel<kFmtI8>, dim3(blocks), dim3(256), 0, s, j);
+  GBS_HIP(hipGetLastError());
+  mark_copy_current(m, fmt);
+  return GBS_OK;
+}
+
+int ensure_vals(gbs_matrix *m, hipStream_t s, bool may_allocate) {
+  return ensure_copy(m, kFmtI8, s, may_allocate);
+}
+int ensure_f4(gbs_matrix *m, hipStream_t s, bool may_allocate) {
+  return ensure_copy(m, kFmtF4, s, may_allocate);
+}
+int ensure_f6(gbs_matrix *m, hipStream_t s, bool may_allocate) {
+  return ensure_copy(m, kFmtF6, s, may_allocate);
+}
+
+uint32_t expand_job_blocks(const ExpandJob &j) {
+  return (uint32_t)((expand_units(j) + 255) / 256);
+}
+
+int launch_expand_jobs(const ExpandJob *d_jobs, uint32_t n, uint32_t blocks, hipStream_t s) {
+  if (!n || !blocks) return GBS_OK;
+  hipLaunchKernelGGL(expand_jobs_kernel, dim3(blocks), dim3(256), 0, s, d_jobs, n);
+  GBS_HIP(hipGetLastError());
+  return GBS_OK;
 }
 
 MfmaOperands mfma_operands(const gbs_matrix *l, const gbs_matrix *r, int fmt, int32_t *d_res) {
@@ -500,12 +555,6 @@ MfmaOperands mfma_operands(const gbs_matrix *l, const gbs_matrix *r, int fmt, in
   return a;
 }
 
-static bool copies_current(const gbs_matrix *m, int fmt) {
-  return fmt == kFmtF4   ? m->f4_version == m->planes_version
-         : fmt == kFmtF6 ? m->f6_version == m->planes_version
-                         : m->vals_version == m->planes_version;
-}
-
 int launch_mfma_gemm(const gbs_matrix *l, const gbs_matrix *r, int32_t *d_res,
                      hipStream_t s, int fmt, int forced_shape, bool allow_split,
                      int forced_splits) {
@@ -514,7 +563,7 @@ int launch_mfma_gemm(const gbs_matrix *l, const gbs_matrix *r, int32_t *d_res,
   if (M > 0x7fffffffull || N > 0x7fffffffull)
     return fail(GBS_ERANGE, "gemm dimensions exceed 2^31");
   const MfmaOperands a = mfma_operands(l, r, fmt, d_res);
-  if (!a.R || !a.L || a.r_ld != a.l_ld || !copies_current(l, fmt) || !copies_current(r, fmt))
+  if (!a.R || !a.L || a.r_ld != a.l_ld || !copy_current(l, fmt) || !copy_current(r, fmt))
     return fail(GBS_EINVAL, "mfma: operand value copies are not current");
   if (a.kbytes == 0) return launch_fill_i32(d_res, M * N, 0, s);
   const MfmaPlan pl = mfma_plan(a.M, a.N, a.kbytes, fmt, forced_shape,

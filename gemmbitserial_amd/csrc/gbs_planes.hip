@@ -45,7 +45,14 @@
 // 16 * WPG of them) x BV = 16 * VG code-side rows (shared by the waves).
 // The rhs operand goes to the MFMA A port so that output lanes run along m,
 // the contiguous dimension of res.
+//
+// The same tile body runs one product per launch (planes_kernel, the
+// plane side's encoding fixed at compile time) and many products in one
+// persistent launch (planes_batch_kernel: lhs = plane side, its encoding
+// from masks at run time, items = code-side tile x plane-side tile x K
+// piece, pieces added into zeroed results with integer atomics).
 #include <stdlib.h>
+#include <string.h>
 
 #include <type_traits>
 
@@ -162,11 +169,47 @@ __host__ __device__ inline CodeMasks code_masks(int enc) {
   }
 }
 
+// Plane-side encoding known only at run time (the batched launch, whose
+// products differ): the top chunk's codes come from three words made of its
+// low plane word lo and high plane word hi (zero when the chunk has one
+// plane) under these masks -- m0 = lo | or0, s = ((hi & sgm) | (lo & sgl))
+// ^ xs, t = hi & (tu | ~lo) -- and code = m0 | t << 1 | s << 3 per nibble:
+//   one unsigned plane       all zero                  code b
+//   one signed plane (-b)    sgl                       1001 (-0.5) for b
+//   two unsigned planes      tu                        lo | hi << 1
+//   two signed ([-2, 1])     sgm                       chunk_reg KIND 3
+//   bipolar (+-1)            or0, sgl, xs              0001 / 1001
+// Lower chunks are always two unsigned planes.
+struct PlaneMasks { uint32_t or0, sgm, sgl, xs, tu; };
+
+__host__ __device__ inline PlaneMasks plane_masks(uint32_t pp, bool issigned, bool bipolar) {
+  if (bipolar) return {~0u, 0u, ~0u, ~0u, 0u};
+  if (pp % 2) return {0u, 0u, issigned ? ~0u : 0u, 0u, 0u};
+  return {0u, issigned ? ~0u : 0u, 0u, 0u, issigned ? 0u : ~0u};
+}
+
+template <int D>
+__device__ __forceinline__ uint32_t gen_reg(uint32_t m0, uint32_t t, uint32_t s) {
+  return nib<D>(m0) | (nib<D>(t) << 1) | (nib<D>(s) << 3);
+}
+
+// One output tile of a product: plane-side rows from pt * BPR, code-side
+// rows from vt * BV, full super-steps jb .. je - 1 and, with `with_tail`,
+// the partly filled last one (jb = 0, je = kw / 16 and the tail for the
+// whole of K); `add` adds the tile's sums into res with integer atomics
+// (K pieces of one product) instead of storing them.
+//
 // PSIGNED with PP == 1: bipolar.  NW waves per block.  CC: chunks of the
 // code side (1: whole values as FP4 codes; 2: two 2-bit chunks, each an
 // FP4 code under its own block scale, CP x 2 instructions per k-step).
-template <bool PLANE_RHS, int PP, bool PSIGNED, int NW, int WPG, int VG, int CC = 1>
-__global__ __launch_bounds__(NW * 64, 8 / NW) void planes_kernel(PlanesProb p) {
+// GEN: the plane side's encoding comes from `pm` and `pp_rt` (PP is then
+// 2 CP plane slots; slots at or past pp_rt read zeros, through a buffer
+// resource of no records, which reads no memory).
+template <bool PLANE_RHS, int PP, bool PSIGNED, int NW, int WPG, int VG, int CC, bool GEN>
+__device__ __forceinline__ void planes_tile(const PlanesProb &p, const PlaneMasks &pm,
+                                            uint32_t pp_rt, uint32_t pt, uint32_t vt,
+                                            uint32_t jb, uint32_t je, bool with_tail, bool add,
+                                            uint8_t *lds) {
   constexpr int NT = NW * 64;                  // threads
   constexpr int CP = (PP + 1) / 2;             // chunks of the plane side
   constexpr int BPR = NW * WPG * 16;           // plane-side rows of the block
@@ -174,14 +217,9 @@ __global__ __launch_bounds__(NW * 64, 8 / NW) void planes_kernel(PlanesProb p) {
   constexpr int TASKS = (BV * 4 + NT - 1) / NT;  // code-side (row, group) tasks per thread
   constexpr bool ALL_TASKS = TASKS * NT == BV * 4;
   constexpr int BUF = 4 * BV * 64;             // bytes of one LDS buffer: 4 k-steps x BV rows
-  constexpr bool BIP = PSIGNED && PP == 1;
-  __shared__ __attribute__((aligned(16))) uint8_t lds[2 * CC * BUF];  // [half][chunk]
+  constexpr bool BIP = !GEN && PSIGNED && PP == 1;
+  static_assert(!GEN || PP % 2 == 0, "run-time plane sides take whole chunks of slots");
 
-  const uint32_t T = gridDim.x, b = blockIdx.x;
-  // consecutive logical tiles (the code-side tiles of one plane-side tile)
-  // run on one XCD (blocks are dealt round-robin over the 8 XCDs)
-  const uint32_t t = (T % 8 == 0) ? (b % 8) * (T / 8) + b / 8 : b;
-  const uint32_t vt = t % p.vtiles, pt = t / p.vtiles;
   const uint32_t v0 = vt * BV, p0 = pt * BPR;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // uniform
@@ -195,6 +233,14 @@ __global__ __launch_bounds__(NW * 64, 8 / NW) void planes_kernel(PlanesProb p) {
       __builtin_amdgcn_make_buffer_rsrc((void *)p.P, (short)0, (int)p.p_bytes, 0x00020000);
   const __amdgpu_buffer_rsrc_t rv =
       __builtin_amdgcn_make_buffer_rsrc((void *)p.V, (short)0, (int)p.v_bytes, 0x00020000);
+  // plane slots of a run-time plane side: absent planes through a resource
+  // of no records (every load returns zero)
+  __amdgpu_buffer_rsrc_t rps[PP];
+#pragma unroll
+  for (int q = 0; q < PP; q++)
+    rps[q] = !GEN || (uint32_t)q < pp_rt
+                 ? rp
+                 : __builtin_amdgcn_make_buffer_rsrc((void *)p.P, (short)0, 0, 0x00020000);
   // plane-side rows of this lane (rows past the operand read its last row,
   // in bounds; their outputs are never stored): byte offset of the row
   uint32_t prow[WPG];
@@ -271,7 +317,7 @@ __global__ __launch_bounds__(NW * 64, 8 / NW) void planes_kernel(PlanesProb p) {
     for (int rg = 0; rg < WPG; rg++)
 #pragma unroll
       for (int q = 0; q < PP; q++)
-        xp[rg][q][H] = ld128(rp, prow[rg] + lo, (uint32_t)q * (uint32_t)p.p_wpp * 8u + so);
+        xp[rg][q][H] = ld128(rps[q], prow[rg] + lo, (uint32_t)q * (uint32_t)p.p_wpp * 8u + so);
   };
   // (code-side planes past the operand's repeat its last plane, so every
   // load is unconditional and no load sits in a branch)
@@ -339,7 +385,12 @@ __global__ __launch_bounds__(NW * 64, 8 / NW) void planes_kernel(PlanesProb p) {
         const uint32_t xl = dword_of(xp[rg][lo][H], c);
         const uint32_t xh = hi < PP ? dword_of(xp[rg][hi < PP ? hi : lo][H], c) : 0u;
         const bool top = ch == CP - 1;
-        if (BIP) pf[rg][ch] = chunk_frag<4>(xl, 0u);
+        if (GEN && top) {
+          const uint32_t m0 = xl | pm.or0, s = ((xh & pm.sgm) | (xl & pm.sgl)) ^ pm.xs;
+          const uint32_t t = xh & (pm.tu | ~xl);
+          pf[rg][ch] = v8i{(int)gen_reg<0>(m0, t, s), (int)gen_reg<1>(m0, t, s),
+                           (int)gen_reg<2>(m0, t, s), (int)gen_reg<3>(m0, t, s), 0, 0, 0, 0};
+        } else if (BIP) pf[rg][ch] = chunk_frag<4>(xl, 0u);
         else if (hi < PP)
           pf[rg][ch] = (top && PSIGNED) ? chunk_frag<3>(xl, xh) : chunk_frag<2>(xl, xh);
         else
@@ -375,11 +426,11 @@ __global__ __launch_bounds__(NW * 64, 8 / NW) void planes_kernel(PlanesProb p) {
   // Stage (j, H) multiplies plane registers [H] with LDS buffer H.
   // Prologue: plane words of both halves of iteration 0, codes of its half
   // 0 in buffer 0, code words of its half 1 in registers.
-  load_plane(0, H0{});
-  load_plane(0, H1{});
-  load_code(0, H0{});
-  store_codes(H0{}, 0);
-  load_code(0, H1{});
+  load_plane(jb, H0{});
+  load_plane(jb, H1{});
+  load_code(jb, H0{});
+  store_codes(H0{}, jb);
+  load_code(jb, H1{});
   __syncthreads();
   // Full super-steps.  Within a half, the codes of the next half go to the
   // other buffer (its last readers passed the barrier ending the previous
@@ -389,7 +440,7 @@ __global__ __launch_bounds__(NW * 64, 8 / NW) void planes_kernel(PlanesProb p) {
   // registers the words of the half after next.  The last iteration's
   // prefetch reads the tail (same form) or, without one, words no one uses
   // (in-bounds rows, or the buffer's zeros past the matrix).
-  for (uint32_t j = 0; j < nfull; j++) {
+  for (uint32_t j = jb; j < je; j++) {
 #pragma unroll
     for (int c = 0; c < 4; c++) {
       kstep(H0{}, c);
@@ -410,7 +461,7 @@ __global__ __launch_bounds__(NW * 64, 8 / NW) void planes_kernel(PlanesProb p) {
     __syncthreads();
   }
   // the partly filled last super-step: only its k-steps with data
-  if (tail_dw) {
+  if (with_tail && tail_dw) {
     const uint32_t nk0 = min(tail_dw, 4u);
 #pragma unroll
     for (int c = 0; c < 4; c++)
@@ -434,17 +485,33 @@ __global__ __launch_bounds__(NW * 64, 8 / NW) void planes_kernel(PlanesProb p) {
         const uint32_t prow_i = p0 + (uint32_t)((wave * WPG + rg) * 16);
         const uint32_t n = PLANE_RHS ? prow_i + 4 * g + r : v0 + 16 * v + 4 * g + r;
         const uint32_t m = PLANE_RHS ? v0 + 16 * v + fr : prow_i + fr;
-        if (n < p.N && m < p.M) p.res[(uint64_t)n * p.M + m] = (int32_t)acc[rg][v][r];
+        if (n < p.N && m < p.M) {
+          int32_t *o = p.res + (uint64_t)n * p.M + m;
+          if (add) atomicAdd(o, (int32_t)acc[rg][v][r]);
+          else *o = (int32_t)acc[rg][v][r];
+        }
       }
+}
+
+template <bool PLANE_RHS, int PP, bool PSIGNED, int NW, int WPG, int VG, int CC = 1>
+__global__ __launch_bounds__(NW * 64, 8 / NW) void planes_kernel(PlanesProb p) {
+  __shared__ __attribute__((aligned(16))) uint8_t lds[2 * CC * 4 * 16 * VG * 64];  // [half][chunk]
+  const uint32_t T = gridDim.x, b = blockIdx.x;
+  // consecutive logical tiles (the code-side tiles of one plane-side tile)
+  // run on one XCD (blocks are dealt round-robin over the 8 XCDs)
+  const uint32_t t = (T % 8 == 0) ? (b % 8) * (T / 8) + b / 8 : b;
+  planes_tile<PLANE_RHS, PP, PSIGNED, NW, WPG, VG, CC, false>(
+      p, PlaneMasks{}, PP, t / p.vtiles, t % p.vtiles, 0, p.kw / 16, true, false, lds);
 }
 
 // Tile configurations: waves per block, plane-side row groups per wave,
 // code-side row groups per block.  Two-chunk code sides take tile 1 only
-// (its LDS holds both chunks; the 8-wave forms spill registers).
+// (its LDS holds both chunks; the 8-wave forms spill registers).  An
+// 8-wave block of 256 code-side rows spills at every plane count and is
+// not carried.
 struct PlanesTile { int NW, WPG, VG; };
-constexpr int kPlanesTiles = 5;
-constexpr PlanesTile kPlanesTile[kPlanesTiles] = {
-    {4, 2, 8}, {4, 2, 4}, {4, 2, 2}, {8, 2, 8}, {8, 2, 16}};
+constexpr int kPlanesTiles = 4;
+constexpr PlanesTile kPlanesTile[kPlanesTiles] = {{4, 2, 8}, {4, 2, 4}, {4, 2, 2}, {8, 2, 8}};
 
 template <bool PR, int PP, bool PS>
 void run_pp2(const PlanesProb &p, uint32_t grid, hipStream_t s) {
@@ -463,11 +530,8 @@ void run_pp(int tile, const PlanesProb &p, uint32_t grid, hipStream_t s) {
     case 2:
       hipLaunchKernelGGL((planes_kernel<PR, PP, PS, 4, 2, 2>), dim3(grid), dim3(256), 0, s, p);
       break;
-    case 3:
-      hipLaunchKernelGGL((planes_kernel<PR, PP, PS, 8, 2, 8>), dim3(grid), dim3(512), 0, s, p);
-      break;
     default:
-      hipLaunchKernelGGL((planes_kernel<PR, PP, PS, 8, 2, 16>), dim3(grid), dim3(512), 0, s, p);
+      hipLaunchKernelGGL((planes_kernel<PR, PP, PS, 8, 2, 8>), dim3(grid), dim3(512), 0, s, p);
       break;
   }
 }
@@ -578,6 +642,64 @@ PlanesPlan planes_plan(const Operand &l, const Operand &r, uint64_t K) {
   return pl;
 }
 
+// One product of the batched launch: lhs = plane side (its encoding in
+// masks), rhs = code side; work item i of the product (i from tile_begin)
+// is code-side tile i % vtiles, plane-side tile (i / vtiles) % ptiles, K
+// piece i / (vtiles ptiles) of jper full super-steps (the last piece also
+// takes the partly filled super-step).  variant: (CP - 1) + 2 (CC - 1)
+// + 4 (VG == 4).
+struct PlanesBatchProb {
+  PlanesProb p;
+  PlaneMasks pm;
+  uint32_t pp, ptiles, splits, jper, variant;
+  uint64_t tile_begin;
+};
+constexpr int kPlanesBatchLds = 2 * 2 * 4 * 64 * 64;  // CC 2, VG 4: 64 KiB
+
+template <int CP, int CC, int VG>
+__device__ __forceinline__ void planes_batch_item(const PlanesBatchProb &d, uint32_t pt,
+                                                  uint32_t vt, uint32_t jb, uint32_t je,
+                                                  bool last, bool add, uint8_t *lds) {
+  planes_tile<false, 2 * CP, false, 4, 2, VG, CC, true>(d.p, d.pm, d.pp, pt, vt, jb, je, last,
+                                                        add, lds);
+}
+
+// Persistent launch over the products' work items: a block takes the next
+// item from one queue counter, finds its product in the per-item table and
+// runs that product's tile form (a uniform branch over the eight forms);
+// every block leaves once the counter passes the last item.
+__global__ __launch_bounds__(256, 2) void planes_batch_kernel(const PlanesBatchProb *probs,
+                                                              uint32_t total,
+                                                              unsigned int *counter,
+                                                              const uint32_t *item_prod) {
+  __shared__ __attribute__((aligned(16))) uint8_t lds[kPlanesBatchLds];
+  __shared__ uint32_t s_item;
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(counter, 1u);
+    __syncthreads();
+    const uint32_t item = __builtin_amdgcn_readfirstlane(s_item);
+    __syncthreads();
+    if (item >= total) break;
+    const PlanesBatchProb &d = probs[item_prod[item]];
+    const uint32_t local = item - (uint32_t)d.tile_begin;
+    const uint32_t vt = local % d.p.vtiles, rest = local / d.p.vtiles;
+    const uint32_t pt = rest % d.ptiles, sp = rest / d.ptiles;
+    const uint32_t nfull = d.p.kw / 16, jb = sp * d.jper, je = min(nfull, jb + d.jper);
+    const bool last = sp + 1 == d.splits, add = d.splits > 1;
+    switch (d.variant) {
+      case 0: planes_batch_item<1, 1, 2>(d, pt, vt, jb, je, last, add, lds); break;
+      case 1: planes_batch_item<2, 1, 2>(d, pt, vt, jb, je, last, add, lds); break;
+      case 2: planes_batch_item<1, 2, 2>(d, pt, vt, jb, je, last, add, lds); break;
+      case 3: planes_batch_item<2, 2, 2>(d, pt, vt, jb, je, last, add, lds); break;
+      case 4: planes_batch_item<1, 1, 4>(d, pt, vt, jb, je, last, add, lds); break;
+      case 5: planes_batch_item<2, 1, 4>(d, pt, vt, jb, je, last, add, lds); break;
+      case 6: planes_batch_item<1, 2, 4>(d, pt, vt, jb, je, last, add, lds); break;
+      default: planes_batch_item<2, 2, 4>(d, pt, vt, jb, je, last, add, lds); break;
+    }
+    __syncthreads();
+  }
+}
+
 }  // namespace
 
 bool planes_supported(const Operand &l, const Operand &r, uint64_t K) {
@@ -626,6 +748,83 @@ int launch_planes_gemm(const Operand &l, const Operand &r, uint64_t K, int32_t *
   const bool psigned = ps.issigned;  // with one plane: bipolar
   if (pl.plane_rhs) run_planes<true>(pl.pp, psigned, pl.cc, pl.tile, p, (uint32_t)tiles, s);
   else run_planes<false>(pl.pp, psigned, pl.cc, pl.tile, p, (uint32_t)tiles, s);
+  GBS_HIP(hipGetLastError());
+  return GBS_OK;
+}
+
+PlanesBatchPlan planes_batch_plan(const Operand &l, const Operand &r, uint64_t K,
+                                  uint32_t min_pieces) {
+  PlanesBatchPlan pl{};
+  pl.ok = false;
+  if (K == 0 || l.nrows == 0 || r.nrows == 0 || l.nbits > 4) return pl;
+  if (l.bipolar && r.bipolar) return pl;  // padding bits would pair -1 with -1
+  if (l.nrows > 0x7fffffffull || r.nrows > 0x7fffffffull) return pl;
+  if ((uint64_t)l.nbits * l.wpp * 8 >= (1ull << 31) || (uint64_t)r.nbits * r.wpp * 8 >= (1ull << 31))
+    return pl;
+  int enc = code_enc(r), cc = 1;
+  if (enc < 0) {
+    enc = code_enc2(r);
+    cc = 2;
+  }
+  if (enc < 0) return pl;
+  if ((double)K * plane_max(l) * code_max(enc) >= 16777216.0) return pl;
+  // 32 code-side rows per tile for narrow rhs operands, else 64
+  const uint32_t vg = r.nrows <= 32 ? 2 : 4;
+  const uint32_t cp = (l.nbits + 1) / 2;
+  pl.variant = (cp - 1) + 2 * (uint32_t)(cc - 1) + (vg == 4 ? 4u : 0u);
+  pl.vtiles = (uint32_t)((r.nrows + 16 * vg - 1) / (16 * vg));
+  pl.ptiles = (uint32_t)((l.nrows + 127) / 128);
+  // K pieces of at most 8 super-steps (8192 values, as the copy formats'
+  // 4 KiB pieces of FP4 codes), more when asked (at least 2 super-steps)
+  const uint32_t nfull = (uint32_t)((K + 63) / 64 / 16);
+  uint32_t want = nfull ? (nfull + 7) / 8 : 1;
+  const uint32_t most = nfull / 2 ? nfull / 2 : 1;
+  if (want < min_pieces) want = min_pieces < most ? min_pieces : most;
+  pl.jper = nfull ? (nfull + want - 1) / want : 0;
+  pl.splits = nfull ? (nfull + pl.jper - 1) / pl.jper : 1;
+  const double kvals = nfull ? (double)pl.jper * 1024 : (double)K;
+  const uint32_t code_planes = cc == 2 ? 4 : 3;  // loads per row, repeats included
+  pl.item_bytes = (128.0 * l.nbits + 16.0 * vg * code_planes) * kvals / 8;
+  pl.ok = true;
+  return pl;
+}
+
+size_t planes_batch_desc_size() { return sizeof(PlanesBatchProb); }
+
+void planes_batch_desc(const Operand &l, const Operand &r, uint64_t K, int32_t *d_res,
+                       const PlanesBatchPlan &pl, uint64_t tile_begin, void *out) {
+  PlanesBatchProb d{};
+  PlanesProb &p = d.p;
+  p.P = l.planes;
+  p.V = r.planes;
+  p.p_wpr = l.wpr;
+  p.v_wpr = r.wpr;
+  p.p_wpp = l.wpp;
+  p.v_wpp = r.wpp;
+  p.p_bytes = (uint32_t)((uint64_t)l.nbits * l.wpp * 8);
+  p.v_bytes = (uint32_t)((uint64_t)r.nbits * r.wpp * 8);
+  p.res = d_res;
+  p.M = p.NP = (uint32_t)l.nrows;
+  p.N = p.NV = (uint32_t)r.nrows;
+  p.kw = (uint32_t)((K + 63) / 64);
+  p.nss = (p.kw + 15) / 16;
+  p.vtiles = pl.vtiles;
+  const int enc1 = code_enc(r);
+  p.v_enc = enc1 >= 0 ? enc1 : code_enc2(r);
+  d.pm = plane_masks(l.nbits, l.issigned, l.bipolar);
+  d.pp = l.nbits;
+  d.ptiles = pl.ptiles;
+  d.splits = pl.splits;
+  d.jper = pl.jper;
+  d.variant = pl.variant;
+  d.tile_begin = tile_begin;
+  memcpy(out, &d, sizeof d);
+}
+
+int launch_planes_batch(const void *d_desc, uint32_t total, unsigned int *counter,
+                        const uint32_t *item_prod, uint32_t grid, hipStream_t s) {
+  hipLaunchKernelGGL(planes_batch_kernel, dim3(grid), dim3(256), 0, s,
+                     (const PlanesBatchProb *)d_desc, total, counter, item_prod);
   GBS_HIP(hipGetLastError());
   return GBS_OK;
 }

@@ -308,17 +308,20 @@ typedef struct gbs_problem {
 } gbs_problem;
 
 /* Plan a batch (validates shapes, builds the tile list and the device
- * descriptors, makes every operand's value copy current in the narrowest
- * format both operands of its product fit: FP4, FP6 or int8).  The
- * matrices and result buffers must outlive the batch.  Operands may be
- * re-imported between runs (same shapes): each run first rebuilds the
- * copies of operands changed since, on the run's stream, outside the
- * graph of gbs_batch_run_graph. */
+ * descriptors).  Products the planes group takes (gbs_batch_set_planes;
+ * it holds an lhs of <= 4 planes with an rhs of <= 4 planes or bipolar,
+ * not both bipolar, every partial sum below 2^24) are read straight from
+ * their bit-planes; the others get their operands' value copies made current in the narrowest
+ * format both operands fit: FP4, FP6 or int8.  The matrices and result
+ * buffers must outlive the batch.  Operands may be re-imported between
+ * runs (same shapes): each run first rebuilds the copies of operands
+ * changed since, on the run's stream, outside the graph of
+ * gbs_batch_run_graph. */
 int gbs_batch_create(const gbs_problem *problems, uint64_t count, gbs_batch **out);
 int gbs_batch_destroy(gbs_batch *b);
-/* Enqueue the whole batch: one persistent launch per copy format present
- * (plus separate popcount launches for products the matrix-core engine
- * cannot hold exactly). */
+/* Enqueue the whole batch: one persistent launch for the planes group and
+ * one per copy format present (plus separate popcount launches for
+ * products the matrix-core engine cannot hold exactly). */
 int gbs_batch_run(gbs_batch *b, void *stream);
 /* Same, captured into a hipGraph on the first call and replayed after. */
 int gbs_batch_run_graph(gbs_batch *b, void *stream);
@@ -326,11 +329,23 @@ int gbs_batch_run_graph(gbs_batch *b, void *stream);
 int gbs_batch_run_per_call(gbs_batch *b, void *stream);
 int gbs_batch_info(const gbs_batch *b, uint64_t *tiles, uint64_t *persistent,
                    uint64_t *separate);
-/* The plan a batch made for problem i: copy format (0 int8, 1 FP4, 2 FP6;
- * -1 for a product run by the popcount engine), tile shape, K splits and K
- * piece (units of 256 nominal bytes). */
+/* The plan a batch made for problem i: group (0 int8, 1 FP4, 2 FP6 copies,
+ * 3 planes; -1 for a product run by the popcount engine), tile shape (for
+ * the planes group its tile form: plane-side chunks - 1 + 2 x (code-side
+ * chunks - 1) + 4 when the tile holds 64 rhs rows, else 32), K splits and
+ * K piece (units of 256 nominal bytes; planes group: super-steps of 1024
+ * values). */
 int gbs_batch_plan_of(const gbs_batch *b, uint64_t problem, int *fmt, int *shape,
                       uint32_t *splits, uint32_t *piece_units);
+/* Which products batches planned from now on put into the planes group:
+ * 0 (the default) none, every product from value copies (faster on the
+ * measured mixed workloads, also with operands re-imported before every
+ * run); 1 those it holds with at most 16 rhs rows whose operands do not
+ * both fit FP4 codes (the class where it is faster than FP6 copies); 2
+ * every product it holds (no value copies are made for them: less HBM).
+ * No reference counterpart. */
+int gbs_batch_set_planes(int mode);
+int gbs_batch_get_planes(int *mode);
 
 /* Batch tile table: the persistent launch's tile shape and largest K piece
  * per batch class (rhs-row bucket x depth bucket) and copy format, measured
