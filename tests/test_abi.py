@@ -102,6 +102,24 @@ def test_matrix_load_rejects_bad_files_before_touching_the_device(tmp_path):
     assert L.gbs_matrix_load(b"/nonexistent/x.gbsp", C.byref(out)) == 1
 
 
+def test_batch_planes_mode_without_device():
+    """Which products batches put into the planes group: copies by default,
+    modes 0-2 set and read back, anything else refused with the mode named."""
+    import gemmbitserial_amd as g
+    L = g.lib()
+    assert g.batch_get_planes() == g.PLANES_NONE
+    try:
+        for m in (g.PLANES_SKINNY, g.PLANES_ALL, g.PLANES_NONE):
+            g.batch_set_planes(m)
+            assert g.batch_get_planes() == m
+        assert L.gbs_batch_set_planes(3) == 1 and b"mode 3" in L.gbs_last_error()
+        assert L.gbs_batch_set_planes(-1) == 1
+        assert g.batch_get_planes() == g.PLANES_NONE
+        assert L.gbs_batch_get_planes(None) == 1
+    finally:
+        g.batch_set_planes(g.PLANES_NONE)
+
+
 def test_batch_tune_table_validation_without_device():
     """The persistent launch's tile table: entries are checked against the
     shapes the persistent kernel carries per format, and the committed table
