@@ -383,3 +383,29 @@ def test_batch_refreshes_many_copies_in_one_launch(gbs, orc):
         b.close()
     finally:
         gbs.batch_set_planes(gbs.PLANES_NONE)
+
+
+def test_batch_edge_shapes(gbs, orc, planes):
+    """One-row operands, depth 1, depths one short of and one past a
+    super-step, and last tiles that hold a single row (129 lhs rows, 65 and
+    33 rhs rows), under every planes-group mode: each result equals the
+    exact product."""
+    rng = np.random.default_rng(59)
+    shapes = [(1, 1, 1, 2, False, 3, False), (1, 1023, 5, 4, True, 2, True),
+              (129, 1025, 65, 3, False, 4, False), (129, 64, 33, 1, True, 3, True),
+              (17, 4096, 1, 2, True, 1, True), (300, 2048, 2, 4, False, 1, False)]
+    probs = [dict(M=M, K=K, N=N, lb=lb, rb=rb, ls=ls, rs=rs,
+                  lhs=W.values(rng, (M, K), lb, ls), rhs=W.values(rng, (N, K), rb, rs))
+             for (M, K, N, lb, ls, rb, rs) in shapes]
+    items = build(gbs, probs)
+    b = gbs.Batch([(l, r, d.ptr) for l, r, d in items])
+    if planes == gbs.PLANES_ALL:
+        assert all(b.plan_of(k)[0] == gbs.BATCH_PLANES for k in range(len(probs)))
+    for run in (b.run, b.run_graph):
+        zero(items)
+        gbs.sync()
+        run()
+        gbs.sync()
+        for g, p in zip(results(items, probs), probs):
+            assert np.array_equal(g, exact(p)), (run.__name__, p["M"], p["K"], p["N"])
+    b.close()
