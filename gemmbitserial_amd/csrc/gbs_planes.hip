@@ -25,11 +25,15 @@
 //               into LDS, and every wave reads them from there.
 // Both use the block scale of v_mfma_scale_f32_16x16x128_f8f6f4 for the
 // chunk weight: FP4 code c (c = 0..4 -> 0, 0.5, 1, 1.5, 2) under scale
-// 2^(1 + p) is the integer c * 2^p, so chunk c of the plane side (planes
-// 2c, 2c + 1) runs under scale 2^(1 + 2c) and the code side under 2^1.
-// Every product and every fp32 partial sum is then an integer of magnitude
-// below K * maxP * maxV, checked against 2^24 on the host, so the sums are
-// exact whatever order the hardware adds in.
+// 2^p is c * 2^(p - 1), so chunk c of the plane side (planes 2c, 2c + 1)
+// runs under scale 2^(2c) and a code-side chunk cc under 2^(2 cc): every
+// product is a quarter of its integer, and the epilogue multiplies the
+// sums by 4.  Chunk 0 x chunk 0 (every instruction of a product whose two
+// sides are single chunks) thus runs under 2^0 on both sides, i.e. the
+// instruction without scales, which issues about 8 % faster.  Every fp32
+// partial sum is a quarter of an integer of magnitude below K * maxP *
+// maxV, checked against 2^24 on the host, so the sums are exact whatever
+// order the hardware adds in.
 //
 // K order: bit 4j + d of 32-bit word w of a row becomes nibble j of
 // register d of the fragment that covers word w, on both sides (any
@@ -404,18 +408,29 @@ __device__ __forceinline__ void planes_tile(const PlanesProb &p, const PlaneMask
         const uint4 cw = *reinterpret_cast<const uint4 *>(
             lds + (H * CC + cc) * BUF + (((c * VG + v) * 4 + g) * 16 + fr) * 16);
         const v8i cf = {(int)cw.x, (int)cw.y, (int)cw.z, (int)cw.w, 0, 0, 0, 0};
-        const int cs = kF4Scale + 2 * cc;  // code chunk cc: 2^(1 + 2 cc)
+        const int cs = 127 + 2 * cc;  // code chunk cc: 2^(2 cc)
 #pragma unroll
         for (int rg = 0; rg < WPG; rg++)
 #pragma unroll
           for (int ch = 0; ch < CP; ch++) {
-            const int ps = kF4Scale + 2 * ch;  // 2^(1 + 2 ch)
-            if constexpr (PLANE_RHS)
+            const int ps = 127 + 2 * ch;  // 2^(2 ch)
+            // both scales 2^0 (chunk 0 x chunk 0): the instruction without
+            // scales (the builtin with zero scale operands), which issues
+            // faster and multiplies the same
+            if (ps == 127 && cs == 127) {
+              if constexpr (PLANE_RHS)
+                acc[rg][v] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
+                    pf[rg][ch], cf, acc[rg][v], 4, 4, 0, 0, 0, 0);
+              else
+                acc[rg][v] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
+                    cf, pf[rg][ch], acc[rg][v], 4, 4, 0, 0, 0, 0);
+            } else if constexpr (PLANE_RHS) {
               acc[rg][v] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
                   pf[rg][ch], cf, acc[rg][v], 4, 4, 0, ps, 0, cs);
-            else
+            } else {
               acc[rg][v] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(
                   cf, pf[rg][ch], acc[rg][v], 4, 4, 0, cs, 0, ps);
+            }
           }
       }
     }
@@ -487,8 +502,9 @@ __device__ __forceinline__ void planes_tile(const PlanesProb &p, const PlaneMask
         const uint32_t m = PLANE_RHS ? v0 + 16 * v + fr : prow_i + fr;
         if (n < p.N && m < p.M) {
           int32_t *o = p.res + (uint64_t)n * p.M + m;
-          if (add) atomicAdd(o, (int32_t)acc[rg][v][r]);
-          else *o = (int32_t)acc[rg][v][r];
+          const int32_t val = (int32_t)(acc[rg][v][r] * 4.0f);  // quarters (see kstep)
+          if (add) atomicAdd(o, val);
+          else *o = val;
         }
       }
 }
