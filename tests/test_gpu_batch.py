@@ -409,3 +409,36 @@ def test_batch_edge_shapes(gbs, orc, planes):
         for g, p in zip(results(items, probs), probs):
             assert np.array_equal(g, exact(p)), (run.__name__, p["M"], p["K"], p["N"])
     b.close()
+
+
+@pytest.mark.parametrize("seed", [61, 62, 63])
+def test_batch_planes_group_random_sweep(gbs, orc, seed):
+    """Seeded random batches of 1..4-bit products (random signedness, rows
+    1..400, depths 1..6000, rhs rows 1..120) with every product the planes
+    group holds in it: plain and graph-replayed runs equal the exact
+    products."""
+    rng = np.random.default_rng(seed)
+    probs = []
+    for _ in range(30):
+        M, N = int(rng.integers(1, 400)), int(rng.integers(1, 120))
+        K = int(rng.choice([int(rng.integers(1, 1100)), int(rng.integers(1000, 6000))]))
+        lb, rb = int(rng.integers(1, 5)), int(rng.integers(1, 5))
+        ls, rs = bool(rng.integers(0, 2)), bool(rng.integers(0, 2))
+        probs.append(dict(M=M, K=K, N=N, lb=lb, rb=rb, ls=ls, rs=rs,
+                          lhs=W.values(rng, (M, K), lb, ls), rhs=W.values(rng, (N, K), rb, rs)))
+    items = build(gbs, probs)
+    gbs.batch_set_planes(gbs.PLANES_ALL)
+    try:
+        b = gbs.Batch([(l, r, d.ptr) for l, r, d in items])
+    finally:
+        gbs.batch_set_planes(gbs.PLANES_NONE)
+    want = [exact(p) for p in probs]
+    for run in (b.run, b.run_graph, b.run_graph):
+        zero(items)
+        gbs.sync()
+        run()
+        gbs.sync()
+        for g, w, p in zip(results(items, probs), want, probs):
+            assert np.array_equal(g, w), (seed, p["M"], p["K"], p["N"], p["lb"], p["ls"],
+                                          p["rb"], p["rs"])
+    b.close()
